@@ -1,0 +1,144 @@
+/* C ABI of the MI355X-native metric-dependent CCH path (libnwcch.so).
+ *
+ * Plain pointers and sizes only.  Every function returns 0 (NWCCH_OK) or a
+ * negative error code; nwcch_last_error() gives the message of the calling
+ * thread's last failure.  No C++ exception crosses this boundary.
+ *
+ * Conventions
+ *   - Vertices of the hierarchy are ranks (0 = lowest), as in python/cch_nb.py.
+ *   - Weights are int32.  NWCCH_INF = 2^31-1 means "no arc / unreachable" and
+ *     absorbs additions; INT32_MIN is not a valid weight.  Finite sums outside
+ *     (-INF, INF) set NWCCH_STATUS_OVERFLOW instead of saturating silently.
+ *   - Customized shortcut weights of one metric are one int32 [mp][2] array
+ *     `ud`: ud[2a] = up (lower -> higher rank), ud[2a+1] = down.  The reference
+ *     keeps them as two arrays (upw, dnw); a drop-in exposes them as two strided
+ *     views of `ud`.
+ *   - Device pointers (`dev` functions) are HIP device memory; `stream` is a
+ *     hipStream_t (NULL = default stream).  Calls are stream-ordered: they
+ *     enqueue work and return; results are valid after the stream reaches them.
+ *     Calls on one handle from several host threads must use distinct streams
+ *     and distinct output buffers.
+ */
+#ifndef NWCCH_H_
+#define NWCCH_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NWCCH_OK 0
+#define NWCCH_ERR_ARG (-1)      /* bad argument: null, size, shape */
+#define NWCCH_ERR_ORDER (-2)    /* order is not a permutation (python/cch.py:125) */
+#define NWCCH_ERR_CHORDAL (-3)  /* an arc (y, z) missing: not a chordal completion (python/cch.py:218) */
+#define NWCCH_ERR_RUNTIME (-4)  /* HIP or internal failure; see nwcch_last_error() */
+#define NWCCH_ERR_CAPACITY (-5) /* a size beyond what the library supports */
+
+#define NWCCH_INF 2147483647
+
+#define NWCCH_STATUS_OVERFLOW 1u
+#define NWCCH_STATUS_BAD_WEIGHT 2u
+#define NWCCH_STATUS_BAD_QUERY 4u
+#define NWCCH_STATUS_QUEUE_FULL 8u
+
+typedef struct nwcch_topo nwcch_topo; /* host topology (metric independent) */
+typedef struct nwcch_dev nwcch_dev;   /* device copy of one topology */
+
+/* Message of the calling thread's last failure ("" if none). */
+const char* nwcch_last_error(void);
+
+/* ------------------------------------------------------------ host side ----- */
+
+/* Nested-dissection order with inertial-flow bisection on the undirected graph
+ * underlying the directed CSR (off[n+1], tgt[m]); rank_out[v] = rank of v.
+ * Replaces cch_nb.nested_dissection (python/cch_nb.py:336-426) and
+ * cch.nested_dissection_order (python/cch.py:42-106); cpp/cch.cpp:323-363. */
+int nwcch_order(int64_t n, const int64_t* off, const int32_t* tgt, const double* x,
+                const double* y, int32_t leaf, int32_t* rank_out);
+
+/* Symbolic contraction + arc map + triangle index + level schedule from a graph
+ * and an order.  Replaces CCH.__init__ (python/cch.py:112-208) and
+ * cch_nb.symbolic + edge_arcs (python/cch_nb.py:431-515); cpp/cch.cpp:377-435. */
+int nwcch_topo_create(int64_t n, const int64_t* off, const int32_t* tgt, const int32_t* rank,
+                      nwcch_topo** out);
+
+/* The same from an existing upward CSR (first[n+1], head[mp]) and arc map
+ * (e_arc[m] with -1 for loops, e_up[m]), i.e. the arrays cch_nb.customize takes
+ * (python/cch_nb.py:530).  e_arc may be NULL (m = 0): no original arcs. */
+int nwcch_topo_from_csr(int64_t n, const int64_t* first, const int32_t* head, int64_t m,
+                        const int64_t* e_arc, const int8_t* e_up, nwcch_topo** out);
+
+void nwcch_topo_destroy(nwcch_topo* t);
+
+/* out8 = {n, m, mp, triangles, depth, levels, loop_edges, mapped_edges}
+ * (the structure line of cpp/cch.cpp:779-781 and python/cch.py:206-208). */
+int nwcch_topo_info(const nwcch_topo* t, int64_t* out8);
+
+/* Copies of the host arrays; any pointer may be NULL to skip it.
+ * first[n+1] (python/cch.py:157), head[mp], parent[n] (-1 = root), e_arc[m],
+ * e_up[m], level[n] (ET height, cpp/cch.cpp:468-480), depth[n] (ET depth). */
+int nwcch_topo_export(const nwcch_topo* t, int64_t* first, int32_t* head, int32_t* parent,
+                      int64_t* e_arc, int8_t* e_up, int32_t* level, int32_t* depth);
+
+/* Lower triangles in pull form: for schedule position p, arc sched_arc[p] has
+ * triangles tri_off[p]..tri_off[p+1] of (a, b) pairs in tri_ab.  The same set as
+ * tri_a/tri_b/tri_c grouped by c (python/cch.py:168-186, 308). */
+int nwcch_topo_export_triangles(const nwcch_topo* t, int32_t* sched_arc, int64_t* tri_off,
+                                int32_t* tri_ab);
+
+/* ---------------------------------------------------------- device side ----- */
+
+/* Upload a topology to GPU `device`.  The handle owns its device memory. */
+int nwcch_dev_create(const nwcch_topo* t, int device, nwcch_dev** out);
+void nwcch_dev_destroy(nwcch_dev* h);
+
+/* out6 = {n, m, mp, triangles, depth, levels} */
+int nwcch_dev_info(const nwcch_dev* h, int64_t* out6);
+
+/* Full customization of one metric: w[m] original arc weights (device) ->
+ * ud[mp][2] (device).  Replaces cch_nb.customize (python/cch_nb.py:529-563),
+ * CCH.customize (python/cch.py:249-287, without the witness list) and
+ * CCH::customize / customize_parallel (cpp/cch.cpp:437-463, 496-529). */
+int nwcch_customize(nwcch_dev* h, const int32_t* w, int32_t* ud, void* stream);
+
+/* Negative-cycle scan of ud[mp][2] (any metric, no handle needed): out2
+ * (device int64[2]) = {number of arcs with up + down < 0 (INF never counts),
+ * smallest such arc or -1}.  Replaces cch_nb.negative_cycle_arcs
+ * (python/cch_nb.py:566-572), CCH::negative_cycle_arcs (cpp/cch.cpp:531-536)
+ * and the scan inside CCH.has_negative_cycle (python/cch.py:662-664). */
+int nwcch_negcycle_scan(const int32_t* ud, int64_t mp, int64_t* out2, void* stream);
+
+/* Every witness arc of the scan, in no particular order: list (device int64,
+ * capacity cap), *count (device int64) = total number (may exceed cap).  With
+ * a sort it is the list CCH.customize returns (python/cch.py:287). */
+int nwcch_negcycle_list(const int32_t* ud, int64_t mp, int64_t* list, int64_t cap, int64_t* count,
+                        void* stream);
+
+/* Elimination-tree queries between ranks s_rank[i] -> t_rank[i], i < nq, one
+ * per wavefront: dist[i] (NWCCH_INF if unreachable) and, if scanned != NULL,
+ * the arcs relaxed.  Replaces cch_nb.query (python/cch_nb.py:577-615),
+ * CCH.query (python/cch.py:675-732, which returns (dist, scanned)) and
+ * Query::run (cpp/cch.cpp:549-575). */
+int nwcch_query(nwcch_dev* h, const int32_t* ud, const int32_t* s_rank, const int32_t* t_rank,
+                int64_t nq, int32_t* dist, int32_t* scanned, void* stream);
+
+/* Read (and optionally clear) the sticky status bits of a handle.  Synchronises
+ * `stream`.  The reference raises Python exceptions / asserts instead
+ * (python/cch.py:125, 218; python/preprocess.py:9-10). */
+int nwcch_status(nwcch_dev* h, uint32_t* out, int clear, void* stream);
+
+/* Launch profiling (the reference's Timing/now() role, cpp/cch.cpp:42-45,
+ * 714-729, at kernel granularity): after nwcch_profile_begin every kernel launch
+ * of the handle is bracketed by two HIP events (up to max_launches launches);
+ * nwcch_profile_end waits for them and writes, per launch, the kernel id
+ * (0 customize level, 2 query, 3 update, 4 batched customize) and its duration
+ * in ms; *count = launches recorded. */
+int nwcch_profile_begin(nwcch_dev* h, int64_t max_launches);
+int nwcch_profile_end(nwcch_dev* h, int32_t* kernel, float* ms, int64_t cap, int64_t* count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NWCCH_H_ */

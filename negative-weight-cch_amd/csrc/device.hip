@@ -1,0 +1,554 @@
+// HIP kernels for the metric-dependent CCH path on gfx950 (MI355X), and the
+// device half of the C ABI.
+//
+// Data layout in HBM (all int32 unless noted):
+//   ud[mp]  int2 {up, dn}: the customized shortcut weights of one metric, pair per
+//           arc, so one 8-byte gather fetches both directions of an arc.
+//   tri_ab[T] int2 {a, b}: for each lower triangle (x; y, z) of arc c=(y,z) the arcs
+//           a=(x,y), b=(x,z); stored grouped by c in schedule order (level of the
+//           tail of c, then lower-triangle count descending) so a level is one
+//           contiguous slice streamed front to back.
+//   tri_off[mp+1] int64: triangle slice of each schedule position.
+//   ae_off/ae_idx: the original arcs of each shortcut, entry (edge << 1) | up.
+//   vinfo[n] {first, deg, parent, depth}, adep[mp] = ET depth of head[a].
+//
+// All arithmetic is int32 with INF = 2^31-1 absorbing additions: INF + x = INF,
+// finite sums are formed in 64 bits and a sum outside (-INF, INF) raises the
+// overflow status bit instead of saturating silently.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+
+#include "nwcch_dev.h"
+
+namespace nwcch {
+
+#define HIP_OK(expr)                                                              \
+  do {                                                                            \
+    hipError_t e_ = (expr);                                                       \
+    if (e_ != hipSuccess)                                                         \
+      throw std::runtime_error(std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+__device__ __forceinline__ int32_t inf_add(int32_t a, int32_t b, uint32_t& st) {
+  if (a == kInf || b == kInf) return kInf;
+  const long long s = (long long)a + (long long)b;
+  if (s >= (long long)kInf || s <= -(long long)kInf) {
+    st |= kStatusOverflow;
+    return s > 0 ? kInf - 1 : -kInf + 1;
+  }
+  return (int32_t)s;
+}
+
+__device__ __forceinline__ int32_t wave_min(int32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+__device__ __forceinline__ uint32_t wave_or(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v |= __shfl_xor(v, o, 64);
+  return v;
+}
+
+// F_c of one arc: min over its original arcs (by direction) and its lower triangles
+// (python/cch.py:234-247 + 279-285; python/cch_nb.py:534-562 in pull form).
+__device__ __forceinline__ void eval_arc_thread(int32_t c, i64 t0, i64 t1, const int32_t* __restrict__ w,
+                                                const int32_t* __restrict__ ae_off,
+                                                const int32_t* __restrict__ ae_idx,
+                                                const int2* __restrict__ tri_ab, const int2* ud,
+                                                int32_t& up, int32_t& dn, uint32_t& st) {
+  up = kInf;
+  dn = kInf;
+  const int32_t e0 = ae_off[c], e1 = ae_off[c + 1];
+  for (int32_t j = e0; j < e1; j++) {
+    const int32_t e = ae_idx[j];
+    const int32_t wk = w[e >> 1];
+    if (wk == (int32_t)0x80000000) st |= kStatusBadWeight;
+    if (e & 1) up = min(up, wk);
+    else dn = min(dn, wk);
+  }
+  for (i64 q = t0; q < t1; q++) {
+    const int2 ab = tri_ab[q];
+    const int2 A = ud[ab.x];
+    const int2 B = ud[ab.y];
+    up = min(up, inf_add(A.y, B.x, st));  // y -> x -> z
+    dn = min(dn, inf_add(B.y, A.x, st));  // z -> x -> y
+  }
+}
+
+// One level of customization.  Blocks [0, nbig_blocks) give one wavefront to each
+// "big" arc (lanes stride over its triangles, then a wavefront min-reduction); the
+// remaining blocks give one thread to each small arc.  Arcs of one level never
+// read each other (their inputs have tails at lower ET heights), so there are no
+// atomics and every arc is written exactly once.
+__global__ void __launch_bounds__(256) k_customize_level(
+    i64 pbeg, i64 pbig, i64 pend, int nbig_blocks, const int32_t* __restrict__ sched_arc,
+    const i64* __restrict__ tri_off, const int2* __restrict__ tri_ab,
+    const int32_t* __restrict__ ae_off, const int32_t* __restrict__ ae_idx,
+    const int32_t* __restrict__ w, int2* ud, uint32_t* status) {
+  uint32_t st = 0;
+  if ((int)blockIdx.x < nbig_blocks) {
+    const int lane = threadIdx.x & 63;
+    const i64 p = pbeg + (i64)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (p >= pbig) return;
+    const int32_t c = sched_arc[p];
+    int32_t up = kInf, dn = kInf;
+    const int32_t e0 = ae_off[c], e1 = ae_off[c + 1];
+    for (int32_t j = e0 + lane; j < e1; j += 64) {
+      const int32_t e = ae_idx[j];
+      const int32_t wk = w[e >> 1];
+      if (wk == (int32_t)0x80000000) st |= kStatusBadWeight;
+      if (e & 1) up = min(up, wk);
+      else dn = min(dn, wk);
+    }
+    const i64 t1 = tri_off[p + 1];
+    for (i64 q = tri_off[p] + lane; q < t1; q += 64) {
+      const int2 ab = tri_ab[q];
+      const int2 A = ud[ab.x];
+      const int2 B = ud[ab.y];
+      up = min(up, inf_add(A.y, B.x, st));
+      dn = min(dn, inf_add(B.y, A.x, st));
+    }
+    up = wave_min(up);
+    dn = wave_min(dn);
+    st = wave_or(st);
+    if (lane == 0) {
+      ud[c] = make_int2(up, dn);
+      if (st) atomicOr(status, st);
+    }
+    return;
+  }
+  const i64 p = pbig + (i64)(blockIdx.x - nbig_blocks) * blockDim.x + threadIdx.x;
+  if (p < pend) {
+    const int32_t c = sched_arc[p];
+    int32_t up, dn;
+    eval_arc_thread(c, tri_off[p], tri_off[p + 1], w, ae_off, ae_idx, tri_ab, ud, up, dn, st);
+    ud[c] = make_int2(up, dn);
+  }
+  if (st) atomicOr(status, st);
+}
+
+// -------------------------------------------------------- negative-cycle scan
+
+// out[0] = number of arcs with up + dn < 0 (INF never negative), out[1] = the
+// smallest such arc id or all-ones (python/cch_nb.py:566-572, cpp/cch.cpp:531-536).
+__global__ void k_scan_init(unsigned long long* out) {
+  out[0] = 0ull;
+  out[1] = ~0ull;
+}
+
+__global__ void __launch_bounds__(256) k_negcycle_scan(const int2* __restrict__ ud, i64 mp,
+                                                       unsigned long long* out) {
+  __shared__ unsigned long long s_cnt[4];
+  __shared__ unsigned long long s_first[4];
+  unsigned long long cnt = 0, first = ~0ull;
+  const i64 stride = (i64)gridDim.x * blockDim.x;
+  for (i64 a = (i64)blockIdx.x * blockDim.x + threadIdx.x; a < mp; a += stride) {
+    const int2 v = ud[a];
+    const bool neg = v.x != kInf && v.y != kInf && ((long long)v.x + (long long)v.y) < 0;
+    if (neg) {
+      cnt++;
+      if ((unsigned long long)a < first) first = (unsigned long long)a;
+    }
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    cnt += __shfl_xor(cnt, o, 64);
+    unsigned long long f2 = __shfl_xor(first, o, 64);
+    first = f2 < first ? f2 : first;
+  }
+  const int wv = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) { s_cnt[wv] = cnt; s_first[wv] = first; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long c = 0, f = ~0ull;
+    for (int i = 0; i < (int)(blockDim.x >> 6); i++) {
+      c += s_cnt[i];
+      f = s_first[i] < f ? s_first[i] : f;
+    }
+    if (c) {
+      atomicAdd(&out[0], c);
+      atomicMin(&out[1], f);
+    }
+  }
+}
+
+__global__ void k_zero_i64(long long* p) { p[0] = 0; }
+
+// Every witness arc, appended in no particular order (the caller sorts).
+__global__ void __launch_bounds__(256) k_negcycle_list(const int2* __restrict__ ud, i64 mp,
+                                                       long long* list, i64 cap,
+                                                       unsigned long long* count) {
+  const i64 stride = (i64)gridDim.x * blockDim.x;
+  for (i64 a = (i64)blockIdx.x * blockDim.x + threadIdx.x; a < mp; a += stride) {
+    const int2 v = ud[a];
+    if (v.x != kInf && v.y != kInf && ((long long)v.x + (long long)v.y) < 0) {
+      const unsigned long long i = atomicAdd(count, 1ull);
+      if ((i64)i < cap) list[i] = a;
+    }
+  }
+}
+
+__global__ void k_scan_finish(const unsigned long long* src, long long* dst) {
+  dst[0] = (long long)src[0];
+  dst[1] = src[1] == ~0ull ? -1ll : (long long)src[1];
+}
+
+// ------------------------------------------------------------ ET query
+
+// One query per wavefront (python/cch.py:675-732, python/cch_nb.py:577-615,
+// cpp/cch.cpp:549-575).  Labels live in LDS indexed by the position of a vertex on
+// its ancestor chain (position = depth(start) - depth(vertex)), so no per-vertex
+// scratch and no reset pass over n is needed; the walk over the s chain records
+// its vertices so the t walk can recognise common ancestors.
+__global__ void __launch_bounds__(64) k_query(const VInfo* __restrict__ vinfo,
+                                              const int32_t* __restrict__ adep,
+                                              const int2* __restrict__ ud,
+                                              const int32_t* __restrict__ qs,
+                                              const int32_t* __restrict__ qt, i64 nq, int32_t n,
+                                              int32_t maxlen, int32_t* __restrict__ dist,
+                                              int32_t* __restrict__ scanned,
+                                              uint32_t* status) {
+  extern __shared__ int32_t lds[];
+  int32_t* df = lds;
+  int32_t* dr = lds + maxlen;
+  int32_t* cs = lds + 2 * maxlen;
+  const i64 q = blockIdx.x;
+  const int lane = threadIdx.x;
+  if (q >= nq) return;
+  uint32_t st = 0;
+  const int32_t rs = qs[q], rt = qt[q];
+  if (rs < 0 || rs >= n || rt < 0 || rt >= n) {
+    if (lane == 0) {
+      dist[q] = kInf;
+      if (scanned) scanned[q] = 0;
+      atomicOr(status, (uint32_t)kStatusBadQuery);
+    }
+    return;
+  }
+  VInfo vs = vinfo[rs];
+  VInfo vt = vinfo[rt];
+  const int32_t ds = vs.depth, dt = vt.depth;
+  for (int32_t i = lane; i <= ds; i += 64) df[i] = kInf;
+  for (int32_t i = lane; i <= dt; i += 64) dr[i] = kInf;
+  __syncthreads();
+  if (lane == 0) { df[0] = 0; dr[0] = 0; }
+  __syncthreads();
+  int32_t nscan = 0;
+  // upward search from s
+  {
+    int32_t x = rs;
+    VInfo vi = vs;
+    int32_t p = 0;
+    while (true) {
+      const int32_t par = vi.parent;
+      VInfo nx;
+      if (par >= 0) nx = vinfo[par];
+      if (lane == 0) cs[p] = x;
+      const int32_t dx = df[p];
+      if (dx != kInf) {
+        nscan += vi.deg;
+        for (int32_t j = lane; j < vi.deg; j += 64) {
+          const int32_t a = vi.first + j;
+          const int32_t pos = ds - adep[a];
+          const int32_t v = inf_add(dx, ud[a].x, st);
+          if (v < df[pos]) df[pos] = v;
+        }
+      }
+      __syncthreads();
+      if (par < 0) break;
+      x = par;
+      vi = nx;
+      p++;
+    }
+  }
+  // downward search from t, meeting at common ancestors
+  int32_t best = kInf;
+  {
+    int32_t x = rt;
+    VInfo vi = vt;
+    int32_t p = 0;
+    while (true) {
+      const int32_t par = vi.parent;
+      VInfo nx;
+      if (par >= 0) nx = vinfo[par];
+      const int32_t dx = dr[p];
+      if (dx != kInf) {
+        const int32_t d = vi.depth;
+        if (d <= ds && cs[ds - d] == x) {
+          const int32_t fx = df[ds - d];
+          const int32_t v = inf_add(dx, fx, st);
+          if (v < best) best = v;
+        }
+        nscan += vi.deg;
+        for (int32_t j = lane; j < vi.deg; j += 64) {
+          const int32_t a = vi.first + j;
+          const int32_t pos = dt - adep[a];
+          const int32_t v = inf_add(dx, ud[a].y, st);
+          if (v < dr[pos]) dr[pos] = v;
+        }
+      }
+      __syncthreads();
+      if (par < 0) break;
+      x = par;
+      vi = nx;
+      p++;
+    }
+  }
+  st = wave_or(st);
+  if (lane == 0) {
+    dist[q] = best;
+    if (scanned) scanned[q] = nscan;
+    if (st) atomicOr(status, st);
+  }
+}
+
+}  // namespace nwcch
+
+// ================================================================ C ABI (device)
+
+#include "../../include/nwcch.h"
+
+using namespace nwcch;
+
+namespace nwcch {
+void set_error(const std::string& s);
+const Topology* topo_of(const nwcch_topo* t);
+Topology* topo_of_mut(nwcch_topo* t);
+}  // namespace nwcch
+
+struct nwcch_dev {
+  DevTopo d;
+};
+
+namespace {
+
+template <class T>
+T* dev_upload(DevTopo& d, const T* host, size_t count) {
+  T* p = nullptr;
+  size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+  HIP_OK(hipMalloc((void**)&p, bytes));
+  d.owned.push_back(p);
+  if (count) HIP_OK(hipMemcpy(p, host, count * sizeof(T), hipMemcpyHostToDevice));
+  return p;
+}
+
+template <class T>
+T* dev_alloc(DevTopo& d, size_t count) {
+  T* p = nullptr;
+  HIP_OK(hipMalloc((void**)&p, std::max<size_t>(count, 1) * sizeof(T)));
+  d.owned.push_back(p);
+  return p;
+}
+
+int fail(const std::exception& e) {
+  set_error(e.what());
+  return NWCCH_ERR_RUNTIME;
+}
+
+}  // namespace
+
+extern "C" int nwcch_dev_create(const nwcch_topo* topo, int device, nwcch_dev** out) {
+  if (!topo || !out) { set_error("null argument"); return NWCCH_ERR_ARG; }
+  *out = nullptr;
+  nwcch_dev* h = new nwcch_dev();
+  try {
+    const Topology& t = *topo_of(topo);
+    DevTopo& d = h->d;
+    HIP_OK(hipSetDevice(device));
+    d.device = device;
+    d.n = t.n; d.m = t.m; d.mp = t.mp; d.T = t.triangles;
+    d.levels = t.levels; d.depth = t.depth;
+    d.lv = t.lv;
+    d.loop_edges = t.loop_edges;
+    d.sched_arc = dev_upload(d, t.sched_arc.data(), t.sched_arc.size());
+    d.tri_off = dev_upload(d, t.tri_off.data(), t.tri_off.size());
+    d.tri_ab = dev_upload(d, reinterpret_cast<const int2*>(t.tri_ab.data()), (size_t)t.triangles);
+    d.ae_off = dev_upload(d, t.ae_off.data(), t.ae_off.size());
+    d.ae_idx = dev_upload(d, t.ae_idx.data(), t.ae_idx.size());
+    std::vector<VInfo> vi(t.n);
+    for (i64 x = 0; x < t.n; x++)
+      vi[x] = VInfo{(int32_t)t.first[x], (int32_t)(t.first[x + 1] - t.first[x]), t.parent[x], t.vdepth[x]};
+    d.vinfo = dev_upload(d, vi.data(), vi.size());
+    std::vector<int32_t> adep(t.mp);
+    for (i64 a = 0; a < t.mp; a++) adep[a] = t.vdepth[t.head[a]];
+    d.adep = dev_upload(d, adep.data(), adep.size());
+    d.status = dev_alloc<uint32_t>(d, 1);
+    HIP_OK(hipMemset(d.status, 0, sizeof(uint32_t)));
+    d.scan = dev_alloc<unsigned long long>(d, 4);
+    HIP_OK(hipDeviceSynchronize());
+  } catch (const std::exception& e) {
+    for (void* p : h->d.owned) (void)hipFree(p);
+    delete h;
+    return fail(e);
+  }
+  *out = h;
+  return NWCCH_OK;
+}
+
+extern "C" void nwcch_dev_destroy(nwcch_dev* h) {
+  if (!h) return;
+  (void)hipSetDevice(h->d.device);
+  (void)hipDeviceSynchronize();
+  for (hipEvent_t e : h->d.ev_pool) (void)hipEventDestroy(e);
+  for (void* p : h->d.owned) (void)hipFree(p);
+  delete h;
+}
+
+extern "C" int nwcch_dev_info(const nwcch_dev* h, int64_t* out6) {
+  if (!h || !out6) { set_error("null argument"); return NWCCH_ERR_ARG; }
+  out6[0] = h->d.n; out6[1] = h->d.m; out6[2] = h->d.mp; out6[3] = h->d.T;
+  out6[4] = h->d.depth; out6[5] = h->d.levels;
+  return NWCCH_OK;
+}
+
+extern "C" int nwcch_customize(nwcch_dev* h, const int32_t* w, int32_t* ud, void* stream) {
+  if (!h || !w || !ud) { set_error("null argument"); return NWCCH_ERR_ARG; }
+  try {
+    DevTopo& d = h->d;
+    hipStream_t s = (hipStream_t)stream;
+    for (const LevelRange& r : d.lv) {
+      const i64 nbig = r.big_end - r.begin, nsmall = r.end - r.big_end;
+      if (nbig + nsmall == 0) continue;
+      const int bb = (int)((nbig + 3) / 4);
+      const int bs = (int)((nsmall + 255) / 256);
+      ProfScope ps(d, s, kKCustomize);
+      hipLaunchKernelGGL(k_customize_level, dim3(bb + bs), dim3(256), 0, s, r.begin, r.big_end, r.end,
+                         bb, d.sched_arc, d.tri_off, d.tri_ab, d.ae_off, d.ae_idx, w,
+                         reinterpret_cast<int2*>(ud), d.status);
+    }
+    HIP_OK(hipGetLastError());
+  } catch (const std::exception& e) {
+    return fail(e);
+  }
+  return NWCCH_OK;
+}
+
+extern "C" int nwcch_negcycle_scan(const int32_t* ud, int64_t mp, int64_t* out2, void* stream) {
+  if (!ud || !out2 || mp < 0) { set_error("null argument"); return NWCCH_ERR_ARG; }
+  try {
+    hipStream_t s = (hipStream_t)stream;
+    unsigned long long* acc = reinterpret_cast<unsigned long long*>(out2);
+    hipLaunchKernelGGL(k_scan_init, dim3(1), dim3(1), 0, s, acc);
+    if (mp > 0) {
+      i64 blocks = (mp + 255) / 256;
+      blocks = std::min<i64>(std::max<i64>(blocks, 1), 4096);
+      hipLaunchKernelGGL(k_negcycle_scan, dim3((unsigned)blocks), dim3(256), 0, s,
+                         reinterpret_cast<const int2*>(ud), (i64)mp, acc);
+    }
+    hipLaunchKernelGGL(k_scan_finish, dim3(1), dim3(1), 0, s, acc, (long long*)out2);
+    HIP_OK(hipGetLastError());
+  } catch (const std::exception& e) {
+    return fail(e);
+  }
+  return NWCCH_OK;
+}
+
+extern "C" int nwcch_negcycle_list(const int32_t* ud, int64_t mp, int64_t* list, int64_t cap,
+                                   int64_t* count, void* stream) {
+  if (!ud || !list || !count || mp < 0 || cap < 0) { set_error("null argument"); return NWCCH_ERR_ARG; }
+  try {
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_zero_i64, dim3(1), dim3(1), 0, s, (long long*)count);
+    if (mp > 0) {
+      i64 blocks = std::min<i64>(std::max<i64>((mp + 255) / 256, 1), 4096);
+      hipLaunchKernelGGL(k_negcycle_list, dim3((unsigned)blocks), dim3(256), 0, s,
+                         reinterpret_cast<const int2*>(ud), (i64)mp, (long long*)list, (i64)cap,
+                         (unsigned long long*)count);
+    }
+    HIP_OK(hipGetLastError());
+  } catch (const std::exception& e) {
+    return fail(e);
+  }
+  return NWCCH_OK;
+}
+
+extern "C" int nwcch_query(nwcch_dev* h, const int32_t* ud, const int32_t* s_rank, const int32_t* t_rank,
+                           int64_t nq, int32_t* dist, int32_t* scanned, void* stream) {
+  if (!h || !ud || !s_rank || !t_rank || !dist || nq < 0) {
+    set_error("null argument");
+    return NWCCH_ERR_ARG;
+  }
+  if (nq == 0) return NWCCH_OK;
+  try {
+    DevTopo& d = h->d;
+    hipStream_t s = (hipStream_t)stream;
+    const int32_t maxlen = d.depth + 1;
+    const size_t lds = (size_t)3 * maxlen * sizeof(int32_t);
+    if (lds > 160 * 1024) throw std::runtime_error("elimination tree too deep for the LDS query kernel");
+    ProfScope ps(d, s, kKQuery);
+    hipLaunchKernelGGL(k_query, dim3((unsigned)nq), dim3(64), lds, s, d.vinfo, d.adep,
+                       reinterpret_cast<const int2*>(ud), s_rank, t_rank, (i64)nq, (int32_t)d.n,
+                       maxlen, dist, scanned, d.status);
+    HIP_OK(hipGetLastError());
+  } catch (const std::exception& e) {
+    return fail(e);
+  }
+  return NWCCH_OK;
+}
+
+extern "C" int nwcch_profile_begin(nwcch_dev* h, int64_t max_launches) {
+  if (!h || max_launches < 0) { set_error("bad argument"); return NWCCH_ERR_ARG; }
+  try {
+    DevTopo& d = h->d;
+    HIP_OK(hipSetDevice(d.device));
+    while ((int64_t)d.ev_pool.size() < 2 * max_launches) {
+      hipEvent_t e;
+      HIP_OK(hipEventCreate(&e));
+      d.ev_pool.push_back(e);
+    }
+    d.ev_used = 0;
+    d.recs.clear();
+    d.prof_on = true;
+  } catch (const std::exception& e) {
+    return fail(e);
+  }
+  return NWCCH_OK;
+}
+
+extern "C" int nwcch_profile_end(nwcch_dev* h, int32_t* kernel, float* ms, int64_t cap, int64_t* count) {
+  if (!h || !count) { set_error("null argument"); return NWCCH_ERR_ARG; }
+  try {
+    DevTopo& d = h->d;
+    d.prof_on = false;
+    int64_t i = 0;
+    for (const auto& r : d.recs) {
+      HIP_OK(hipEventSynchronize(r.b));
+      if (i < cap) {
+        float t = 0.f;
+        HIP_OK(hipEventElapsedTime(&t, r.a, r.b));
+        if (kernel) kernel[i] = r.kernel;
+        if (ms) ms[i] = t;
+      }
+      i++;
+    }
+    *count = i;
+    d.recs.clear();
+    d.ev_used = 0;
+  } catch (const std::exception& e) {
+    return fail(e);
+  }
+  return NWCCH_OK;
+}
+
+extern "C" int nwcch_status(nwcch_dev* h, uint32_t* out, int clear, void* stream) {
+  if (!h || !out) { set_error("null argument"); return NWCCH_ERR_ARG; }
+  try {
+    hipStream_t s = (hipStream_t)stream;
+    HIP_OK(hipMemcpyAsync(out, h->d.status, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+    if (clear) {
+      HIP_OK(hipMemsetAsync(h->d.status, 0, sizeof(uint32_t), s));
+      HIP_OK(hipStreamSynchronize(s));
+    }
+  } catch (const std::exception& e) {
+    return fail(e);
+  }
+  return NWCCH_OK;
+}

@@ -1,0 +1,655 @@
+// Metric-independent CCH preprocessing on the host: undirected CSR, nested
+// dissection with inertial-flow bisection, symbolic contraction, the original-arc
+// to shortcut map, the pull-form lower-triangle index and the level schedule.
+//
+// Reference behaviour followed (not its code):
+//   undirected CSR          python/cch_nb.py:22-71, cpp/cch.cpp:97-130
+//   inertial-flow ND        python/inertial_flow.py:14-150, python/cch.py:42-106,
+//                           docs/technical_note.md:194-196
+//   symbolic contraction    python/cch.py:136-166, python/cch_nb.py:431-481
+//   edge -> arc map         python/cch.py:188-204, python/cch_nb.py:496-515
+//   lower triangles         python/cch.py:168-186 (by_c grouping: cch.py:308)
+//   levels                  cpp/cch.cpp:468-480
+#include "nwcch_topo.h"
+
+#include <algorithm>
+#include <atomic>
+#include <memory>
+#include <cmath>
+#include <condition_variable>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <numeric>
+#include <stdexcept>
+#include <thread>
+
+namespace nwcch {
+
+// ------------------------------------------------------------ undirected CSR
+
+void undirected_csr(i64 n, const i64* off, const i32* tgt, std::vector<i64>& noff,
+                    std::vector<i32>& nbr) {
+  std::vector<i64> deg(n + 1, 0);
+  for (i64 u = 0; u < n; u++)
+    for (i64 k = off[u]; k < off[u + 1]; k++) {
+      i64 v = tgt[k];
+      if (v < 0 || v >= n) throw std::runtime_error("arc head out of range");
+      if (v != u) { deg[u + 1]++; deg[v + 1]++; }
+    }
+  for (i64 i = 0; i < n; i++) deg[i + 1] += deg[i];
+  std::vector<i32> tmp(deg[n]);
+  std::vector<i64> pos(deg.begin(), deg.end() - 1);
+  for (i64 u = 0; u < n; u++)
+    for (i64 k = off[u]; k < off[u + 1]; k++) {
+      i64 v = tgt[k];
+      if (v != u) { tmp[pos[u]++] = (i32)v; tmp[pos[v]++] = (i32)u; }
+    }
+  noff.assign(n + 1, 0);
+  nbr.clear();
+  nbr.reserve(deg[n]);
+  for (i64 u = 0; u < n; u++) {
+    std::sort(tmp.begin() + deg[u], tmp.begin() + deg[u + 1]);
+    i32 last = -1;
+    for (i64 k = deg[u]; k < deg[u + 1]; k++)
+      if (tmp[k] != last) { nbr.push_back(tmp[k]); last = tmp[k]; }
+    noff[u + 1] = (i64)nbr.size();
+  }
+}
+
+// ------------------------------------------------- inertial-flow bisection
+
+namespace {
+
+constexpr int kInfCap = 1 << 29;
+
+// Membership of the cell being split.  Cells split concurrently are disjoint;
+// cell[v] holds the id of the cell v currently belongs to (written and read
+// atomically, since a worker reads the entries of neighbours in other cells) and
+// loc[v] its index there, read only when the id matches.
+struct Member {
+  std::atomic<i64>* cell;
+  i32* loc;
+  i64 id;
+  i32 operator[](i32 v) const {
+    return cell[v].load(std::memory_order_relaxed) == id ? loc[v] : -1;
+  }
+};
+
+// Max-flow network of one cell on the node-split graph: node 2*l = in(l),
+// 2*l+1 = out(l) for local vertex l; S = 2*s, T = 2*s+1.  Arcs in CSR with
+// explicit reverse indices.
+struct CellFlow {
+  int s = 0, S = 0, T = 0, N = 0;
+  std::vector<int> off, to, rev, cap;
+  std::vector<int> level, it, queue;
+
+  // Only "active" vertices take part: interior vertices of the source or sink
+  // set (all of whose cell neighbours share their role) cannot lie on a useful
+  // augmenting path; the caller assigns them to their side directly.
+  void build(const std::vector<i32>& cell, const Member& loc,
+             const std::vector<i64>& noff, const std::vector<i32>& nbr,
+             const std::vector<int8_t>& role, const std::vector<char>& active) {
+    s = (int)cell.size();
+    S = 2 * s;
+    T = 2 * s + 1;
+    N = 2 * s + 2;
+    std::vector<int> deg(N + 1, 0);
+    auto add_deg = [&](int a, int b) { deg[a + 1]++; deg[b + 1]++; };
+    for (int l = 0; l < s; l++) {
+      if (!active[l]) continue;
+      i32 v = cell[l];
+      add_deg(2 * l, 2 * l + 1);
+      for (i64 j = noff[v]; j < noff[v + 1]; j++) {
+        i32 u = nbr[j];
+        if (loc[u] >= 0 && active[loc[u]]) add_deg(2 * l + 1, 2 * loc[u]);
+      }
+      if (role[l] == 1) add_deg(S, 2 * l);
+      if (role[l] == 2) add_deg(2 * l + 1, T);
+    }
+    for (int i = 0; i < N; i++) deg[i + 1] += deg[i];
+    off = deg;
+    to.assign(deg[N], 0);
+    rev.assign(deg[N], 0);
+    cap.assign(deg[N], 0);
+    std::vector<int> pos(deg.begin(), deg.end() - 1);
+    auto add = [&](int a, int b, int c) {
+      int ia = pos[a]++, ib = pos[b]++;
+      to[ia] = b; cap[ia] = c; rev[ia] = ib;
+      to[ib] = a; cap[ib] = 0; rev[ib] = ia;
+    };
+    for (int l = 0; l < s; l++) {
+      if (!active[l]) continue;
+      i32 v = cell[l];
+      // sources and sinks cannot be cut (infinite vertex capacity)
+      add(2 * l, 2 * l + 1, role[l] ? kInfCap : 1);
+      for (i64 j = noff[v]; j < noff[v + 1]; j++) {
+        i32 u = nbr[j];
+        if (loc[u] >= 0 && active[loc[u]]) add(2 * l + 1, 2 * loc[u], kInfCap);
+      }
+      if (role[l] == 1) add(S, 2 * l, kInfCap);
+      if (role[l] == 2) add(2 * l + 1, T, kInfCap);
+    }
+    level.assign(N, -1);
+    it.assign(N, 0);
+    queue.assign(N, 0);
+  }
+
+  bool bfs() {
+    std::fill(level.begin(), level.end(), -1);
+    int qh = 0, qt = 0;
+    queue[qt++] = S;
+    level[S] = 0;
+    while (qh < qt) {
+      int x = queue[qh++];
+      // nodes at or beyond T's level cannot be on a shortest augmenting path
+      if (level[T] >= 0 && level[x] >= level[T]) break;
+      for (int e = off[x]; e < off[x + 1]; e++)
+        if (cap[e] > 0 && level[to[e]] < 0) {
+          level[to[e]] = level[x] + 1;
+          queue[qt++] = to[e];
+        }
+    }
+    return level[T] >= 0;
+  }
+
+  // One augmenting path of one unit (every S-T path crosses a unit vertex arc
+  // unless a source touches a sink, which the caller excludes).
+  int augment() {
+    std::vector<int>& path = stack_;
+    path.clear();
+    int x = S;
+    while (true) {
+      if (x == T) {
+        int f = kInfCap;
+        for (int e : path) f = std::min(f, cap[e]);
+        for (int e : path) { cap[e] -= f; cap[rev[e]] += f; }
+        return f;
+      }
+      bool moved = false;
+      for (int& e = it[x]; e < off[x + 1]; e++) {
+        int y = to[e];
+        if (cap[e] > 0 && level[y] == level[x] + 1) {
+          path.push_back(e);
+          x = y;
+          moved = true;
+          break;
+        }
+      }
+      if (!moved) {
+        if (path.empty()) return 0;
+        level[x] = -1;  // dead end
+        int e = path.back();
+        path.pop_back();
+        x = to[rev[e]];
+        it[x]++;
+      }
+    }
+  }
+
+  long maxflow(long limit) {
+    long flow = 0;
+    while (flow < limit && bfs()) {
+      for (int i = 0; i < N; i++) it[i] = off[i];
+      int f;
+      while (flow < limit && (f = augment()) > 0) flow += f;
+    }
+    return flow;
+  }
+
+  // residual reachability from S (after maxflow)
+  void reach(std::vector<char>& seen) {
+    seen.assign(N, 0);
+    int qh = 0, qt = 0;
+    queue[qt++] = S;
+    seen[S] = 1;
+    while (qh < qt) {
+      int x = queue[qh++];
+      for (int e = off[x]; e < off[x + 1]; e++)
+        if (cap[e] > 0 && !seen[to[e]]) { seen[to[e]] = 1; queue[qt++] = to[e]; }
+    }
+  }
+
+  std::vector<int> stack_;
+};
+
+struct Split {
+  std::vector<i32> A, B, sep;
+  bool ok = false;
+};
+
+// Geometric median split with the boundary of the smaller-boundary side as
+// separator (the reference's fallback, python/cch.py:65-97).
+Split geometric_split(const std::vector<i32>& cell, const Member& loc,
+                      const std::vector<i64>& noff, const std::vector<i32>& nbr,
+                      const double* x, const double* y) {
+  Split best;
+  const int s = (int)cell.size();
+  std::vector<int> idx(s);
+  std::vector<int8_t> side(s);
+  for (int d = 0; d < 4; d++) {
+    auto key = [&](int l) {
+      i32 v = cell[l];
+      switch (d) {
+        case 0: return x[v];
+        case 1: return y[v];
+        case 2: return x[v] + y[v];
+        default: return x[v] - y[v];
+      }
+    };
+    std::iota(idx.begin(), idx.end(), 0);
+    std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return key(a) < key(b); });
+    int half = s / 2;
+    for (int i = 0; i < s; i++) side[idx[i]] = i < half ? 0 : 1;
+    std::vector<i32> bA, bB;
+    for (int l = 0; l < s; l++) {
+      i32 v = cell[l];
+      bool bd = false;
+      for (i64 j = noff[v]; j < noff[v + 1] && !bd; j++) {
+        i32 u = nbr[j];
+        if (loc[u] >= 0 && side[loc[u]] != side[l]) bd = true;
+      }
+      if (bd) (side[l] == 0 ? bA : bB).push_back(l);
+    }
+    int sepside = bB.size() < bA.size() ? 1 : 0;
+    std::vector<i32>& sp = sepside ? bB : bA;
+    if (!best.ok || sp.size() < best.sep.size()) {
+      best.ok = true;
+      best.sep.clear();
+      best.A.clear();
+      best.B.clear();
+      std::vector<char> insep(s, 0);
+      for (i32 l : sp) insep[l] = 1;
+      for (int l = 0; l < s; l++) {
+        if (insep[l]) best.sep.push_back(cell[l]);
+        else if (side[l] == 0) best.A.push_back(cell[l]);
+        else best.B.push_back(cell[l]);
+      }
+    }
+  }
+  return best;
+}
+
+// One cut direction: min vertex cut between the first and last 25% of the cell
+// along the projection (python/inertial_flow.py:14-133).  ok = false if a source
+// touches a sink (no finite vertex cut exists).
+Split direction_cut(int d, const std::vector<i32>& cell, const Member& loc,
+                    const std::vector<i64>& noff, const std::vector<i32>& nbr,
+                    const double* x, const double* y) {
+  Split cur;
+  const int s = (int)cell.size();
+  const int k = std::max(1, (int)(s * 0.25));
+  std::vector<int> idx(s);
+  std::vector<int8_t> role(s, 0);
+  std::vector<char> seen, active(s, 1);
+  auto key = [&](int l) {
+    i32 v = cell[l];
+    switch (d) {
+      case 0: return x[v];
+      case 1: return y[v];
+      case 2: return x[v] + y[v];
+      default: return x[v] - y[v];
+    }
+  };
+  std::iota(idx.begin(), idx.end(), 0);
+  std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return key(a) < key(b); });
+  for (int i = 0; i < k; i++) role[idx[i]] = 1;
+  for (int i = s - k; i < s; i++) role[idx[i]] = 2;
+  for (int i = 0; i < k; i++) {
+    i32 v = cell[idx[i]];
+    for (i64 j = noff[v]; j < noff[v + 1]; j++) {
+      i32 u = nbr[j];
+      if (loc[u] >= 0 && role[loc[u]] == 2) return cur;
+    }
+  }
+  for (int l = 0; l < s; l++) {
+    if (!role[l]) continue;
+    bool inner = true;
+    i32 v = cell[l];
+    for (i64 j = noff[v]; j < noff[v + 1] && inner; j++) {
+      i32 u = nbr[j];
+      if (loc[u] >= 0 && role[loc[u]] != role[l]) inner = false;
+    }
+    if (inner) active[l] = 0;
+  }
+  CellFlow net;
+  net.build(cell, loc, noff, nbr, role, active);
+  net.maxflow((long)s + 1);
+  net.reach(seen);
+  cur.ok = true;
+  for (int l = 0; l < s; l++) {
+    bool in_r = seen[2 * l], out_r = seen[2 * l + 1];
+    if (!active[l]) in_r = out_r = role[l] == 1;
+    if (out_r) cur.A.push_back(cell[l]);
+    else if (in_r) cur.sep.push_back(cell[l]);
+    else cur.B.push_back(cell[l]);
+  }
+  return cur;
+}
+
+// Best of the 4 directions: smallest separator, then most balanced, then the
+// lowest direction index.  Big cells run the directions on 4 threads; the
+// result does not depend on the thread count.
+Split inertial_split(const std::vector<i32>& cell, const Member& loc,
+                     const std::vector<i64>& noff, const std::vector<i32>& nbr,
+                     const double* x, const double* y, bool parallel) {
+  Split cut[4];
+  if (parallel) {
+    std::vector<std::thread> th;
+    for (int d = 0; d < 4; d++)
+      th.emplace_back([&, d] { cut[d] = direction_cut(d, cell, loc, noff, nbr, x, y); });
+    for (auto& t : th) t.join();
+  } else {
+    for (int d = 0; d < 4; d++) cut[d] = direction_cut(d, cell, loc, noff, nbr, x, y);
+  }
+  int best = -1;
+  auto bal = [](const Split& q) { return std::min(q.A.size(), q.B.size()); };
+  for (int d = 0; d < 4; d++) {
+    if (!cut[d].ok) continue;
+    if (best < 0 || cut[d].sep.size() < cut[best].sep.size() ||
+        (cut[d].sep.size() == cut[best].sep.size() && bal(cut[d]) > bal(cut[best])))
+      best = d;
+  }
+  if (best < 0) return Split{};
+  return std::move(cut[best]);
+}
+
+}  // namespace
+
+void nested_dissection(i64 n, const std::vector<i64>& noff, const std::vector<i32>& nbr,
+                       const double* x, const double* y, int leaf, i32* rank) {
+  if (leaf < 2) leaf = 2;
+  for (i64 v = 0; v < n; v++) rank[v] = -1;
+  // cells are split by a pool of workers; see Member for the shared arrays
+  std::unique_ptr<std::atomic<i64>[]> cellof(new std::atomic<i64>[n]);
+  for (i64 v = 0; v < n; v++) cellof[v].store(-1, std::memory_order_relaxed);
+  std::vector<i32> locv(n, -1);
+  std::atomic<i64> next_id{0};
+  struct Task { std::vector<i32> nodes; i64 lo; };
+  std::vector<Task> stack;
+  {
+    Task t;
+    t.nodes.resize(n);
+    std::iota(t.nodes.begin(), t.nodes.end(), 0);
+    t.lo = 0;
+    stack.push_back(std::move(t));
+  }
+  std::mutex mu;
+  std::condition_variable cv;
+  int busy = 0;
+  std::string err;
+  auto work = [&](Task t, std::vector<Task>& out) {
+    std::vector<i32>& cell = t.nodes;
+    const i64 s = (i64)cell.size();
+    if (s == 0) return;
+    if (s <= leaf) {
+      // small cell: ascending undirected degree, stable (python/cch.py:51-56)
+      std::stable_sort(cell.begin(), cell.end(), [&](i32 a, i32 b) {
+        return noff[a + 1] - noff[a] < noff[b + 1] - noff[b];
+      });
+      for (i64 i = 0; i < s; i++) rank[cell[i]] = (i32)(t.lo + i);
+      return;
+    }
+    const Member loc{cellof.get(), locv.data(), next_id.fetch_add(1)};
+    for (i64 l = 0; l < s; l++) {
+      locv[cell[l]] = (i32)l;
+      cellof[cell[l]].store(loc.id, std::memory_order_relaxed);
+    }
+    Split sp = inertial_split(cell, loc, noff, nbr, x, y, s >= 20000);
+    if (!sp.ok) sp = geometric_split(cell, loc, noff, nbr, x, y);
+    const i64 hi = t.lo + s;
+    const i64 ns = (i64)sp.sep.size();
+    for (i64 i = 0; i < ns; i++) rank[sp.sep[i]] = (i32)(hi - ns + i);
+    const i64 na = (i64)sp.A.size();
+    out.push_back(Task{std::move(sp.A), t.lo});
+    out.push_back(Task{std::move(sp.B), t.lo + na});
+  };
+  int nth = (int)std::thread::hardware_concurrency();
+  if (const char* e = std::getenv("NWCCH_THREADS")) nth = std::atoi(e);
+  nth = std::max(1, std::min(nth, 16));
+  auto worker = [&] {
+    std::vector<Task> out;
+    std::unique_lock<std::mutex> lk(mu);
+    while (true) {
+      cv.wait(lk, [&] { return !stack.empty() || busy == 0 || !err.empty(); });
+      if (!err.empty() || (stack.empty() && busy == 0)) break;
+      Task t = std::move(stack.back());
+      stack.pop_back();
+      busy++;
+      lk.unlock();
+      out.clear();
+      try {
+        work(std::move(t), out);
+      } catch (const std::exception& e) {
+        lk.lock();
+        err = e.what();
+        busy--;
+        cv.notify_all();
+        break;
+      }
+      lk.lock();
+      for (auto& o : out) stack.push_back(std::move(o));
+      busy--;
+      cv.notify_all();
+    }
+  };
+  std::vector<std::thread> pool;
+  for (int i = 0; i < nth; i++) pool.emplace_back(worker);
+  for (auto& t : pool) t.join();
+  if (!err.empty()) throw std::runtime_error(err);
+  for (i64 v = 0; v < n; v++)
+    if (rank[v] < 0) throw std::runtime_error("nested dissection left a vertex unranked");
+}
+
+// -------------------------------------------------------- symbolic contraction
+
+void symbolic(Topology& t, i64 n, const std::vector<i64>& noff, const std::vector<i32>& nbr,
+              const i32* rank) {
+  t.n = n;
+  t.rank.assign(rank, rank + n);
+  std::vector<i32> inv(n, -1);
+  for (i64 v = 0; v < n; v++) {
+    i32 r = rank[v];
+    if (r < 0 || r >= n || inv[r] >= 0) throw std::runtime_error("order must be a permutation");
+    inv[r] = (i32)v;
+  }
+  t.parent.assign(n, -1);
+  t.first.assign(n + 1, 0);
+  t.head.clear();
+  t.head.reserve(4 * n);
+  std::vector<i32> kid_head(n, -1), kid_next(n, -1), mark(n, -1), row;
+  for (i64 x = 0; x < n; x++) {
+    row.clear();
+    const i32 v = inv[x];
+    for (i64 j = noff[v]; j < noff[v + 1]; j++) {
+      i32 r = rank[nbr[j]];
+      if (r > x && mark[r] != x) { mark[r] = (i32)x; row.push_back(r); }
+    }
+    // upper neighbours of each child, minus x itself, join x's row
+    for (i32 c = kid_head[x]; c >= 0; c = kid_next[c])
+      for (i64 a = t.first[c]; a < t.first[c + 1]; a++) {
+        i32 r = t.head[a];
+        if (r != x && mark[r] != x) { mark[r] = (i32)x; row.push_back(r); }
+      }
+    std::sort(row.begin(), row.end());
+    t.head.insert(t.head.end(), row.begin(), row.end());
+    t.first[x + 1] = (i64)t.head.size();
+    if (!row.empty()) {
+      i32 p = row[0];
+      t.parent[x] = p;
+      kid_next[x] = kid_head[p];
+      kid_head[p] = (i32)x;
+    }
+  }
+  t.mp = (i64)t.head.size();
+  if (t.mp >= ((i64)1 << 31) - 1) throw std::runtime_error("too many shortcut arcs for int32 ids");
+}
+
+static i64 find_arc(const Topology& t, i64 lo, i64 hi) {
+  auto b = t.head.begin() + t.first[lo], e = t.head.begin() + t.first[lo + 1];
+  auto it = std::lower_bound(b, e, (i32)hi);
+  if (it == e || *it != hi) return -1;
+  return it - t.head.begin();
+}
+
+void map_arcs(Topology& t, const i64* off, const i32* tgt, i64 m) {
+  t.m = m;
+  t.e_arc.assign(m, -1);
+  t.e_up.assign(m, 0);
+  t.loop_edges.clear();
+  for (i64 u = 0; u < t.n; u++) {
+    const i64 ru = t.rank[u];
+    for (i64 k = off[u]; k < off[u + 1]; k++) {
+      const i64 rv = t.rank[tgt[k]];
+      if (ru == rv) { t.loop_edges.push_back((i32)k); continue; }
+      i64 a = ru < rv ? find_arc(t, ru, rv) : find_arc(t, rv, ru);
+      if (a < 0) throw std::runtime_error("original arc missing from the contraction");
+      t.e_arc[k] = a;
+      t.e_up[k] = ru < rv ? 1 : 0;
+    }
+  }
+}
+
+// ---------------------------------------------- triangle index + schedule
+
+void build_index(Topology& t) {
+  const i64 n = t.n, mp = t.mp;
+  if ((i64)t.first.size() != n + 1 || (i64)t.first[n] != mp)
+    throw std::runtime_error("first/head sizes disagree");
+  // elimination tree depth and height; parent = first upper neighbour
+  t.parent.assign(n, -1);
+  for (i64 x = 0; x < n; x++) {
+    if (t.first[x + 1] > t.first[x]) {
+      i32 p = t.head[t.first[x]];
+      if (p <= x || p >= n) throw std::runtime_error("upward arc does not point upward");
+      t.parent[x] = p;
+    }
+  }
+  t.vdepth.assign(n, 0);
+  t.depth = 0;
+  for (i64 x = n - 1; x >= 0; x--) {
+    i32 p = t.parent[x];
+    if (p >= 0) { t.vdepth[x] = t.vdepth[p] + 1; t.depth = std::max(t.depth, t.vdepth[x]); }
+  }
+  t.vlevel.assign(n, 0);
+  i32 maxlvl = 0;
+  for (i64 x = 0; x < n; x++) {
+    i32 p = t.parent[x];
+    if (p >= 0 && t.vlevel[p] < t.vlevel[x] + 1) t.vlevel[p] = t.vlevel[x] + 1;
+    maxlvl = std::max(maxlvl, t.vlevel[x]);
+  }
+  t.levels = maxlvl + 1;
+
+  // lower-triangle counts per arc (pull form: triangles whose top arc is c)
+  std::vector<i64> cnt(mp + 1, 0);
+  t.triangles = 0;
+  for (i64 x = 0; x < n; x++) {
+    const i64 s = t.first[x], e = t.first[x + 1];
+    t.triangles += (e - s) * (e - s - 1) / 2;
+    for (i64 i = s; i < e; i++) {
+      const i64 y = t.head[i];
+      i64 j = t.first[y];
+      const i64 je = t.first[y + 1];
+      for (i64 k = i + 1; k < e; k++) {
+        const i32 z = t.head[k];
+        while (j < je && t.head[j] < z) j++;
+        if (j >= je || t.head[j] != z) throw std::runtime_error("upward CSR is not chordal");
+        cnt[j]++;
+      }
+    }
+  }
+  // schedule: arcs grouped by the ET height of their tail, within a level by
+  // lower-triangle count, descending (big arcs first), ties by arc id
+  std::vector<i32> tail(mp);
+  for (i64 x = 0; x < n; x++)
+    for (i64 a = t.first[x]; a < t.first[x + 1]; a++) tail[a] = (i32)x;
+  t.sched_arc.resize(mp);
+  std::iota(t.sched_arc.begin(), t.sched_arc.end(), 0);
+  std::stable_sort(t.sched_arc.begin(), t.sched_arc.end(), [&](i32 a, i32 b) {
+    i32 la = t.vlevel[tail[a]], lb = t.vlevel[tail[b]];
+    if (la != lb) return la < lb;
+    return cnt[a] > cnt[b];
+  });
+  t.arc_pos.resize(mp);
+  for (i64 p = 0; p < mp; p++) t.arc_pos[t.sched_arc[p]] = (i32)p;
+  t.tri_off.assign(mp + 1, 0);
+  for (i64 p = 0; p < mp; p++) t.tri_off[p + 1] = t.tri_off[p] + cnt[t.sched_arc[p]];
+  t.lv.assign(t.levels, LevelRange{0, 0, 0, 0});
+  {
+    i64 p = 0;
+    for (i32 L = 0; L < t.levels; L++) {
+      LevelRange r{p, p, p, 0};
+      while (p < mp && t.vlevel[tail[t.sched_arc[p]]] == L) {
+        if (cnt[t.sched_arc[p]] >= kBigArc) r.big_end = p + 1;
+        p++;
+      }
+      r.end = p;
+      r.triangles = t.tri_off[r.end] - t.tri_off[r.begin];
+      t.lv[L] = r;
+    }
+  }
+  // scatter (a, b) pairs into schedule order, x ascending within an arc
+  t.tri_ab.assign(2 * t.triangles, 0);
+  std::vector<i64> fill(mp);
+  for (i64 a = 0; a < mp; a++) fill[a] = t.tri_off[t.arc_pos[a]];
+  for (i64 x = 0; x < n; x++) {
+    const i64 s = t.first[x], e = t.first[x + 1];
+    for (i64 i = s; i < e; i++) {
+      const i64 y = t.head[i];
+      i64 j = t.first[y];
+      for (i64 k = i + 1; k < e; k++) {
+        const i32 z = t.head[k];
+        while (t.head[j] < z) j++;
+        const i64 q = fill[j]++;
+        t.tri_ab[2 * q] = (i32)i;
+        t.tri_ab[2 * q + 1] = (i32)k;
+      }
+    }
+  }
+  // original arcs grouped by shortcut
+  t.ae_off.assign(mp + 1, 0);
+  if (!t.e_arc.empty()) {
+    if (t.m >= ((i64)1 << 30)) throw std::runtime_error("too many original arcs for int32 edge ids");
+    for (i64 k = 0; k < t.m; k++)
+      if (t.e_arc[k] >= 0) t.ae_off[t.e_arc[k] + 1]++;
+    for (i64 a = 0; a < mp; a++) t.ae_off[a + 1] += t.ae_off[a];
+    t.ae_idx.assign(t.ae_off[mp], 0);
+    std::vector<i32> pos(t.ae_off.begin(), t.ae_off.end() - 1);
+    for (i64 k = 0; k < t.m; k++) {
+      i64 a = t.e_arc[k];
+      if (a >= 0) t.ae_idx[pos[a]++] = (i32)((k << 1) | (t.e_up[k] ? 1 : 0));
+    }
+  }
+  t.arc_tail = std::move(tail);
+}
+
+void build_side_index(Topology& t) {
+  if (t.has_side) return;
+  const i64 n = t.n, mp = t.mp;
+  t.side_off.assign(mp + 1, 0);
+  for (i64 x = 0; x < n; x++) {
+    const i64 s = t.first[x], e = t.first[x + 1];
+    const i64 d = e - s;
+    // arc i of row x is a lower side of the (d-1) triangles pairing it with the other arcs
+    for (i64 i = s; i < e; i++) t.side_off[i + 1] += d - 1;
+  }
+  for (i64 a = 0; a < mp; a++) t.side_off[a + 1] += t.side_off[a];
+  t.side_top.assign(t.side_off[mp], 0);
+  std::vector<i64> pos(t.side_off.begin(), t.side_off.end() - 1);
+  for (i64 x = 0; x < n; x++) {
+    const i64 s = t.first[x], e = t.first[x + 1];
+    for (i64 i = s; i < e; i++) {
+      const i64 y = t.head[i];
+      i64 j = t.first[y];
+      for (i64 k = i + 1; k < e; k++) {
+        const i32 z = t.head[k];
+        while (t.head[j] < z) j++;
+        t.side_top[pos[i]++] = (i32)j;
+        t.side_top[pos[k]++] = (i32)j;
+      }
+    }
+  }
+  t.has_side = true;
+}
+
+}  // namespace nwcch

@@ -1,0 +1,84 @@
+// Device-resident CCH topology (internal header shared by the HIP sources).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "nwcch_topo.h"
+
+namespace nwcch {
+
+constexpr int32_t kInf = 0x7fffffff;  // "no path": absorbs every addition
+
+// status bits written by kernels (OR-ed into DevTopo::status)
+enum : uint32_t {
+  kStatusOverflow = 1u,    // a finite sum left the int32 range
+  kStatusBadWeight = 2u,   // an input weight equal to INT32_MIN
+  kStatusBadQuery = 4u,    // a query vertex out of range
+  kStatusQueueFull = 8u,   // partial-update frontier exceeded its capacity
+};
+
+// Packed per-vertex record for the query walk.
+struct VInfo {
+  int32_t first, deg, parent, depth;
+};
+
+struct DevTopo {
+  int device = 0;
+  i64 n = 0, m = 0, mp = 0, T = 0;
+  int32_t levels = 0, depth = 0;
+  std::vector<LevelRange> lv;        // host copy of the level schedule
+  std::vector<int32_t> loop_edges;   // host copy
+
+  // customization
+  int32_t* sched_arc = nullptr;      // [mp]
+  i64* tri_off = nullptr;            // [mp+1] by schedule position
+  int2* tri_ab = nullptr;            // [T]
+  int32_t* ae_off = nullptr;         // [mp+1] by arc id
+  int32_t* ae_idx = nullptr;         // [#mapped original arcs]
+  // query
+  VInfo* vinfo = nullptr;            // [n]
+  int32_t* adep = nullptr;           // [mp] ET depth of the arc's head
+  // partial update (lazy)
+  bool has_update = false;
+  int32_t* arc_pos = nullptr;        // [mp]
+  int32_t* arc_lvl = nullptr;        // [mp] ET height of the tail
+  i64* side_off = nullptr;           // [mp+1]
+  int32_t* side_top = nullptr;       // [2T]
+  int32_t* lvl_cap_off = nullptr;    // [levels+1] frontier bucket offsets (= arcs per level)
+  // scratch
+  uint32_t* status = nullptr;        // [1]
+  unsigned long long* scan = nullptr;  // [4]
+  std::vector<void*> owned;
+
+  // optional launch profiling: every launch is bracketed by two events
+  struct Rec { int kernel; hipEvent_t a, b; };
+  bool prof_on = false;
+  std::vector<hipEvent_t> ev_pool;
+  size_t ev_used = 0;
+  std::vector<Rec> recs;
+};
+
+// kernel ids reported by nwcch_profile_read
+enum : int { kKCustomize = 0, kKScan = 1, kKQuery = 2, kKUpdate = 3, kKBatch = 4, kKOther = 5 };
+
+// Bracket one launch with events when profiling is on.
+struct ProfScope {
+  DevTopo& d;
+  hipStream_t s;
+  int kernel;
+  hipEvent_t b = nullptr;
+  ProfScope(DevTopo& dd, hipStream_t ss, int k) : d(dd), s(ss), kernel(k) {
+    if (!d.prof_on || d.ev_used + 2 > d.ev_pool.size()) return;
+    hipEvent_t a = d.ev_pool[d.ev_used++];
+    b = d.ev_pool[d.ev_used++];
+    (void)hipEventRecord(a, s);
+    d.recs.push_back(DevTopo::Rec{kernel, a, b});
+  }
+  ~ProfScope() {
+    if (b) (void)hipEventRecord(b, s);
+  }
+};
+
+}  // namespace nwcch

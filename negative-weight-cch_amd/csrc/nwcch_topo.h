@@ -1,0 +1,76 @@
+// Host-side topology of a Customizable Contraction Hierarchy (internal header).
+//
+// Everything here is metric-independent and built once per graph + order on the
+// host: the upward CSR of the chordal completion in rank space, the original-arc
+// to shortcut map, the elimination tree, the lower-triangle index in "pull" form
+// (for every shortcut the pairs of shortcuts that form its lower triangles) and
+// the level schedule the device kernels follow.
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace nwcch {
+
+using i32 = int32_t;
+using i64 = int64_t;
+
+// One level of the customization schedule: schedule positions [begin, end) hold
+// the shortcuts whose lower endpoint has ET height `level`; positions
+// [begin, big_end) are the shortcuts with at least kBigArc lower triangles.
+struct LevelRange {
+  i64 begin, big_end, end;
+  i64 triangles;  // lower triangles of all shortcuts in the level
+};
+
+struct Topology {
+  i64 n = 0;        // vertices
+  i64 m = 0;        // original directed arcs
+  i64 mp = 0;       // shortcut arcs (m+)
+  i64 triangles = 0;
+  i32 depth = 0;    // max ET depth (arcs from the root)
+  i32 levels = 0;   // number of ET heights (= schedule levels)
+
+  std::vector<i32> rank;    // rank[v] (may be empty if built from a CSR)
+  std::vector<i64> first;   // [n+1] upward CSR in rank space
+  std::vector<i32> head;    // [mp], rows sorted ascending
+  std::vector<i32> parent;  // [n], -1 for roots
+  std::vector<i64> e_arc;   // [m], -1 for self loops
+  std::vector<int8_t> e_up; // [m], 1 = lower -> higher rank
+  std::vector<i32> vdepth;  // [n] ET depth (root = 0)
+  std::vector<i32> vlevel;  // [n] ET height (leaf = 0)
+
+  // pull-form lower-triangle index, arcs in schedule order
+  std::vector<i32> sched_arc;   // [mp] schedule position -> arc id
+  std::vector<i32> arc_pos;     // [mp] arc id -> schedule position
+  std::vector<i64> tri_off;     // [mp+1] by schedule position
+  std::vector<i32> tri_ab;      // [2*T] interleaved (a, b) = (arc(x,y), arc(x,z))
+  std::vector<LevelRange> lv;   // [levels]
+
+  // original arcs grouped by shortcut (by arc id): entry = (edge << 1) | e_up
+  std::vector<i32> ae_off;      // [mp+1]
+  std::vector<i32> ae_idx;
+  std::vector<i32> loop_edges;  // original arcs with e_arc == -1
+
+  // partial-update index (built lazily): for each arc, the top arcs of the
+  // lower triangles it is a side of
+  bool has_side = false;
+  std::vector<i64> side_off;    // [mp+1] by arc id
+  std::vector<i32> side_top;    // [2*T]
+  std::vector<i32> arc_tail;    // [mp]
+};
+
+constexpr i64 kBigArc = 48;   // lower-triangle count from which one wavefront owns an arc
+
+// Builders (host_prep.cpp).  They throw std::runtime_error on bad input.
+void undirected_csr(i64 n, const i64* off, const i32* tgt, std::vector<i64>& noff,
+                    std::vector<i32>& nbr);
+void nested_dissection(i64 n, const std::vector<i64>& noff, const std::vector<i32>& nbr,
+                       const double* x, const double* y, int leaf, i32* rank_out);
+void symbolic(Topology& t, i64 n, const std::vector<i64>& noff, const std::vector<i32>& nbr,
+              const i32* rank);
+void map_arcs(Topology& t, const i64* off, const i32* tgt, i64 m);
+void build_index(Topology& t);
+void build_side_index(Topology& t);
+
+}  // namespace nwcch

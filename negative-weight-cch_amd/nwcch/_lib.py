@@ -1,0 +1,182 @@
+"""ctypes binding of libnwcch.so (the C ABI declared in include/nwcch.h).
+
+The product path is this library and nothing else: if it is missing, importing
+the GPU entry points raises instead of falling back to any CPU code.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "libnwcch.so"
+INF = 2147483647
+
+NWCCH_OK = 0
+STATUS_BITS = {1: "int32 overflow", 2: "INT32_MIN weight", 4: "query vertex out of range",
+               8: "update frontier full"}
+
+_lib = None
+
+
+class NwcchError(RuntimeError):
+    pass
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise NwcchError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+        L = C.CDLL(str(LIB_PATH))
+        p, i32, i64, vp = C.c_void_p, C.c_int32, C.c_int64, C.c_void_p
+        sigs = {
+            "nwcch_last_error": ([], C.c_char_p),
+            "nwcch_order": ([i64, p, p, p, p, i32, p], C.c_int),
+            "nwcch_topo_create": ([i64, p, p, p, C.POINTER(vp)], C.c_int),
+            "nwcch_topo_from_csr": ([i64, p, p, i64, p, p, C.POINTER(vp)], C.c_int),
+            "nwcch_topo_destroy": ([vp], None),
+            "nwcch_topo_info": ([vp, p], C.c_int),
+            "nwcch_topo_export": ([vp, p, p, p, p, p, p, p], C.c_int),
+            "nwcch_topo_export_triangles": ([vp, p, p, p], C.c_int),
+            "nwcch_dev_create": ([vp, C.c_int, C.POINTER(vp)], C.c_int),
+            "nwcch_dev_destroy": ([vp], None),
+            "nwcch_dev_info": ([vp, p], C.c_int),
+            "nwcch_customize": ([vp, p, p, vp], C.c_int),
+            "nwcch_negcycle_scan": ([p, i64, p, vp], C.c_int),
+            "nwcch_negcycle_list": ([p, i64, p, i64, p, vp], C.c_int),
+            "nwcch_query": ([vp, p, p, p, i64, p, p, vp], C.c_int),
+            "nwcch_status": ([vp, p, C.c_int, vp], C.c_int),
+            "nwcch_profile_begin": ([vp, i64], C.c_int),
+            "nwcch_profile_end": ([vp, p, p, i64, p], C.c_int),
+        }
+        optional = {
+            "nwcch_update": ([vp, p, p, p, i64, p, vp], C.c_int),
+            "nwcch_update_result": ([vp, p, vp], C.c_int),
+            "nwcch_update_changed": ([vp, p, i64, vp], C.c_int),
+            "nwcch_customize_batch": ([vp, p, i32, p, i32, vp], C.c_int),
+            "nwcch_query_batch": ([vp, p, i32, p, p, p, i64, p, p, vp], C.c_int),
+        }
+        for name, (args, res) in sigs.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        for name, (args, res) in optional.items():
+            if hasattr(L, name):
+                f = getattr(L, name)
+                f.argtypes = args
+                f.restype = res
+        _lib = L
+    return _lib
+
+
+def check(rc: int, what: str):
+    if rc != NWCCH_OK:
+        msg = lib().nwcch_last_error().decode(errors="replace")
+        raise NwcchError(f"{what} failed ({rc}): {msg}")
+
+
+def ptr(a) -> int | None:
+    """Address of a numpy array or torch tensor (None for None)."""
+    if a is None:
+        return None
+    if isinstance(a, np.ndarray):
+        assert a.flags["C_CONTIGUOUS"], "arrays passed to libnwcch must be contiguous"
+        return a.ctypes.data
+    if hasattr(a, "data_ptr"):
+        assert a.is_contiguous(), "tensors passed to libnwcch must be contiguous"
+        return a.data_ptr()
+    raise TypeError(type(a))
+
+
+def exported_symbols():
+    """Names declared in include/nwcch.h (for the symbol-export test)."""
+    return ["nwcch_last_error", "nwcch_order", "nwcch_topo_create", "nwcch_topo_from_csr",
+            "nwcch_topo_destroy", "nwcch_topo_info", "nwcch_topo_export",
+            "nwcch_topo_export_triangles", "nwcch_dev_create", "nwcch_dev_destroy",
+            "nwcch_dev_info", "nwcch_customize", "nwcch_negcycle_scan", "nwcch_negcycle_list",
+            "nwcch_query", "nwcch_status", "nwcch_profile_begin", "nwcch_profile_end"]
+
+
+# ------------------------------------------------------------ host topology
+
+class Topology:
+    """Metric-independent CCH structure built by the native host code."""
+
+    def __init__(self, handle):
+        self._h = handle
+        info = np.zeros(8, np.int64)
+        check(lib().nwcch_topo_info(self._h, ptr(info)), "nwcch_topo_info")
+        (self.n, self.m, self.mp, self.triangles, self.depth, self.levels,
+         self.n_loops, self.n_mapped) = (int(v) for v in info)
+
+    @classmethod
+    def from_graph(cls, n, off, tgt, rank):
+        off = np.ascontiguousarray(off, np.int64)
+        tgt = np.ascontiguousarray(tgt, np.int32)
+        rank = np.ascontiguousarray(rank, np.int32)
+        if off.shape != (n + 1,) or rank.shape != (n,):
+            raise ValueError("off must be [n+1] and rank [n]")
+        h = C.c_void_p()
+        check(lib().nwcch_topo_create(n, ptr(off), ptr(tgt), ptr(rank), C.byref(h)), "nwcch_topo_create")
+        t = cls(h)
+        t.rank = rank.copy()
+        return t
+
+    @classmethod
+    def from_csr(cls, n, first, head, e_arc=None, e_up=None):
+        first = np.ascontiguousarray(first, np.int64)
+        head = np.ascontiguousarray(head, np.int32)
+        m = 0 if e_arc is None else int(len(e_arc))
+        ea = None if e_arc is None else np.ascontiguousarray(e_arc, np.int64)
+        eu = None if e_up is None else np.ascontiguousarray(e_up, np.int8)
+        h = C.c_void_p()
+        check(lib().nwcch_topo_from_csr(n, ptr(first), ptr(head), m, ptr(ea), ptr(eu), C.byref(h)),
+              "nwcch_topo_from_csr")
+        t = cls(h)
+        t.rank = None
+        return t
+
+    def export(self):
+        first = np.empty(self.n + 1, np.int64)
+        head = np.empty(self.mp, np.int32)
+        parent = np.empty(self.n, np.int32)
+        e_arc = np.empty(self.m, np.int64)
+        e_up = np.empty(self.m, np.int8)
+        level = np.empty(self.n, np.int32)
+        depth = np.empty(self.n, np.int32)
+        check(lib().nwcch_topo_export(self._h, ptr(first), ptr(head), ptr(parent), ptr(e_arc),
+                                      ptr(e_up), ptr(level), ptr(depth)), "nwcch_topo_export")
+        return dict(first=first, head=head, parent=parent, e_arc=e_arc, e_up=e_up,
+                    level=level, depth=depth)
+
+    def export_triangles(self):
+        sched = np.empty(self.mp, np.int32)
+        toff = np.empty(self.mp + 1, np.int64)
+        tab = np.empty(2 * self.triangles, np.int32)
+        check(lib().nwcch_topo_export_triangles(self._h, ptr(sched), ptr(toff), ptr(tab)),
+              "nwcch_topo_export_triangles")
+        return sched, toff, tab.reshape(-1, 2)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().nwcch_topo_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def order(n, off, tgt, x, y, leaf=64) -> np.ndarray:
+    """Nested-dissection order (inertial flow), rank[v]."""
+    off = np.ascontiguousarray(off, np.int64)
+    tgt = np.ascontiguousarray(tgt, np.int32)
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.ascontiguousarray(y, np.float64)
+    rank = np.empty(n, np.int32)
+    check(lib().nwcch_order(n, ptr(off), ptr(tgt), ptr(x), ptr(y), int(leaf), ptr(rank)), "nwcch_order")
+    return rank

@@ -1,0 +1,156 @@
+"""Device side of the path: a topology resident in HBM and the HIP entry points
+(customize, negative-cycle scan, ET queries) called through the C ABI.
+
+PyTorch provides the device memory and the stream; every computation is one of
+libnwcch's kernels.  Nothing here falls back to the CPU: without a GPU or
+without the built library the calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import INF, NwcchError, check, lib, ptr
+
+
+def _stream(stream=None) -> int:
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return s.cuda_stream
+
+
+def require_gpu():
+    if not torch.cuda.is_available():
+        raise NwcchError("no GPU: the MI355X path needs a HIP device (there is no CPU fallback)")
+
+
+def as_pairs(up, dn) -> torch.Tensor:
+    """The [mp, 2] int32 tensor behind (up, dn): the storage itself when up/dn
+    are the two columns of one such tensor (what customize returns), else an
+    interleaved device copy."""
+    if (isinstance(up, torch.Tensor) and isinstance(dn, torch.Tensor) and up.is_cuda
+            and up.dtype == torch.int32 and dn.dtype == torch.int32 and up.dim() == 1
+            and up.stride() == (2,) and dn.stride() == (2,)
+            and dn.data_ptr() == up.data_ptr() + 4 and up.untyped_storage().data_ptr() == dn.untyped_storage().data_ptr()):
+        base = up.as_strided((up.shape[0], 2), (2, 1))
+        return base
+    dev = torch.device("cuda")
+    u = torch.as_tensor(up).to(dev, torch.int32)
+    d = torch.as_tensor(dn).to(dev, torch.int32)
+    return torch.stack([u, d], 1).contiguous()
+
+
+class DeviceTopology:
+    """One topology uploaded to one GPU (owns the device copy)."""
+
+    def __init__(self, topo: _lib.Topology, device: int | None = None):
+        require_gpu()
+        self.topo = topo
+        self.device = torch.cuda.current_device() if device is None else int(device)
+        h = C.c_void_p()
+        with torch.cuda.device(self.device):
+            check(lib().nwcch_dev_create(topo._h, self.device, C.byref(h)), "nwcch_dev_create")
+        self._h = h
+        self.n, self.m, self.mp = topo.n, topo.m, topo.mp
+        self.triangles, self.depth, self.levels = topo.triangles, topo.depth, topo.levels
+        self._scan_out = torch.empty(2, dtype=torch.int64, device=self.dev)
+
+    @property
+    def dev(self):
+        return torch.device("cuda", self.device)
+
+    # ---------------------------------------------------------------- metric
+    def weights(self, w) -> torch.Tensor:
+        w = torch.as_tensor(w)
+        if w.dtype.is_floating_point:
+            wf = w.double()
+            fin = torch.isfinite(wf)
+            if not torch.equal(wf[fin], torch.round(wf[fin])):
+                raise ValueError("weights must be integers (the path computes in int32)")
+            w = torch.where(fin, wf, torch.full_like(wf, INF)).to(torch.int64)
+        w = w.to(torch.int64)
+        if w.numel() and (int(w.min()) <= -INF or int(w.max()) > INF):
+            raise ValueError("weights must lie in (-2^31+1, 2^31-1]; 2^31-1 means no arc")
+        if w.shape != (self.m,):
+            raise ValueError(f"expected {self.m} original arc weights, got {tuple(w.shape)}")
+        return w.to(self.dev, torch.int32).contiguous()
+
+    def customize(self, w, out: torch.Tensor | None = None, stream=None) -> torch.Tensor:
+        """Full customization -> ud [mp, 2] int32 (up, down)."""
+        w = self.weights(w) if not (isinstance(w, torch.Tensor) and w.is_cuda and w.dtype == torch.int32) else w
+        if out is None:
+            out = torch.empty((self.mp, 2), dtype=torch.int32, device=self.dev)
+        assert out.shape == (self.mp, 2) and out.dtype == torch.int32 and out.is_contiguous()
+        check(lib().nwcch_customize(self._h, ptr(w), ptr(out), _stream(stream)), "nwcch_customize")
+        return out
+
+    def negcycle_scan(self, ud: torch.Tensor, out: torch.Tensor | None = None, stream=None):
+        """Device int64[2] = (count, first witness or -1); not synchronised."""
+        out = self._scan_out if out is None else out
+        check(lib().nwcch_negcycle_scan(ptr(ud), ud.shape[0], ptr(out), _stream(stream)), "nwcch_negcycle_scan")
+        return out
+
+    def negcycle_list(self, ud: torch.Tensor, stream=None) -> list[int]:
+        """All witness arcs, ascending (python/cch.py:287)."""
+        cnt, _ = (int(v) for v in self.negcycle_scan(ud, stream=stream).cpu())
+        if cnt == 0:
+            return []
+        lst = torch.empty(cnt, dtype=torch.int64, device=self.dev)
+        num = torch.empty(1, dtype=torch.int64, device=self.dev)
+        check(lib().nwcch_negcycle_list(ptr(ud), ud.shape[0], ptr(lst), cnt, ptr(num), _stream(stream)),
+              "nwcch_negcycle_list")
+        return sorted(lst.cpu().tolist())
+
+    def query(self, ud: torch.Tensor, s_rank, t_rank, want_scanned=True, stream=None):
+        """ET queries between ranks -> (dist int32 [nq], scanned int32 [nq] or None)."""
+        qs = torch.as_tensor(s_rank).to(self.dev, torch.int32).contiguous()
+        qt = torch.as_tensor(t_rank).to(self.dev, torch.int32).contiguous()
+        if qs.shape != qt.shape or qs.dim() != 1:
+            raise ValueError("s_rank and t_rank must be 1-d of equal length")
+        nq = qs.shape[0]
+        dist = torch.empty(nq, dtype=torch.int32, device=self.dev)
+        sc = torch.empty(nq, dtype=torch.int32, device=self.dev) if want_scanned else None
+        check(lib().nwcch_query(self._h, ptr(ud), ptr(qs), ptr(qt), nq, ptr(dist), ptr(sc), _stream(stream)),
+              "nwcch_query")
+        return dist, sc
+
+    # -------------------------------------------------------------- profiling
+    def profile_begin(self, max_launches: int):
+        self._prof_cap = int(max_launches)
+        check(lib().nwcch_profile_begin(self._h, self._prof_cap), "nwcch_profile_begin")
+
+    def profile_end(self):
+        """(kernel ids int32[], durations ms float32[]) of the launches since
+        profile_begin (synchronises)."""
+        cap = getattr(self, "_prof_cap", 0)
+        kid = np.zeros(max(cap, 1), np.int32)
+        ms = np.zeros(max(cap, 1), np.float32)
+        cnt = np.zeros(1, np.int64)
+        check(lib().nwcch_profile_end(self._h, ptr(kid), ptr(ms), cap, ptr(cnt)), "nwcch_profile_end")
+        k = min(int(cnt[0]), cap)
+        return kid[:k], ms[:k]
+
+    # ---------------------------------------------------------------- status
+    def status(self, clear=True, stream=None) -> int:
+        st = np.zeros(1, np.uint32)
+        check(lib().nwcch_status(self._h, ptr(st), int(clear), _stream(stream)), "nwcch_status")
+        return int(st[0])
+
+    def check_status(self, stream=None):
+        st = self.status(clear=True, stream=stream)
+        if st:
+            bits = [v for k, v in _lib.STATUS_BITS.items() if st & k]
+            raise NwcchError("device status: " + ", ".join(bits))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().nwcch_dev_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,409 @@
+"""Seeded synthetic road graphs with int32 energy weights.
+
+Every generator returns a :class:`RoadGraph`: a directed CSR (``off`` int64[n+1],
+``tgt`` int32[m], arcs of one tail sorted by head), int32 arc weights ``w[m]`` and
+vertex coordinates in metres.  Weights follow one energy formula::
+
+    w(u, v) = round(alpha * euclid_len_m(u, v)) + slope * (h(v) - h(u))
+
+with integer heights ``h`` in metres.  The height term is a potential difference,
+so every directed cycle has length ``sum(round(alpha * len)) > 0``: the metric is
+conservative while a large share of the arcs is negative.  This is the same role
+as the EV-energy generators of the reference (python/graphs.py:63-132), with
+integer arithmetic instead of float64.
+
+The arc order is the reference's ``g.adj`` order restated for a CSR: arc ``k`` of
+``off[u]..off[u+1]`` is the k-th out-arc of ``u`` (python/cch.py:257 enumerates
+edge weights in exactly this order), so an "edge index" of
+``CCH.update_edges`` (python/cch.py:349) is an index into ``tgt``/``w``.
+"""
+from __future__ import annotations
+
+import dataclasses
+
+import numpy as np
+
+ALPHA = 40      # energy per metre of road
+SLOPE = 25      # energy per metre of climb (regenerated on descent)
+
+
+@dataclasses.dataclass
+class RoadGraph:
+    n: int
+    off: np.ndarray          # int64[n+1]
+    tgt: np.ndarray          # int32[m]
+    w: np.ndarray            # int32[m]
+    x: np.ndarray            # float64[n], metres
+    y: np.ndarray            # float64[n], metres
+    h: np.ndarray | None = None   # int32[n] heights, metres
+    name: str = ""
+    extra: dict = dataclasses.field(default_factory=dict)
+
+    @property
+    def m(self) -> int:
+        return int(self.tgt.shape[0])
+
+    def tails(self) -> np.ndarray:
+        return np.repeat(np.arange(self.n, dtype=np.int32), np.diff(self.off))
+
+    def negative_fraction(self) -> float:
+        return float((self.w < 0).mean()) if self.m else 0.0
+
+    def adj(self):
+        """Adjacency lists ``[[(v, w), ...], ...]`` (the reference's ``Graph.adj``)."""
+        out = []
+        for u in range(self.n):
+            s, e = int(self.off[u]), int(self.off[u + 1])
+            out.append(list(zip(self.tgt[s:e].tolist(), self.w[s:e].tolist())))
+        return out
+
+    def with_weights(self, w: np.ndarray) -> "RoadGraph":
+        w = np.asarray(w, dtype=np.int32)
+        assert w.shape == self.w.shape
+        return dataclasses.replace(self, w=w)
+
+
+def from_arcs(n: int, arcs, xy=None, name="") -> RoadGraph:
+    """Build a RoadGraph from ``[(u, v, w), ...]`` keeping the given per-tail order
+    (the reference's ``make_graph`` appends arcs in list order, test_techniques.py:37-41)."""
+    arcs = list(arcs)
+    tails = np.array([a[0] for a in arcs], dtype=np.int64)
+    order = np.argsort(tails, kind="stable")
+    off = np.zeros(n + 1, np.int64)
+    np.add.at(off, tails + 1, 1)
+    off = np.cumsum(off)
+    tgt = np.array([arcs[i][1] for i in order], dtype=np.int32).reshape(-1)
+    w = np.array([arcs[i][2] for i in order], dtype=np.int32).reshape(-1)
+    if xy is None:
+        x = np.arange(n, dtype=np.float64)
+        y = np.zeros(n, np.float64)
+    else:
+        x = np.array([p[0] for p in xy], np.float64)
+        y = np.array([p[1] for p in xy], np.float64)
+    return RoadGraph(n, off, tgt, w, x, y, None, name)
+
+
+def _csr_from_pairs(n, u, v):
+    """Sort directed pairs by (u, v) and return off, tgt, permutation."""
+    key = u.astype(np.int64) * n + v.astype(np.int64)
+    perm = np.argsort(key, kind="stable")
+    u_s = u[perm]
+    off = np.zeros(n + 1, np.int64)
+    np.add.at(off, u_s.astype(np.int64) + 1, 1)
+    return np.cumsum(off), v[perm].astype(np.int32), perm
+
+
+def _largest_scc(n, u, v):
+    from scipy.sparse import csr_matrix
+    from scipy.sparse.csgraph import connected_components
+    a = csr_matrix((np.ones(len(u), np.int8), (u, v)), shape=(n, n))
+    _, lab = connected_components(a, directed=True, connection="strong")
+    big = np.argmax(np.bincount(lab))
+    keep = lab == big
+    newid = -np.ones(n, np.int64)
+    newid[keep] = np.arange(int(keep.sum()))
+    sel = keep[u] & keep[v]
+    return keep, newid, sel
+
+
+def _energy(x, y, h, u, v, alpha=ALPHA, slope=SLOPE):
+    ln = np.hypot(x[v] - x[u], y[v] - y[u])
+    w = np.rint(alpha * ln).astype(np.int64) + slope * (h[v].astype(np.int64) - h[u].astype(np.int64))
+    assert np.abs(w).max(initial=0) < 2 ** 30, "weights must stay well inside int32"
+    return w.astype(np.int32)
+
+
+# ------------------------------------------------------------------ config 0
+
+def grid_graph(side=64, seed=0, drop=0.10, bumps=8, bump_sigma=5.0, hmax=400,
+               spacing=1.0, jitter=0.3, alpha=ALPHA, slope=SLOPE) -> RoadGraph:
+    """side x side 4-neighbour grid, unit spacing in metres, coordinates jittered by
+    U(-jitter, jitter), 10% of the undirected edges dropped, largest SCC kept.
+    Heights: sum of ``bumps`` seeded Gaussian bumps (sigma ``bump_sigma`` grid units)
+    rounded to integer metres and clipped to [0, hmax]."""
+    rng = np.random.default_rng(seed)
+    n0 = side * side
+    ii, jj = np.meshgrid(np.arange(side), np.arange(side), indexing="xy")
+    x = (ii.reshape(-1) + rng.uniform(-jitter, jitter, n0)) * spacing
+    y = (jj.reshape(-1) + rng.uniform(-jitter, jitter, n0)) * spacing
+    ids = np.arange(n0).reshape(side, side)
+    eu = np.concatenate([ids[:, :-1].reshape(-1), ids[:-1, :].reshape(-1)])
+    ev = np.concatenate([ids[:, 1:].reshape(-1), ids[1:, :].reshape(-1)])
+    keep = rng.random(eu.shape[0]) >= drop
+    eu, ev = eu[keep], ev[keep]
+    cx = rng.uniform(0, side, bumps) * spacing
+    cy = rng.uniform(0, side, bumps) * spacing
+    amp = rng.uniform(0.4, 1.0, bumps) * hmax
+    hf = np.zeros(n0)
+    for k in range(bumps):
+        hf += amp[k] * np.exp(-((x - cx[k]) ** 2 + (y - cy[k]) ** 2) / (2 * (bump_sigma * spacing) ** 2))
+    h = np.clip(np.rint(hf), 0, hmax).astype(np.int32)
+    u = np.concatenate([eu, ev])
+    v = np.concatenate([ev, eu])
+    keepv, newid, sel = _largest_scc(n0, u, v)
+    u, v = newid[u[sel]], newid[v[sel]]
+    x, y, h = x[keepv], y[keepv], h[keepv]
+    n = int(keepv.sum())
+    off, tgt, perm = _csr_from_pairs(n, u, v)
+    tails = np.repeat(np.arange(n), np.diff(off))
+    w = _energy(x, y, h, tails, tgt.astype(np.int64), alpha, slope)
+    g = RoadGraph(n, off, tgt, w, x, y, h, f"grid{side}_s{seed}")
+    g.extra.update(dict(bump_sigma=bump_sigma, spacing=spacing, drop=drop))
+    return g
+
+
+# ------------------------------------------------------------- configs 1-4
+
+def fractal_height(rng, box, hmax, res=512, beta=3.6):
+    """Seeded smooth fractal field on a res x res lattice over [0, box]^2 (spectral
+    synthesis with a k^-beta power spectrum), scaled to [0, hmax]."""
+    kx = np.fft.fftfreq(res)[:, None]
+    ky = np.fft.rfftfreq(res)[None, :]
+    k = np.hypot(kx, ky)
+    k[0, 0] = 1.0
+    amp = k ** (-beta / 2.0)
+    amp[0, 0] = 0.0
+    spec = amp * (rng.standard_normal(amp.shape) + 1j * rng.standard_normal(amp.shape))
+    f = np.fft.irfft2(spec, s=(res, res))
+    f = (f - f.min()) / (f.max() - f.min()) * hmax
+    return f
+
+
+def sample_field(f, box, x, y):
+    res = f.shape[0]
+    gx = np.clip(x / box * (res - 1), 0, res - 1 - 1e-9)
+    gy = np.clip(y / box * (res - 1), 0, res - 1 - 1e-9)
+    i0 = gx.astype(np.int64)
+    j0 = gy.astype(np.int64)
+    fx, fy = gx - i0, gy - j0
+    v = (f[i0, j0] * (1 - fx) * (1 - fy) + f[i0 + 1, j0] * fx * (1 - fy)
+         + f[i0, j0 + 1] * (1 - fx) * fy + f[i0 + 1, j0 + 1] * fx * fy)
+    return v
+
+
+def road_graph(n_points, seed, box=60000.0, clusters=200, sigma=2000.0, out_degree=2.8,
+               oneway=0.05, hmax=800, alpha=ALPHA, slope=SLOPE, neg_share=0.10,
+               name=None) -> RoadGraph:
+    """Road-like graph: a seeded clustered Poisson (Thomas) point process, its
+    Delaunay triangulation thinned to a mean out-degree of ``out_degree`` (Euclidean
+    MST plus random Delaunay edges, so the result stays connected), ``oneway`` of
+    the kept streets one-way, largest SCC kept, integer fractal heights."""
+    from scipy.spatial import Delaunay
+    from scipy.sparse import coo_matrix
+    from scipy.sparse.csgraph import minimum_spanning_tree
+
+    rng = np.random.default_rng(seed)
+    centers = rng.uniform(0, box, (clusters, 2))
+    pts = np.empty((0, 2))
+    need = n_points
+    while need > 0:
+        c = centers[rng.integers(0, clusters, need)]
+        p = c + rng.normal(0.0, sigma, (need, 2))
+        p = p[(p >= 0).all(1) & (p <= box).all(1)]
+        pts = np.concatenate([pts, p])
+        need = n_points - pts.shape[0]
+    pts = pts[:n_points]
+    pts, uniq = np.unique(np.round(pts, 3), axis=0, return_index=True)
+    pts = pts[rng.permutation(pts.shape[0])]
+    n0 = pts.shape[0]
+    tri = Delaunay(pts).simplices
+    e = np.concatenate([tri[:, [0, 1]], tri[:, [1, 2]], tri[:, [2, 0]]])
+    e.sort(axis=1)
+    e = np.unique(e, axis=0)
+    ln = np.hypot(*(pts[e[:, 0]] - pts[e[:, 1]]).T)
+    mst = minimum_spanning_tree(coo_matrix((ln + 1e-9, (e[:, 0], e[:, 1])), shape=(n0, n0))).tocoo()
+    mst_e = np.stack([np.minimum(mst.row, mst.col), np.maximum(mst.row, mst.col)], 1)
+    key_all = e[:, 0].astype(np.int64) * n0 + e[:, 1]
+    key_mst = mst_e[:, 0].astype(np.int64) * n0 + mst_e[:, 1]
+    rest = e[~np.isin(key_all, key_mst)]
+    # target undirected edge count: arcs = E * (2 - oneway) = out_degree * n
+    target = int(round(out_degree * n0 / (2.0 - oneway)))
+    extra = max(0, min(rest.shape[0], target - mst_e.shape[0]))
+    pick = rest[rng.choice(rest.shape[0], extra, replace=False)]
+    und = np.concatenate([mst_e, pick])
+    one = rng.random(und.shape[0]) < oneway
+    flip = rng.random(und.shape[0]) < 0.5
+    a, b = und[:, 0], und[:, 1]
+    a1 = np.where(flip, b, a)
+    b1 = np.where(flip, a, b)
+    u = np.concatenate([a1, b1[~one]])
+    v = np.concatenate([b1, a1[~one]])
+    keepv, newid, sel = _largest_scc(n0, u, v)
+    u, v = newid[u[sel]], newid[v[sel]]
+    pts = pts[keepv]
+    n = int(keepv.sum())
+    off, tgt, perm = _csr_from_pairs(n, u, v)
+    tails = np.repeat(np.arange(n), np.diff(off))
+    x, y = pts[:, 0].copy(), pts[:, 1].copy()
+    hf = sample_field(fractal_height(rng, box, hmax), box, x, y)
+    dz = vertical_scale(x, y, hf, tails, tgt, alpha, slope, neg_share)
+    h = np.rint(hf * dz).astype(np.int32)
+    w = _energy(x, y, h, tails, tgt.astype(np.int64), alpha, slope)
+    g = RoadGraph(n, off, tgt, w, x, y, h, name or f"road{n_points}_s{seed}")
+    g.extra.update(dict(box=box, clusters=clusters, sigma=sigma, hmax=hmax, dz_scale=dz))
+    return g
+
+
+def vertical_scale(x, y, hf, tails, heads, alpha, slope, neg_share=0.10):
+    """Vertical exaggeration of a height field so that about ``neg_share`` of the arcs
+    come out negative.  This is the reference's rule for its ``ev`` metric on road
+    graphs (python/dimacs.py:155-160, python/bigcsr.py:155-161: the 80th percentile
+    of downhill energy slopes is scaled to 1).  A literal 0-800 m field over a
+    60 km box gives grades far below alpha/slope = 1.6 and no negative arc at all."""
+    ln = np.hypot(x[heads] - x[tails], y[heads] - y[tails])
+    drop = hf[tails] - hf[heads]
+    sel = (drop > 0) & (ln > 0)
+    if not sel.any():
+        return 1.0
+    s = slope * drop[sel] / (alpha * ln[sel])
+    q = np.quantile(s, 1.0 - 2.0 * neg_share)
+    return float(1.0 / q) if q > 0 else 1.0
+
+
+def metric_family(g: RoadGraph, K: int, seed: int, hmax=1200, slope_range=(5, 40),
+                  box=None, alpha=ALPHA, neg_share=0.10):
+    """K metrics on one topology: metric k has its own slope coefficient drawn from
+    ``slope_range`` and its own seeded fractal height field in [0, hmax].
+    Returns int32 [K, m] weights and the slopes (vertical scale as in
+    :func:`vertical_scale`)."""
+    rng = np.random.default_rng(seed)
+    box = box or g.extra.get("box", float(max(g.x.max(), g.y.max())))
+    tails = g.tails().astype(np.int64)
+    heads = g.tgt.astype(np.int64)
+    W = np.empty((K, g.m), np.int32)
+    slopes = rng.integers(slope_range[0], slope_range[1] + 1, K)
+    dzs = np.empty(K)
+    for k in range(K):
+        hf = sample_field(fractal_height(rng, box, hmax), box, g.x, g.y)
+        dzs[k] = vertical_scale(g.x, g.y, hf, tails, heads, alpha, int(slopes[k]), neg_share)
+        h = np.rint(hf * dzs[k]).astype(np.int32)
+        W[k] = _energy(g.x, g.y, h, tails, heads, alpha, int(slopes[k]))
+    return W, slopes
+
+
+def find_directed_triangle(g: RoadGraph, rng=None):
+    """Arc indices (k1, k2, k3) of a directed 3-cycle u->v->w->u, or None."""
+    rng = rng or np.random.default_rng(0)
+    off, tgt = g.off, g.tgt
+    start = int(rng.integers(0, g.n))
+    for du in range(g.n):
+        uu = (start + du) % g.n
+        for k1 in range(off[uu], off[uu + 1]):
+            v = int(tgt[k1])
+            for k2 in range(off[v], off[v + 1]):
+                ww = int(tgt[k2])
+                if ww == uu:
+                    continue
+                for k3 in range(off[ww], off[ww + 1]):
+                    if int(tgt[k3]) == uu:
+                        return int(k1), int(k2), int(k3)
+    return None
+
+
+def inject_negative_cycle(w: np.ndarray, arcs, total=-1):
+    """Set the weights of the given cycle arcs so that they sum to ``total``."""
+    w = np.array(w, dtype=np.int32, copy=True)
+    k1, k2, k3 = arcs
+    w[k1], w[k2] = 0, 0
+    w[k3] = total
+    return w
+
+
+CONFIGS = {
+    0: dict(kind="grid", side=64, seed=0),
+    1: dict(kind="road", n_points=264_000, seed=1, hmax=800),
+    2: dict(kind="road", n_points=321_000, seed=2, hmax=800),
+    3: dict(kind="road", n_points=1_100_000, seed=3, hmax=800),
+    4: dict(kind="road", n_points=23_900_000, seed=4, hmax=3000),
+}
+
+
+def make_config(idx: int) -> RoadGraph:
+    c = dict(CONFIGS[idx])
+    kind = c.pop("kind")
+    if kind == "grid":
+        return grid_graph(**c)
+    n_points = c.pop("n_points")
+    return road_graph(n_points, name=f"c{idx}", **c)
+
+
+def random_pairs(n, count, seed):
+    rng = np.random.default_rng(seed)
+    return rng.integers(0, n, count).astype(np.int32), rng.integers(0, n, count).astype(np.int32)
+
+
+# ------------------------------------------------------- small test families
+
+def ev_grid(side=15, seed=0, alpha=10, up=10, down=6):
+    """side x side 8-connected grid over random bump terrain, integer EV energy
+    ``alpha*len + (up if climbing else down)*dz`` (the shape of the reference's
+    python/graphs.py:63-79 generator; regenerative braking recovers less than
+    climbing costs, so every cycle is positive while steep descents are negative)."""
+    rng = np.random.default_rng(seed)
+    n = side * side
+    x = (np.arange(n) % side).astype(np.float64)
+    y = (np.arange(n) // side).astype(np.float64)
+    z = np.zeros(n)
+    for _ in range(12):
+        cx, cy = rng.uniform(0, side, 2)
+        amp = rng.uniform(-0.6, 0.6) * side
+        rad = rng.uniform(0.05, 0.2) * side
+        z += amp * np.exp(-((x - cx) ** 2 + (y - cy) ** 2) / (2 * rad * rad))
+    z = np.rint(z * 4).astype(np.int64)
+    arcs = []
+    for v in range(n):
+        i, j = v % side, v // side
+        for dx, dy in ((1, 0), (-1, 0), (0, 1), (0, -1), (1, 1), (1, -1), (-1, 1), (-1, -1)):
+            a, b = i + dx, j + dy
+            if 0 <= a < side and 0 <= b < side:
+                u = b * side + a
+                dz = int(z[u] - z[v])
+                ln = int(round(alpha * 4 * np.hypot(dx, dy)))
+                arcs.append((v, u, ln + (up if dz > 0 else down) * dz))
+    g = from_arcs(n, arcs, xy=list(zip(x, y)), name=f"evgrid{side}_s{seed}")
+    g.h = z.astype(np.int32)
+    return g
+
+
+def ev_geometric(n=300, seed=0, k=3, alpha=100, up=10, down=6):
+    """Random points in the unit square joined to their k nearest neighbours in
+    both directions, integer EV energy (cf. python/graphs.py:82-132)."""
+    rng = np.random.default_rng(seed)
+    pts = rng.random((n, 2))
+    z = np.zeros(n)
+    for _ in range(8):
+        c = rng.random(2)
+        z += rng.uniform(-1, 1) * np.exp(-((pts - c) ** 2).sum(1) / (2 * rng.uniform(0.05, 0.2) ** 2))
+    z = np.rint(z * 60).astype(np.int64)
+    d = ((pts[:, None, :] - pts[None, :, :]) ** 2).sum(-1)
+    np.fill_diagonal(d, np.inf)
+    nn = np.argsort(d, axis=1)[:, :k]
+    pairs = set()
+    for v in range(n):
+        for u in nn[v]:
+            pairs.add((v, int(u)))
+            pairs.add((int(u), v))
+    arcs = []
+    for v, u in sorted(pairs):
+        dz = int(z[u] - z[v])
+        ln = int(round(alpha * np.sqrt(d[v, u])))
+        arcs.append((v, u, ln + (up if dz > 0 else down) * dz))
+    g = from_arcs(n, arcs, xy=[tuple(p) for p in pts], name=f"geo{n}_s{seed}")
+    g.h = z.astype(np.int32)
+    return g
+
+
+def shifted_random(n=200, seed=0, deg=3, cmax=100, spread=300):
+    """Random sparse digraph with weights c + phi(u) - phi(v), c > 0: every cycle is
+    positive, many arcs are negative (python/graphs.py:135-151).  No geometry:
+    coordinates are random, as the reference's test does (test_cch.py:35)."""
+    rng = np.random.default_rng(seed)
+    phi = rng.integers(0, spread, n)
+    arcs = []
+    for v in range(n):
+        for _ in range(deg):
+            u = int(rng.integers(0, n))
+            if u != v:
+                arcs.append((v, u, int(rng.integers(1, cmax)) + int(phi[v] - phi[u])))
+    pts = rng.random((n, 2))
+    return from_arcs(n, arcs, xy=[tuple(p) for p in pts], name=f"shift{n}_s{seed}")

@@ -1,0 +1,164 @@
+"""GPU parity of the HIP path against the CPU oracle: customization, the
+negative-cycle scan and ET queries must be bit-identical (integer work)."""
+import numpy as np
+import pytest
+import torch
+
+from conftest import order_of, small_graphs
+from nwcch import _lib, cch_nb, graphs
+from nwcch.device import DeviceTopology
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+INF = O.INF
+
+
+def _setup(g, rank):
+    topo = _lib.Topology.from_graph(g.n, g.off, g.tgt, rank)
+    dt = DeviceTopology(topo)
+    st = O.Structure(g.n, g.off, g.tgt, rank)
+    return dt, st
+
+
+def _check_customize(dt, st, w):
+    ud = dt.customize(w)
+    up, dn, ovf = st.customize(w)
+    got = ud.cpu().numpy()
+    assert np.array_equal(got[:, 0], up), "up differs"
+    assert np.array_equal(got[:, 1], dn), "down differs"
+    assert dt.status() == 0 and not ovf
+    cnt, first = (int(v) for v in dt.negcycle_scan(ud).cpu())
+    assert (cnt, first) == st.negcycle(up, dn)
+    return ud, up, dn
+
+
+def test_customize_scan_query_c0(c0):
+    g, rank = c0
+    print("case: config 0 grid", g.n, g.m, flush=True)
+    dt, st = _setup(g, rank)
+    ud, up, dn = _check_customize(dt, st, g.w)
+    s, t = graphs.random_pairs(g.n, 1000, seed=2)
+    dist, sc = dt.query(ud, rank[s], rank[t])
+    d0, s0, _ = st.query(up, dn, s, t)
+    assert np.array_equal(dist.cpu().numpy(), d0)
+    assert np.array_equal(sc.cpu().numpy().astype(np.int64), s0)
+    # against Bellman-Ford on a sample (the config-0 acceptance check)
+    for i in range(0, 1000, 50):
+        bf, _ = O.bellman_ford(g.n, g.off, g.tgt, g.w, int(s[i]))
+        assert dist[i].item() == bf[t[i]]
+    # s == t and the scan bound
+    dist2, _ = dt.query(ud, rank[:64], rank[:64])
+    assert (dist2 == 0).all()
+    assert int(sc.max()) <= st.depth * (st.depth + 1)
+
+
+@pytest.mark.parametrize("idx", range(12))
+def test_small_family(idx):
+    g = small_graphs()[idx]
+    print("case: small family", idx, g.name, flush=True)
+    rank = order_of(g)
+    dt, st = _setup(g, rank)
+    ud, up, dn = _check_customize(dt, st, g.w)
+    qs = np.repeat(np.arange(g.n, dtype=np.int32), 3)
+    qt = np.random.default_rng(idx).integers(0, g.n, len(qs)).astype(np.int32)
+    dist, sc = dt.query(ud, rank[qs], rank[qt])
+    d0, s0, _ = st.query(up, dn, qs, qt)
+    assert np.array_equal(dist.cpu().numpy(), d0)
+    assert np.array_equal(sc.cpu().numpy().astype(np.int64), s0)
+
+
+def test_road_config1_bit_exact():
+    g = graphs.make_config(1)
+    print("case: config 1 road graph", g.n, g.m, flush=True)
+    rank = _lib.order(g.n, g.off, g.tgt, g.x, g.y, 64)
+    dt, st = _setup(g, rank)
+    ud, up, dn = _check_customize(dt, st, g.w)
+    s, t = graphs.random_pairs(g.n, 2000, seed=11)
+    dist, sc = dt.query(ud, rank[s], rank[t])
+    d0, s0, _ = st.query(up, dn, s, t, threads=8)
+    assert np.array_equal(dist.cpu().numpy(), d0)
+    assert np.array_equal(sc.cpu().numpy().astype(np.int64), s0)
+    p, neg = O.johnson(g.n, g.off, g.tgt, g.w)
+    assert not neg
+    dj = O.Dijkstra(g.n, g.off, g.tgt, g.w, p)
+    for i in range(20):
+        assert dist[i].item() == dj(s[i], t[i])
+
+
+def test_injected_negative_cycle_is_flagged(c0):
+    """A 3-arc loop of total -1 (the batched config's injected cycle) is found by
+    the scan, with the oracle's count and first witness."""
+    g, rank = c0
+    tri = graphs.find_directed_triangle(g)
+    if tri is None:  # the grid has no directed triangles: use a 2-cycle
+        k = int(g.off[10])
+        v = int(g.tgt[k])
+        back = [j for j in range(g.off[v], g.off[v + 1]) if g.tgt[j] == 10][0]
+        w = g.w.copy()
+        w[k], w[back] = 0, -1
+    else:
+        w = graphs.inject_negative_cycle(g.w, tri)
+    dt, st = _setup(g, rank)
+    ud, up, dn = _check_customize(dt, st, w)
+    cnt = int(dt.negcycle_scan(ud)[0])
+    assert cnt > 0 and O.has_negative_cycle(g.n, g.off, g.tgt, w)
+    lst = dt.negcycle_list(ud)
+    want = np.nonzero((up != INF) & (dn != INF) & (up.astype(np.int64) + dn < 0))[0]
+    assert lst == want.tolist()
+
+
+def test_unreachable_pairs_and_inf():
+    """One-way arcs leave pairs unreachable: INF must absorb negative arcs."""
+    g = graphs.from_arcs(5, [(0, 1, 3), (1, 2, -2), (3, 4, -7), (4, 3, 9), (2, 0, 4)])
+    rank = np.array([4, 0, 2, 1, 3], np.int32)
+    dt, st = _setup(g, rank)
+    ud, up, dn = _check_customize(dt, st, g.w)
+    s = np.array([0, 0, 3, 1, 4], np.int32)
+    t = np.array([2, 3, 4, 0, 0], np.int32)
+    dist, _ = dt.query(ud, rank[s], rank[t])
+    assert dist.cpu().tolist() == [1, INF, -7, 2, INF]
+
+
+def test_overflow_is_reported_not_saturated():
+    g = graphs.from_arcs(3, [(0, 1, 2 ** 30 + 5), (1, 2, 2 ** 30 + 5)])
+    rank = np.array([1, 0, 2], np.int32)
+    topo = _lib.Topology.from_graph(g.n, g.off, g.tgt, rank)
+    dt = DeviceTopology(topo)
+    dt.customize(g.w)
+    assert dt.status() & 1
+    st = O.Structure(g.n, g.off, g.tgt, rank)
+    assert st.customize(g.w)[2]
+
+
+def test_bad_query_vertex_sets_status(c0):
+    g, rank = c0
+    dt, st = _setup(g, rank)
+    ud = dt.customize(g.w)
+    dist, _ = dt.query(ud, [0, g.n + 5], [1, 2])
+    assert dist[1].item() == INF
+    with pytest.raises(_lib.NwcchError, match="out of range"):
+        dt.check_status()
+
+
+def test_cch_nb_dropin(c0):
+    """The python/cch_nb.py call sequence of run_nb.py:79-141, on the GPU."""
+    g, _ = c0
+    noff, nbr, rev = cch_nb.undirected_csr(g.n, g.off, g.tgt)
+    rank = cch_nb.nested_dissection(g.n, noff, nbr, rev, g.x, g.y, 64)
+    first, head, parent = cch_nb.symbolic(g.n, noff, nbr, rank)
+    e_arc, e_up = cch_nb.edge_arcs(g.n, g.off, g.tgt, rank, first, head)
+    upw, dnw = cch_nb.customize(g.n, first, head, e_arc, e_up, g.w.astype(np.float64))
+    assert cch_nb.negative_cycle_arcs(upw, dnw) == 0
+    st = O.Structure(g.n, g.off, g.tgt, rank.astype(np.int32))
+    up, dn, _ = st.customize(g.w)
+    assert np.array_equal(upw.cpu().numpy(), up) and np.array_equal(dnw.cpu().numpy(), dn)
+    df = np.full(g.n, np.inf)
+    s, t = graphs.random_pairs(g.n, 20, seed=7)
+    d0, _, _ = st.query(up, dn, s, t)
+    for i in range(20):
+        got = cch_nb.query(rank[s[i]], rank[t[i]], first, head, parent, upw, dnw, df, df, None, i + 1)
+        assert got == (np.inf if d0[i] == INF else d0[i])
+    # separate (non-view) arrays go through an interleaving copy
+    assert cch_nb.negative_cycle_arcs(upw.clone(), dnw.clone()) == 0
+    dist, _ = cch_nb.query_many(rank[s], rank[t], first, head, upw, dnw)
+    assert np.array_equal(dist.cpu().numpy(), d0)
