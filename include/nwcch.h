@@ -123,6 +123,51 @@ int nwcch_negcycle_list(const int32_t* ud, int64_t mp, int64_t* list, int64_t ca
 int nwcch_query(nwcch_dev* h, const int32_t* ud, const int32_t* s_rank, const int32_t* t_rank,
                 int64_t nq, int32_t* dist, int32_t* scanned, void* stream);
 
+/* The same on a metric stored with a stride: arc a's (up, down) pair is at
+ * ud + 2 * a * ud_stride (ud_stride = K and ud = UD + 2k for metric k of a
+ * batched [mp][K][2] customization). */
+int nwcch_query_strided(nwcch_dev* h, const int32_t* ud, int64_t ud_stride, const int32_t* s_rank,
+                        const int32_t* t_rank, int64_t nq, int32_t* dist, int32_t* scanned, void* stream);
+
+/* Batched customization of K metrics on one topology in one level sweep:
+ * W[K][m] weights (device, metric-major) -> UD[mp][K][2] (device, arc-major:
+ * the K pairs of one arc are contiguous).  Equals K calls of nwcch_customize,
+ * i.e. K runs of cch_nb.customize (python/cch_nb.py:529-563) or of
+ * CCH::customize (cpp/cch.cpp:437-463), which the reference runs one metric at a
+ * time (python/run_nb.py:121-124; cpp/cch.cpp:805-816). */
+int nwcch_customize_batch(nwcch_dev* h, const int32_t* W, int32_t K, int32_t* UD, void* stream);
+
+/* Per-metric negative-cycle scan of UD[mp][K][2]: out (device int64[2K]) =
+ * {count of metric 0..K-1, smallest witness arc of metric 0..K-1 or -1}
+ * (python/cch_nb.py:566-572 per metric). */
+int nwcch_negcycle_scan_batch(const int32_t* UD, int64_t mp, int32_t K, int64_t* out, void* stream);
+
+/* Build (once, on the host) and upload the partial-update index of the topology
+ * the handle was created from: for every arc the top arcs of the lower
+ * triangles it is a side of, arc heights and frontier buckets.  The lazy
+ * build_update_index of python/cch.py:289-347 (by_c, as_side, tail,
+ * arc_edges). */
+int nwcch_update_prepare(nwcch_dev* h, nwcch_topo* t);
+
+/* Partial re-customization of one metric in place.  w[m] and ud[mp][2] are
+ * the metric's device weights and its current customization; the k changes
+ * (edge_idx[i], new_w[i]) are HOST arrays of distinct original-arc indices (the
+ * order of python/cch.py:257) and their new weights.  Writes w, re-evaluates
+ * only the arcs whose value can change, lowest first, and updates ud.
+ * result (device int64[4]) = {arcs evaluated, arcs changed, changed arcs with
+ * up + down < 0, smallest such arc or -1}; the changed arcs (unordered) go to
+ * `changed` (device int32, capacity changed_cap) or, if NULL, to an internal
+ * buffer read by nwcch_update_changed.  mode 0 = auto, 1 = one persistent
+ * workgroup, 2 = one launch per ET height.  Updates on one handle must be
+ * issued on one stream.  Replaces CCH.update_edges (python/cch.py:349-402) and
+ * CCH.update_created_negative_cycle (python/cch.py:666-671). */
+int nwcch_update(nwcch_dev* h, int32_t* w, int32_t* ud, const int64_t* edge_idx, const int32_t* new_w,
+                 int64_t k, int32_t mode, int32_t* changed, int64_t changed_cap, int64_t* result,
+                 void* stream);
+
+/* Copy the first `cap` changed arcs of the last update into out (device). */
+int nwcch_update_changed(nwcch_dev* h, int32_t* out, int64_t cap, void* stream);
+
 /* Read (and optionally clear) the sticky status bits of a handle.  Synchronises
  * `stream`.  The reference raises Python exceptions / asserts instead
  * (python/cch.py:125, 218; python/preprocess.py:9-10). */

@@ -209,7 +209,7 @@ __global__ void k_scan_finish(const unsigned long long* src, long long* dst) {
 // its vertices so the t walk can recognise common ancestors.
 __global__ void __launch_bounds__(64) k_query(const VInfo* __restrict__ vinfo,
                                               const int32_t* __restrict__ adep,
-                                              const int2* __restrict__ ud,
+                                              const int2* __restrict__ ud, i64 ud_stride,
                                               const int32_t* __restrict__ qs,
                                               const int32_t* __restrict__ qt, i64 nq, int32_t n,
                                               int32_t maxlen, int32_t* __restrict__ dist,
@@ -257,7 +257,7 @@ __global__ void __launch_bounds__(64) k_query(const VInfo* __restrict__ vinfo,
         for (int32_t j = lane; j < vi.deg; j += 64) {
           const int32_t a = vi.first + j;
           const int32_t pos = ds - adep[a];
-          const int32_t v = inf_add(dx, ud[a].x, st);
+          const int32_t v = inf_add(dx, ud[(i64)a * ud_stride].x, st);
           if (v < df[pos]) df[pos] = v;
         }
       }
@@ -290,7 +290,7 @@ __global__ void __launch_bounds__(64) k_query(const VInfo* __restrict__ vinfo,
         for (int32_t j = lane; j < vi.deg; j += 64) {
           const int32_t a = vi.first + j;
           const int32_t pos = dt - adep[a];
-          const int32_t v = inf_add(dx, ud[a].y, st);
+          const int32_t v = inf_add(dx, ud[(i64)a * ud_stride].y, st);
           if (v < dr[pos]) dr[pos] = v;
         }
       }
@@ -317,15 +317,6 @@ __global__ void __launch_bounds__(64) k_query(const VInfo* __restrict__ vinfo,
 
 using namespace nwcch;
 
-namespace nwcch {
-void set_error(const std::string& s);
-const Topology* topo_of(const nwcch_topo* t);
-Topology* topo_of_mut(nwcch_topo* t);
-}  // namespace nwcch
-
-struct nwcch_dev {
-  DevTopo d;
-};
 
 namespace {
 
@@ -469,9 +460,10 @@ extern "C" int nwcch_negcycle_list(const int32_t* ud, int64_t mp, int64_t* list,
   return NWCCH_OK;
 }
 
-extern "C" int nwcch_query(nwcch_dev* h, const int32_t* ud, const int32_t* s_rank, const int32_t* t_rank,
-                           int64_t nq, int32_t* dist, int32_t* scanned, void* stream) {
-  if (!h || !ud || !s_rank || !t_rank || !dist || nq < 0) {
+extern "C" int nwcch_query_strided(nwcch_dev* h, const int32_t* ud, int64_t ud_stride, const int32_t* s_rank,
+                                   const int32_t* t_rank, int64_t nq, int32_t* dist, int32_t* scanned,
+                                   void* stream) {
+  if (!h || !ud || !s_rank || !t_rank || !dist || nq < 0 || ud_stride < 1) {
     set_error("null argument");
     return NWCCH_ERR_ARG;
   }
@@ -484,13 +476,18 @@ extern "C" int nwcch_query(nwcch_dev* h, const int32_t* ud, const int32_t* s_ran
     if (lds > 160 * 1024) throw std::runtime_error("elimination tree too deep for the LDS query kernel");
     ProfScope ps(d, s, kKQuery);
     hipLaunchKernelGGL(k_query, dim3((unsigned)nq), dim3(64), lds, s, d.vinfo, d.adep,
-                       reinterpret_cast<const int2*>(ud), s_rank, t_rank, (i64)nq, (int32_t)d.n,
-                       maxlen, dist, scanned, d.status);
+                       reinterpret_cast<const int2*>(ud), (i64)ud_stride, s_rank, t_rank, (i64)nq,
+                       (int32_t)d.n, maxlen, dist, scanned, d.status);
     HIP_OK(hipGetLastError());
   } catch (const std::exception& e) {
     return fail(e);
   }
   return NWCCH_OK;
+}
+
+extern "C" int nwcch_query(nwcch_dev* h, const int32_t* ud, const int32_t* s_rank, const int32_t* t_rank,
+                           int64_t nq, int32_t* dist, int32_t* scanned, void* stream) {
+  return nwcch_query_strided(h, ud, 1, s_rank, t_rank, nq, dist, scanned, stream);
 }
 
 extern "C" int nwcch_profile_begin(nwcch_dev* h, int64_t max_launches) {

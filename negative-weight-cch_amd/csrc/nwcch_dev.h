@@ -3,9 +3,12 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <string>
 #include <vector>
 
 #include "nwcch_topo.h"
+
+struct nwcch_topo;
 
 namespace nwcch {
 
@@ -47,6 +50,19 @@ struct DevTopo {
   i64* side_off = nullptr;           // [mp+1]
   int32_t* side_top = nullptr;       // [2T]
   int32_t* lvl_cap_off = nullptr;    // [levels+1] frontier bucket offsets (= arcs per level)
+  int32_t* e_arc32 = nullptr;        // [m] shortcut of each original arc, -1 for loops
+  std::vector<int32_t> e_arc_host;   // [m]
+  std::vector<int32_t> arc_lvl_host; // [mp]
+  // update workspace (one update at a time per handle)
+  int32_t* qstamp = nullptr;         // [mp] epoch at which an arc was queued
+  int32_t* qarcs = nullptr;          // [mp] frontier buckets, bucket L at lvl_cap_off[L]
+  int32_t* qcnt = nullptr;           // [levels] bucket fill
+  int32_t* chg = nullptr;            // [mp] changed arcs of the last update (unordered)
+  long long* upd_res = nullptr;      // [4] evaluated, changed, negative changed, first negative
+  i64* stage_e = nullptr;            // staging of the changes (grown on demand)
+  int32_t* stage_w = nullptr;
+  i64 stage_cap = 0;
+  int32_t epoch = 0;
   // scratch
   uint32_t* status = nullptr;        // [1]
   unsigned long long* scan = nullptr;  // [4]
@@ -81,4 +97,12 @@ struct ProfScope {
   }
 };
 
+void set_error(const std::string& s);
+const Topology* topo_of(const struct ::nwcch_topo* t);
+Topology* topo_of_mut(struct ::nwcch_topo* t);
+
 }  // namespace nwcch
+
+struct nwcch_dev {
+  nwcch::DevTopo d;
+};

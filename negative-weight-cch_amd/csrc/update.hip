@@ -1,0 +1,370 @@
+// Partial re-customization on the GPU (python/cch.py:289-402, 666-671;
+// Theorem 4 of docs/technical_note.md:149-174).
+//
+// The reference pops queued arcs from a heap keyed by the rank of their lower
+// endpoint.  Here the queue is a device-side frontier bucketed by the ET height
+// of the lower endpoint: an arc's inputs all have lower endpoints of smaller
+// height, and an arc whose value changed only enqueues top arcs of triangles it
+// is a side of, whose lower endpoint is strictly higher.  Processing the buckets
+// in increasing height therefore evaluates every queued arc exactly once, after
+// all its inputs are final -- the same set of arcs, the same values and the same
+// changed set as the heap order.  An arc is queued at most once per update
+// (epoch stamps), so bucket L never holds more than the arcs of height L.
+//
+// Two drivers share one arc evaluation: a single persistent workgroup that walks
+// the non-empty buckets with workgroup barriers (small updates, latency bound),
+// and one launch per height with many workgroups (large updates).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "nwcch_dev.h"
+
+namespace nwcch {
+
+#define HIP_OK(expr)                                                              \
+  do {                                                                            \
+    hipError_t e_ = (expr);                                                       \
+    if (e_ != hipSuccess)                                                         \
+      throw std::runtime_error(std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+namespace {
+
+__device__ __forceinline__ int32_t inf_add_u(int32_t a, int32_t b, uint32_t& st) {
+  if (a == kInf || b == kInf) return kInf;
+  const long long s = (long long)a + (long long)b;
+  if (s >= (long long)kInf || s <= -(long long)kInf) {
+    st |= kStatusOverflow;
+    return s > 0 ? kInf - 1 : -kInf + 1;
+  }
+  return (int32_t)s;
+}
+
+struct UpdArgs {
+  const int32_t* ae_off;
+  const int32_t* ae_idx;
+  const int32_t* arc_pos;
+  const i64* tri_off;
+  const int2* tri_ab;
+  const i64* side_off;
+  const int32_t* side_top;
+  const int32_t* arc_lvl;
+  const int32_t* lvl_cap_off;
+  const int32_t* e_arc32;
+  int32_t* w;
+  int2* ud;
+  int32_t* qstamp;
+  int32_t* qarcs;
+  int32_t* qcnt;
+  int32_t* chg;
+  i64 chg_cap;
+  long long* res;  // evaluated, changed, negative changed, first negative changed
+  uint32_t* status;
+  int32_t epoch;
+};
+
+__device__ __forceinline__ void enqueue(const UpdArgs& A, int32_t c) {
+  if (atomicExch(&A.qstamp[c], A.epoch) != A.epoch) {
+    const int32_t L = A.arc_lvl[c];
+    const int32_t slot = atomicAdd(&A.qcnt[L], 1);
+    A.qarcs[A.lvl_cap_off[L] + slot] = c;
+  }
+}
+
+// One wavefront re-evaluates arc c from its original arcs and lower triangles;
+// if the value changed it is stored, recorded, checked for a negative 2-cycle
+// (Corollary 3a) and the top arcs of its side triangles are queued.
+__device__ void wave_update_arc(const UpdArgs& A, int32_t c, int lane, uint32_t& st) {
+  int32_t up = kInf, dn = kInf;
+  const int32_t e0 = A.ae_off[c], e1 = A.ae_off[c + 1];
+  for (int32_t j = e0 + lane; j < e1; j += 64) {
+    const int32_t e = A.ae_idx[j];
+    const int32_t wk = A.w[e >> 1];
+    if (wk == (int32_t)0x80000000) st |= kStatusBadWeight;
+    if (e & 1) up = min(up, wk);
+    else dn = min(dn, wk);
+  }
+  const int32_t p = A.arc_pos[c];
+  const i64 t1 = A.tri_off[p + 1];
+  for (i64 q = A.tri_off[p] + lane; q < t1; q += 64) {
+    const int2 ab = A.tri_ab[q];
+    const int2 X = A.ud[ab.x];
+    const int2 Y = A.ud[ab.y];
+    up = min(up, inf_add_u(X.y, Y.x, st));
+    dn = min(dn, inf_add_u(Y.y, X.x, st));
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    up = min(up, __shfl_xor(up, o, 64));
+    dn = min(dn, __shfl_xor(dn, o, 64));
+  }
+  const int2 old = A.ud[c];
+  if (up == old.x && dn == old.y) return;  // wave-uniform
+  if (lane == 0) {
+    A.ud[c] = make_int2(up, dn);
+    const long long i = atomicAdd((unsigned long long*)&A.res[1], 1ull);
+    if (i < A.chg_cap) A.chg[i] = c;
+    if (up != kInf && dn != kInf && (long long)up + (long long)dn < 0) {
+      atomicAdd((unsigned long long*)&A.res[2], 1ull);
+      atomicMin(&A.res[3], (long long)c);
+    }
+  }
+  const i64 s1 = A.side_off[c + 1];
+  for (i64 j = A.side_off[c] + lane; j < s1; j += 64) enqueue(A, A.side_top[j]);
+}
+
+__global__ void k_upd_reset(UpdArgs A, int32_t levels) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  for (int L = i; L < levels; L += gridDim.x * blockDim.x) A.qcnt[L] = 0;
+  if (i == 0) {
+    A.res[0] = 0;
+    A.res[1] = 0;
+    A.res[2] = 0;
+    A.res[3] = 0x7fffffffffffffffll;
+  }
+}
+
+// write the new weights and queue the touched arcs (python/cch.py:358-362)
+__global__ void k_upd_seed(UpdArgs A, const i64* __restrict__ edge_idx, const int32_t* __restrict__ new_w,
+                           i64 k, i64 m) {
+  const i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= k) return;
+  const i64 e = edge_idx[i];
+  if (e < 0 || e >= m) {
+    atomicOr(A.status, (uint32_t)kStatusBadQuery);
+    return;
+  }
+  A.w[e] = new_w[i];
+  const int32_t a = A.e_arc32[e];
+  if (a >= 0) enqueue(A, a);
+}
+
+__global__ void __launch_bounds__(256) k_upd_level(UpdArgs A, int32_t L) {
+  const int32_t n = A.qcnt[L];
+  const int32_t base = A.lvl_cap_off[L];
+  const int lane = threadIdx.x & 63;
+  const int32_t wave = (int32_t)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  const int32_t nwaves = (int32_t)((gridDim.x * blockDim.x) >> 6);
+  uint32_t st = 0;
+  for (int32_t j = wave; j < n; j += nwaves) wave_update_arc(A, A.qarcs[base + j], lane, st);
+  if (st) atomicOr(A.status, st);
+}
+
+__global__ void k_upd_finish(UpdArgs A, int32_t levels) {
+  __shared__ long long s[256];
+  long long acc = 0;
+  for (int L = threadIdx.x; L < levels; L += blockDim.x) acc += A.qcnt[L];
+  s[threadIdx.x] = acc;
+  __syncthreads();
+  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    A.res[0] = s[0];
+    if (A.res[3] == 0x7fffffffffffffffll) A.res[3] = -1;
+  }
+}
+
+// Single workgroup: all non-empty buckets from `L0` up, barrier between heights.
+__global__ void __launch_bounds__(1024) k_upd_persistent(UpdArgs A, int32_t L0, int32_t levels) {
+  __shared__ int32_t s_next;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int nwaves = blockDim.x >> 6;
+  uint32_t st = 0;
+  int32_t L = L0;
+  while (true) {
+    // next non-empty bucket at height >= L
+    if (threadIdx.x == 0) s_next = levels;
+    __syncthreads();
+    for (int32_t base = L; base < levels; base += blockDim.x) {
+      const int32_t l = base + threadIdx.x;
+      if (l < levels && __hip_atomic_load(&A.qcnt[l], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) > 0)
+        atomicMin(&s_next, l);
+      __syncthreads();
+      if (s_next < levels) break;
+    }
+    const int32_t nl = s_next;
+    __syncthreads();
+    if (nl >= levels) break;
+    L = nl;
+    const int32_t n = __hip_atomic_load(&A.qcnt[L], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const int32_t base = A.lvl_cap_off[L];
+    for (int32_t j = wave; j < n; j += nwaves) wave_update_arc(A, A.qarcs[base + j], lane, st);
+    __syncthreads();
+    L++;
+  }
+  if (st) atomicOr(A.status, st);
+}
+
+}  // namespace
+}  // namespace nwcch
+
+// ================================================================ C ABI (update)
+
+#include "../../include/nwcch.h"
+
+using namespace nwcch;
+
+namespace {
+
+template <class T>
+T* up_upload(DevTopo& d, const T* host, size_t count) {
+  T* p = nullptr;
+  HIP_OK(hipMalloc((void**)&p, std::max<size_t>(count, 1) * sizeof(T)));
+  d.owned.push_back(p);
+  if (count) HIP_OK(hipMemcpy(p, host, count * sizeof(T), hipMemcpyHostToDevice));
+  return p;
+}
+
+template <class T>
+T* up_alloc(DevTopo& d, size_t count) {
+  T* p = nullptr;
+  HIP_OK(hipMalloc((void**)&p, std::max<size_t>(count, 1) * sizeof(T)));
+  d.owned.push_back(p);
+  return p;
+}
+
+UpdArgs make_args(DevTopo& d, int32_t* w, int32_t* ud, int32_t* chg, i64 chg_cap) {
+  UpdArgs A;
+  A.ae_off = d.ae_off; A.ae_idx = d.ae_idx; A.arc_pos = d.arc_pos; A.tri_off = d.tri_off;
+  A.tri_ab = d.tri_ab; A.side_off = d.side_off; A.side_top = d.side_top; A.arc_lvl = d.arc_lvl;
+  A.lvl_cap_off = d.lvl_cap_off; A.e_arc32 = d.e_arc32; A.w = w;
+  A.ud = reinterpret_cast<int2*>(ud); A.qstamp = d.qstamp; A.qarcs = d.qarcs; A.qcnt = d.qcnt;
+  A.chg = chg ? chg : d.chg; A.chg_cap = chg ? chg_cap : d.mp; A.res = d.upd_res;
+  A.status = d.status; A.epoch = d.epoch;
+  return A;
+}
+
+}  // namespace
+
+// Builds the update index of a topology on the host (once) and uploads it.
+extern "C" int nwcch_update_prepare(nwcch_dev* h, nwcch_topo* topo) {
+  if (!h || !topo) { set_error("null argument"); return NWCCH_ERR_ARG; }
+  try {
+    DevTopo& d = h->d;
+    if (d.has_update) return NWCCH_OK;
+    Topology& t = *topo_of_mut(topo);
+    if (t.mp != d.mp || t.n != d.n) throw std::runtime_error("topology does not match the device handle");
+    build_side_index(t);
+    HIP_OK(hipSetDevice(d.device));
+    d.arc_pos = up_upload(d, t.arc_pos.data(), t.arc_pos.size());
+    std::vector<int32_t> alvl(t.mp);
+    for (i64 a = 0; a < t.mp; a++) alvl[a] = t.vlevel[t.arc_tail[a]];
+    d.arc_lvl_host = alvl;
+    d.arc_lvl = up_upload(d, alvl.data(), alvl.size());
+    d.side_off = up_upload(d, t.side_off.data(), t.side_off.size());
+    d.side_top = up_upload(d, t.side_top.data(), t.side_top.size());
+    std::vector<int32_t> cap(t.levels + 1, 0);
+    for (int32_t L = 0; L < t.levels; L++) cap[L + 1] = cap[L] + (int32_t)(t.lv[L].end - t.lv[L].begin);
+    d.lvl_cap_off = up_upload(d, cap.data(), cap.size());
+    std::vector<int32_t> ea(t.m);
+    for (i64 k = 0; k < t.m; k++) ea[k] = (int32_t)t.e_arc[k];
+    d.e_arc_host = ea;
+    d.e_arc32 = up_upload(d, ea.data(), ea.size());
+    d.qstamp = up_alloc<int32_t>(d, t.mp);
+    HIP_OK(hipMemset(d.qstamp, 0, std::max<size_t>(t.mp, 1) * sizeof(int32_t)));
+    d.qarcs = up_alloc<int32_t>(d, t.mp);
+    d.qcnt = up_alloc<int32_t>(d, t.levels);
+    d.chg = up_alloc<int32_t>(d, t.mp);
+    d.upd_res = up_alloc<long long>(d, 4);
+    HIP_OK(hipDeviceSynchronize());
+    d.has_update = true;
+  } catch (const std::exception& e) {
+    set_error(e.what());
+    return NWCCH_ERR_RUNTIME;
+  }
+  return NWCCH_OK;
+}
+
+extern "C" int nwcch_update(nwcch_dev* h, int32_t* w, int32_t* ud, const int64_t* edge_idx,
+                            const int32_t* new_w, int64_t k, int32_t mode, int32_t* changed,
+                            int64_t changed_cap, int64_t* result, void* stream) {
+  if (!h || !w || !ud || (k > 0 && (!edge_idx || !new_w)) || k < 0 || !result || changed_cap < 0) {
+    set_error("null argument");
+    return NWCCH_ERR_ARG;
+  }
+  DevTopo& d = h->d;
+  if (!d.has_update) { set_error("nwcch_update_prepare was not called"); return NWCCH_ERR_ARG; }
+  try {
+    hipStream_t s = (hipStream_t)stream;
+    // validate on the host and find the lowest height that can hold a queued arc
+    int32_t L0 = d.levels;
+    for (int64_t i = 0; i < k; i++) {
+      const int64_t e = edge_idx[i];
+      if (e < 0 || e >= d.m) throw std::invalid_argument("edge index out of range");
+      if (new_w[i] == (int32_t)0x80000000) throw std::invalid_argument("INT32_MIN is not a valid weight");
+      const int32_t a = d.e_arc_host[e];
+      if (a >= 0) L0 = std::min(L0, d.arc_lvl_host[a]);
+    }
+    if (k > d.stage_cap) {
+      i64 cap = std::max<i64>(k, 2 * d.stage_cap);
+      HIP_OK(hipStreamSynchronize(s));
+      i64* e2 = nullptr;
+      int32_t* w2 = nullptr;
+      HIP_OK(hipMalloc((void**)&e2, cap * sizeof(i64)));
+      HIP_OK(hipMalloc((void**)&w2, cap * sizeof(int32_t)));
+      d.owned.push_back(e2);
+      d.owned.push_back(w2);
+      d.stage_e = e2;
+      d.stage_w = w2;
+      d.stage_cap = cap;
+    }
+    if (k > 0) {
+      HIP_OK(hipMemcpyAsync(d.stage_e, edge_idx, k * sizeof(i64), hipMemcpyHostToDevice, s));
+      HIP_OK(hipMemcpyAsync(d.stage_w, new_w, k * sizeof(int32_t), hipMemcpyHostToDevice, s));
+    }
+    d.epoch++;
+    UpdArgs A = make_args(d, w, ud, changed, changed_cap);
+    hipLaunchKernelGGL(k_upd_reset, dim3(std::max(1, std::min(64, (d.levels + 255) / 256))), dim3(256), 0, s,
+                       A, d.levels);
+    if (k > 0)
+      hipLaunchKernelGGL(k_upd_seed, dim3((unsigned)((k + 255) / 256)), dim3(256), 0, s, A, d.stage_e,
+                         d.stage_w, (i64)k, d.m);
+    // mode: 0 = auto, 1 = one persistent workgroup, 2 = one launch per height
+    const bool persistent = mode == 1 || (mode == 0 && k <= 64);
+    if (L0 < d.levels) {
+      if (persistent) {
+        ProfScope ps(d, s, kKUpdate);
+        hipLaunchKernelGGL(k_upd_persistent, dim3(1), dim3(1024), 0, s, A, L0, d.levels);
+      } else {
+        for (int32_t L = L0; L < d.levels; L++) {
+          const i64 cap = d.lv[L].end - d.lv[L].begin;
+          const int blocks = (int)std::max<i64>(1, std::min<i64>((cap + 3) / 4, 1024));
+          ProfScope ps(d, s, kKUpdate);
+          hipLaunchKernelGGL(k_upd_level, dim3(blocks), dim3(256), 0, s, A, L);
+        }
+      }
+    }
+    hipLaunchKernelGGL(k_upd_finish, dim3(1), dim3(256), 0, s, A, d.levels);
+    HIP_OK(hipMemcpyAsync(result, d.upd_res, 4 * sizeof(long long), hipMemcpyDeviceToDevice, s));
+    HIP_OK(hipGetLastError());
+  } catch (const std::invalid_argument& e) {
+    set_error(e.what());
+    return NWCCH_ERR_ARG;
+  } catch (const std::exception& e) {
+    set_error(e.what());
+    return NWCCH_ERR_RUNTIME;
+  }
+  return NWCCH_OK;
+}
+
+extern "C" int nwcch_update_changed(nwcch_dev* h, int32_t* out, int64_t cap, void* stream) {
+  if (!h || !out || cap < 0) { set_error("null argument"); return NWCCH_ERR_ARG; }
+  DevTopo& d = h->d;
+  if (!d.has_update) { set_error("nwcch_update_prepare was not called"); return NWCCH_ERR_ARG; }
+  try {
+    const i64 n = std::min<i64>(cap, d.mp);
+    if (n > 0)
+      HIP_OK(hipMemcpyAsync(out, d.chg, n * sizeof(int32_t), hipMemcpyDeviceToDevice, (hipStream_t)stream));
+  } catch (const std::exception& e) {
+    set_error(e.what());
+    return NWCCH_ERR_RUNTIME;
+  }
+  return NWCCH_OK;
+}

@@ -49,15 +49,15 @@ def lib():
             "nwcch_query": ([vp, p, p, p, i64, p, p, vp], C.c_int),
             "nwcch_status": ([vp, p, C.c_int, vp], C.c_int),
             "nwcch_profile_begin": ([vp, i64], C.c_int),
-            "nwcch_profile_end": ([vp, p, p, i64, p], C.c_int),
-        }
-        optional = {
-            "nwcch_update": ([vp, p, p, p, i64, p, vp], C.c_int),
-            "nwcch_update_result": ([vp, p, vp], C.c_int),
+            "nwcch_update_prepare": ([vp, vp], C.c_int),
+            "nwcch_update": ([vp, p, p, p, p, i64, i32, p, i64, p, vp], C.c_int),
             "nwcch_update_changed": ([vp, p, i64, vp], C.c_int),
-            "nwcch_customize_batch": ([vp, p, i32, p, i32, vp], C.c_int),
-            "nwcch_query_batch": ([vp, p, i32, p, p, p, i64, p, p, vp], C.c_int),
+            "nwcch_profile_end": ([vp, p, p, i64, p], C.c_int),
+            "nwcch_query_strided": ([vp, p, i64, p, p, i64, p, p, vp], C.c_int),
+            "nwcch_customize_batch": ([vp, p, i32, p, vp], C.c_int),
+            "nwcch_negcycle_scan_batch": ([p, i64, i32, p, vp], C.c_int),
         }
+        optional = {}
         for name, (args, res) in sigs.items():
             f = getattr(L, name)
             f.argtypes = args
@@ -96,7 +96,9 @@ def exported_symbols():
             "nwcch_topo_destroy", "nwcch_topo_info", "nwcch_topo_export",
             "nwcch_topo_export_triangles", "nwcch_dev_create", "nwcch_dev_destroy",
             "nwcch_dev_info", "nwcch_customize", "nwcch_negcycle_scan", "nwcch_negcycle_list",
-            "nwcch_query", "nwcch_status", "nwcch_profile_begin", "nwcch_profile_end"]
+            "nwcch_query", "nwcch_status", "nwcch_profile_begin", "nwcch_profile_end",
+            "nwcch_update_prepare", "nwcch_update", "nwcch_update_changed", "nwcch_query_strided",
+            "nwcch_customize_batch", "nwcch_negcycle_scan_batch"]
 
 
 # ------------------------------------------------------------ host topology

@@ -103,18 +103,102 @@ class DeviceTopology:
               "nwcch_negcycle_list")
         return sorted(lst.cpu().tolist())
 
-    def query(self, ud: torch.Tensor, s_rank, t_rank, want_scanned=True, stream=None):
-        """ET queries between ranks -> (dist int32 [nq], scanned int32 [nq] or None)."""
+    def query(self, ud: torch.Tensor, s_rank, t_rank, want_scanned=True, stream=None, metric: int = 0,
+              dist: torch.Tensor | None = None):
+        """ET queries between ranks -> (dist int32 [nq], scanned int32 [nq] or None).
+        ud is one metric [mp, 2] or a batch [mp, K, 2] (then `metric` picks one)."""
         qs = torch.as_tensor(s_rank).to(self.dev, torch.int32).contiguous()
         qt = torch.as_tensor(t_rank).to(self.dev, torch.int32).contiguous()
         if qs.shape != qt.shape or qs.dim() != 1:
             raise ValueError("s_rank and t_rank must be 1-d of equal length")
+        if ud.dim() == 3:
+            K = ud.shape[1]
+            if not 0 <= metric < K:
+                raise ValueError("metric out of range")
+            assert ud.shape == (self.mp, K, 2) and ud.is_contiguous()
+            base, stride = ud.data_ptr() + 8 * metric, K
+        else:
+            assert ud.shape == (self.mp, 2) and ud.is_contiguous()
+            base, stride = ud.data_ptr(), 1
         nq = qs.shape[0]
-        dist = torch.empty(nq, dtype=torch.int32, device=self.dev)
+        if dist is None:
+            dist = torch.empty(nq, dtype=torch.int32, device=self.dev)
         sc = torch.empty(nq, dtype=torch.int32, device=self.dev) if want_scanned else None
-        check(lib().nwcch_query(self._h, ptr(ud), ptr(qs), ptr(qt), nq, ptr(dist), ptr(sc), _stream(stream)),
-              "nwcch_query")
+        check(lib().nwcch_query_strided(self._h, base, stride, ptr(qs), ptr(qt), nq, ptr(dist), ptr(sc),
+                                        _stream(stream)), "nwcch_query_strided")
         return dist, sc
+
+    # ---------------------------------------------------------------- batched
+    def weights_batch(self, W) -> torch.Tensor:
+        W = torch.as_tensor(W)
+        if W.dim() != 2 or W.shape[1] != self.m:
+            raise ValueError(f"expected [K, {self.m}] weights")
+        if W.dtype != torch.int32 or not W.is_cuda:
+            W64 = W.to(torch.int64)
+            if W64.numel() and (int(W64.min()) <= -INF or int(W64.max()) > INF):
+                raise ValueError("weights must lie in (-2^31+1, 2^31-1]")
+            W = W64.to(self.dev, torch.int32)
+        return W.contiguous()
+
+    def customize_batch(self, W, out: torch.Tensor | None = None, stream=None) -> torch.Tensor:
+        """K metrics at once: W [K, m] -> UD [mp, K, 2] (arc-major)."""
+        W = self.weights_batch(W)
+        K = W.shape[0]
+        if out is None:
+            out = torch.empty((self.mp, K, 2), dtype=torch.int32, device=self.dev)
+        assert out.shape == (self.mp, K, 2) and out.dtype == torch.int32 and out.is_contiguous()
+        check(lib().nwcch_customize_batch(self._h, ptr(W), K, ptr(out), _stream(stream)), "nwcch_customize_batch")
+        return out
+
+    def negcycle_scan_batch(self, UD: torch.Tensor, out: torch.Tensor | None = None, stream=None):
+        """Device int64 [2, K]: per-metric witness counts and smallest witness (or -1)."""
+        K = UD.shape[1]
+        if out is None:
+            out = torch.empty((2, K), dtype=torch.int64, device=self.dev)
+        check(lib().nwcch_negcycle_scan_batch(ptr(UD), UD.shape[0], K, ptr(out), _stream(stream)),
+              "nwcch_negcycle_scan_batch")
+        return out
+
+    # ---------------------------------------------------------- partial update
+    def prepare_update(self):
+        if not getattr(self, "_upd_ready", False):
+            check(lib().nwcch_update_prepare(self._h, self.topo._h), "nwcch_update_prepare")
+            self._upd_ready = True
+            self._upd_res = torch.empty(4, dtype=torch.int64, device=self.dev)
+
+    def update(self, w: torch.Tensor, ud: torch.Tensor, edge_idx, new_w, mode: int = 0,
+               changed: torch.Tensor | None = None, stream=None) -> torch.Tensor:
+        """Partial re-customization in place (w and ud are modified).  Changes
+        with a repeated edge index keep the last weight (sequential assignment,
+        python/cch.py:359-360).  Returns the device int64[4] result
+        (evaluated, changed, negative changed, first negative changed or -1);
+        not synchronised."""
+        self.prepare_update()
+        e = np.asarray(edge_idx, np.int64).reshape(-1)
+        v = np.asarray(new_w, np.int64).reshape(-1)
+        if e.shape != v.shape:
+            raise ValueError("edge_idx and new_w must have the same length")
+        if len(e) and (v.min() <= -INF or v.max() > INF):
+            raise ValueError("weights must lie in (-2^31+1, 2^31-1]")
+        if len(e):
+            last = len(e) - 1 - np.unique(e[::-1], return_index=True)[1]
+            last.sort()
+            e, v = e[last], v[last]
+        e = np.ascontiguousarray(e)
+        v = np.ascontiguousarray(v.astype(np.int32))
+        assert w.dtype == torch.int32 and w.is_cuda and w.shape == (self.m,)
+        assert ud.dtype == torch.int32 and ud.shape == (self.mp, 2) and ud.is_contiguous()
+        cap = 0 if changed is None else changed.numel()
+        check(lib().nwcch_update(self._h, ptr(w), ptr(ud), ptr(e), ptr(v), len(e), int(mode),
+                                 ptr(changed), cap, ptr(self._upd_res), _stream(stream)), "nwcch_update")
+        return self._upd_res
+
+    def last_changed(self, count: int, stream=None) -> torch.Tensor:
+        """The changed arcs of the last update, ascending (python/cch.py:380, 396)."""
+        out = torch.empty(max(count, 1), dtype=torch.int32, device=self.dev)
+        if count:
+            check(lib().nwcch_update_changed(self._h, ptr(out), count, _stream(stream)), "nwcch_update_changed")
+        return torch.sort(out[:count]).values
 
     # -------------------------------------------------------------- profiling
     def profile_begin(self, max_launches: int):

@@ -672,3 +672,38 @@ i64 orc_restricted(i64 n, const i64* off, const i32* tgt, const i32* w, const i3
   }
   return best >= INF64 ? INF32 : best;
 }
+
+/* Customization with stop_on_negative_cycle (python/cch.py:264-278): triangles
+ * in (x, i, k) order; before relaxing a triangle whose first arc is i, the arcs
+ * of every not yet checked vertex whose row ends at or before i are tested.
+ * Returns the first witness arc found that way, or -1 if the loop finishes
+ * without one (the reference then returns the full list). */
+i64 orc_customize_stop(i64 n, const i64* first, const i32* head, i64 m, const i64* e_arc,
+                       const int8_t* e_up, const i32* w, i32* up, i32* dn) {
+  const i64 mp = first[n];
+  int ovf = 0;
+  base_weights(m, mp, e_arc, e_up, w, up, dn);
+  i64 xc = 0;
+  for (i64 x = 0; x < n; x++) {
+    const i64 s = first[x], e = first[x + 1];
+    for (i64 i = s; i < e; i++) {
+      const i64 y = head[i];
+      i64 j = first[y];
+      const i64 je = first[y + 1];
+      for (i64 kk = i + 1; kk < e; kk++) {
+        while (xc < n && first[xc + 1] <= i) {
+          for (i64 aa = first[xc]; aa < first[xc + 1]; aa++)
+            if (up[aa] != INF32 && dn[aa] != INF32 && (i64)up[aa] + (i64)dn[aa] < 0) return aa;
+          xc++;
+        }
+        const i32 z = head[kk];
+        while (j < je && head[j] < z) j++;
+        i32 v = add_inf(dn[i], up[kk], &ovf);
+        if (v < up[j]) up[j] = v;
+        v = add_inf(dn[kk], up[i], &ovf);
+        if (v < dn[j]) dn[j] = v;
+      }
+    }
+  }
+  return -1;
+}

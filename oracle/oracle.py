@@ -42,6 +42,8 @@ def olib():
         L.orc_customize.restype = C.c_int
         L.orc_customize_par.argtypes = [i64, p, p, i64, p, p, p, p, p, C.c_int]
         L.orc_customize_par.restype = C.c_int
+        L.orc_customize_stop.argtypes = [i64, p, p, i64, p, p, p, p, p]
+        L.orc_customize_stop.restype = i64
         L.orc_negcycle.argtypes = [i64, p, p, p]
         L.orc_negcycle.restype = i64
         L.orc_query_many.argtypes = [i64, p, p, p, p, p, i64, p, p, p, p, C.c_int]
@@ -134,6 +136,14 @@ class Structure:
         ovf = olib().orc_customize(self.n, _p(self.first), _p(self.head), self.m, _p(self.e_arc),
                                    _p(self.e_up), _p(w), _p(up), _p(dn))
         return up, dn, bool(ovf)
+
+    def customize_stop(self, w):
+        """Early-stop witness of python/cch.py:270-278, or -1."""
+        w = np.ascontiguousarray(w, np.int32)
+        up = np.empty(self.mp, np.int32)
+        dn = np.empty(self.mp, np.int32)
+        return int(olib().orc_customize_stop(self.n, _p(self.first), _p(self.head), self.m, _p(self.e_arc),
+                                             _p(self.e_up), _p(w), _p(up), _p(dn)))
 
     def customize_par(self, w, threads=8):
         w = np.ascontiguousarray(w, np.int32)

@@ -1,0 +1,130 @@
+"""GPU partial re-customization against the oracle's restatement of
+python/cch.py:349-402 (bitwise values, evaluated count, changed set) and the
+python/test_cch.py update scenarios through the CCH drop-in."""
+import numpy as np
+import pytest
+import torch
+
+from conftest import order_of, small_graphs
+from nwcch import _lib, graphs
+from nwcch.cch import CCH
+from nwcch.device import DeviceTopology
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+INF = O.INF
+
+
+def _setup(g, rank):
+    topo = _lib.Topology.from_graph(g.n, g.off, g.tgt, rank)
+    dt = DeviceTopology(topo)
+    st = O.Structure(g.n, g.off, g.tgt, rank)
+    return dt, st
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+@pytest.mark.parametrize("k", [1, 10, 1000])
+def test_update_matches_oracle(c0, k, mode):
+    """Random changes of +-U(0,500) on k arcs, four rounds in a row."""
+    g, rank = c0
+    print(f"case: config 0 update k={k} mode={mode}", flush=True)
+    dt, st = _setup(g, rank)
+    w = dt.weights(g.w)
+    ud = dt.customize(w)
+    up, dn, _ = st.customize(g.w)
+    upd = O.Updater(st, g.w, up, dn)
+    rng = np.random.default_rng(k * 10 + mode)
+    for rnd in range(4):
+        idx = rng.choice(g.m, k, replace=False)
+        new = (upd.w[idx] + rng.integers(-500, 501, k)).astype(np.int32)
+        res = dt.update(w, ud, idx, new, mode=mode)
+        ev, nch, nneg, fneg = (int(v) for v in res.cpu())
+        ev0, ch0, _ = upd.update(idx, new)
+        got = ud.cpu().numpy()
+        assert np.array_equal(got[:, 0], upd.up) and np.array_equal(got[:, 1], upd.dn), rnd
+        assert np.array_equal(w.cpu().numpy(), upd.w)
+        assert ev == ev0, (ev, ev0)
+        assert dt.last_changed(nch).cpu().numpy().tolist() == ch0.tolist()
+        neg0 = st.negcycle(upd.up[ch0], upd.dn[ch0])[0]
+        assert nneg == neg0
+        assert dt.status() == 0
+
+
+def test_update_equals_full_customization_road():
+    g = graphs.make_config(1)
+    print("case: config 1 update k=1000", flush=True)
+    rank = _lib.order(g.n, g.off, g.tgt, g.x, g.y, 64)
+    dt, st = _setup(g, rank)
+    w = dt.weights(g.w)
+    ud = dt.customize(w)
+    rng = np.random.default_rng(5)
+    wh = g.w.copy()
+    for k in (1, 10, 1000):
+        idx = rng.choice(g.m, k, replace=False)
+        new = (wh[idx] + rng.integers(-500, 501, k)).astype(np.int32)
+        wh[idx] = new
+        res = dt.update(w, ud, idx, new)
+        up, dn, _ = st.customize(wh)
+        got = ud.cpu().numpy()
+        assert np.array_equal(got[:, 0], up) and np.array_equal(got[:, 1], dn), k
+        assert int(res[1].item()) <= int(res[0].item())
+
+
+def test_cch_dropin_scenario():
+    """python/test_cch.py:28-106 through the CCH drop-in: queries vs Bellman-Ford,
+    raise/restore rounds equal full customization, a -1e6 update creates a
+    negative cycle that the changed-arc check sees, restoring removes it."""
+    rng = np.random.default_rng(3)
+    for g in small_graphs()[::2]:
+        print("case: drop-in scenario", g.name, flush=True)
+        c = CCH(g, xy=list(zip(g.x, g.y)), leaf=8, log=lambda *_: None)
+        assert not c.customize(g), "false negative cycle"
+        for _ in range(10):
+            s, t = (int(v) for v in rng.integers(0, g.n, 2))
+            truth, _ = O.bellman_ford(g.n, g.off, g.tgt, g.w, s)
+            d, scanned = c.query(s, t)
+            assert d == (float("inf") if truth[t] == INF else truth[t])
+            assert scanned <= c.depth * (c.depth + 1)
+        w = g.w.copy()
+        for _ in range(3):
+            picked = rng.choice(g.m, 10, replace=False)
+            old = w[picked].copy()
+            for new in (old + rng.integers(1, 50, 10), old):
+                w[picked] = new
+                c.update_edges(list(zip(picked.tolist(), new.tolist())))
+                assert not c.update_created_negative_cycle()
+                full = CCH(g.with_weights(w), order=c.rank, log=lambda *_: None)
+                full.customize(g.with_weights(w))
+                assert torch.equal(c.ud, full.ud)
+        k = int(g.off[int(np.nonzero(np.diff(g.off) > 0)[0][0])])
+        u = int(np.searchsorted(g.off, k, side="right") - 1)
+        v = int(g.tgt[k])
+        if u not in g.tgt[g.off[v]:g.off[v + 1]]:
+            continue
+        old = int(w[k])
+        c.update_edges([(k, -10 ** 6)])
+        assert c.update_created_negative_cycle() and c.has_negative_cycle()
+        assert c.last_changed == sorted(c.last_changed)
+        c.update_edges([(k, old)])
+        assert not c.has_negative_cycle()
+
+
+def test_stop_on_negative_cycle_witness(c0):
+    """customize(g, stop_on_negative_cycle=True) returns the reference's witness:
+    the first negative arc of a vertex checked during elimination
+    (python/cch.py:270-278), else the full list."""
+    g, rank = c0
+    k = int(g.off[100])
+    v = int(g.tgt[k])
+    back = [j for j in range(g.off[v], g.off[v + 1]) if g.tgt[j] == 100][0]
+    w = g.w.copy()
+    w[k], w[back] = 0, -1
+    c = CCH(g.with_weights(w), order=rank, log=lambda *_: None)
+    full = c.customize(g.with_weights(w))
+    st = O.Structure(g.n, g.off, g.tgt, rank)
+    up, dn, _ = st.customize(w)
+    neg = np.nonzero((up != INF) & (dn != INF) & (up.astype(np.int64) + dn < 0))[0]
+    assert full == neg.tolist() and len(full) > 0
+    early = c.customize(g.with_weights(w), stop_on_negative_cycle=True)
+    wit = st.customize_stop(w)   # the reference's elimination loop with early checks
+    assert early == ([wit] if wit >= 0 else neg.tolist())
