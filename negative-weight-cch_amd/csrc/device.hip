@@ -56,81 +56,101 @@ __device__ __forceinline__ uint32_t wave_or(uint32_t v) {
   return v;
 }
 
-// F_c of one arc: min over its original arcs (by direction) and its lower triangles
-// (python/cch.py:234-247 + 279-285; python/cch_nb.py:534-562 in pull form).
-__device__ __forceinline__ void eval_arc_thread(int32_t c, i64 t0, i64 t1, const int32_t* __restrict__ w,
-                                                const int32_t* __restrict__ ae_off,
-                                                const int32_t* __restrict__ ae_idx,
-                                                const int2* __restrict__ tri_ab, const int2* ud,
-                                                int32_t& up, int32_t& dn, uint32_t& st) {
-  up = kInf;
-  dn = kInf;
-  const int32_t e0 = ae_off[c], e1 = ae_off[c + 1];
-  for (int32_t j = e0; j < e1; j++) {
-    const int32_t e = ae_idx[j];
-    const int32_t wk = w[e >> 1];
-    if (wk == (int32_t)0x80000000) st |= kStatusBadWeight;
-    if (e & 1) up = min(up, wk);
-    else dn = min(dn, wk);
+// Launch geometry of one level: team class k (64 >> k lanes per arc) owns
+// schedule positions [cls[k], cls[k+1]) and blocks [blk[k], blk[k+1]).
+struct LevelLaunch {
+  i64 cls[kTeamClasses + 1];
+  int32_t blk[kTeamClasses + 1];
+};
+
+LevelLaunch level_launch(const LevelRange& r) {
+  LevelLaunch L;
+  int32_t b = 0;
+  for (int k = 0; k <= kTeamClasses; k++) {
+    L.cls[k] = r.cls[k];
+    L.blk[k] = b;
+    if (k < kTeamClasses) {
+      const i64 lanes = (r.cls[k + 1] - r.cls[k]) * (i64)(64 >> k);
+      b += (int32_t)((lanes + 255) / 256);
+    }
   }
-  for (i64 q = t0; q < t1; q++) {
-    const int2 ab = tri_ab[q];
-    const int2 A = ud[ab.x];
-    const int2 B = ud[ab.y];
-    up = min(up, inf_add(A.y, B.x, st));  // y -> x -> z
-    dn = min(dn, inf_add(B.y, A.x, st));  // z -> x -> y
-  }
+  return L;
 }
 
-// One level of customization.  Blocks [0, nbig_blocks) give one wavefront to each
-// "big" arc (lanes stride over its triangles, then a wavefront min-reduction); the
-// remaining blocks give one thread to each small arc.  Arcs of one level never
-// read each other (their inputs have tails at lower ET heights), so there are no
-// atomics and every arc is written exactly once.
-__global__ void __launch_bounds__(256) k_customize_level(
-    i64 pbeg, i64 pbig, i64 pend, int nbig_blocks, const int32_t* __restrict__ sched_arc,
-    const i64* __restrict__ tri_off, const int2* __restrict__ tri_ab,
-    const int32_t* __restrict__ ae_off, const int32_t* __restrict__ ae_idx,
-    const int32_t* __restrict__ w, int2* ud, uint32_t* status) {
+// Base values of every arc (python/cch.py:234-247, python/cch_nb.py:534-542) in
+// pull form: ud[c] = min over the original arcs mapped onto c, per direction,
+// INF if none.  One launch over all arcs; the level kernels start from it.
+__global__ void __launch_bounds__(256) k_base_all(i64 mp, const int32_t* __restrict__ ae_off,
+                                                  const int32_t* __restrict__ ae_idx,
+                                                  const int32_t* __restrict__ w, int2* ud, uint32_t* status) {
   uint32_t st = 0;
-  if ((int)blockIdx.x < nbig_blocks) {
-    const int lane = threadIdx.x & 63;
-    const i64 p = pbeg + (i64)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (p >= pbig) return;
-    const int32_t c = sched_arc[p];
+  const i64 stride = (i64)gridDim.x * blockDim.x;
+  for (i64 c = (i64)blockIdx.x * blockDim.x + threadIdx.x; c < mp; c += stride) {
     int32_t up = kInf, dn = kInf;
-    const int32_t e0 = ae_off[c], e1 = ae_off[c + 1];
-    for (int32_t j = e0 + lane; j < e1; j += 64) {
+    const int32_t e1 = ae_off[c + 1];
+    for (int32_t j = ae_off[c]; j < e1; j++) {
       const int32_t e = ae_idx[j];
       const int32_t wk = w[e >> 1];
       if (wk == (int32_t)0x80000000) st |= kStatusBadWeight;
       if (e & 1) up = min(up, wk);
       else dn = min(dn, wk);
     }
-    const i64 t1 = tri_off[p + 1];
-    for (i64 q = tri_off[p] + lane; q < t1; q += 64) {
+    ud[c] = make_int2(up, dn);
+  }
+  if (st) atomicOr(status, st);
+}
+
+// One level of customization: F_c of every arc of the level, i.e. the min of its
+// base value and its lower triangles (python/cch.py:279-285;
+// python/cch_nb.py:543-562 in pull form).  A team of TS lanes owns an arc, each
+// lane folds at most a few triangles (two gathers in flight per step), and the
+// team is reduced with cross-lane minima.  Arcs of one level never read each
+// other -- their inputs have lower endpoints of smaller ET height -- so there are
+// no atomics and every arc is written exactly once.
+__global__ void __launch_bounds__(256) k_customize_level(LevelLaunch L, const PosRec* __restrict__ prec,
+                                                         const int2* __restrict__ tri_ab, int2* ud,
+                                                         uint32_t* status) {
+  const int32_t b = blockIdx.x;
+  int k = 0;
+  while (k < kTeamClasses - 1 && b >= L.blk[k + 1]) k++;
+  const int tlog = 6 - k;
+  const int TS = 1 << tlog;
+  const int t = threadIdx.x & (TS - 1);
+  const i64 p = L.cls[k] + (i64)(b - L.blk[k]) * (256 >> tlog) + (threadIdx.x >> tlog);
+  const bool valid = p < L.cls[k + 1];
+  uint32_t st = 0;
+  int32_t up = kInf, dn = kInf;
+  int32_t c = 0;
+  if (valid) {
+    const PosRec r = prec[p];
+    c = r.arc;
+    if (t == 0) {
+      const int2 base = ud[c];
+      up = base.x;
+      dn = base.y;
+    }
+    const i64 t1 = r.tri_begin + r.tri_count;
+    i64 q = r.tri_begin + t;
+    for (; q + TS < t1; q += 2 * TS) {
+      const int2 ab0 = tri_ab[q];
+      const int2 ab1 = tri_ab[q + TS];
+      const int2 A0 = ud[ab0.x], B0 = ud[ab0.y];
+      const int2 A1 = ud[ab1.x], B1 = ud[ab1.y];
+      up = min(up, min(inf_add(A0.y, B0.x, st), inf_add(A1.y, B1.x, st)));  // y -> x -> z
+      dn = min(dn, min(inf_add(B0.y, A0.x, st), inf_add(B1.y, A1.x, st)));  // z -> x -> y
+    }
+    if (q < t1) {
       const int2 ab = tri_ab[q];
-      const int2 A = ud[ab.x];
-      const int2 B = ud[ab.y];
+      const int2 A = ud[ab.x], B = ud[ab.y];
       up = min(up, inf_add(A.y, B.x, st));
       dn = min(dn, inf_add(B.y, A.x, st));
     }
-    up = wave_min(up);
-    dn = wave_min(dn);
-    st = wave_or(st);
-    if (lane == 0) {
-      ud[c] = make_int2(up, dn);
-      if (st) atomicOr(status, st);
-    }
-    return;
   }
-  const i64 p = pbig + (i64)(blockIdx.x - nbig_blocks) * blockDim.x + threadIdx.x;
-  if (p < pend) {
-    const int32_t c = sched_arc[p];
-    int32_t up, dn;
-    eval_arc_thread(c, tri_off[p], tri_off[p + 1], w, ae_off, ae_idx, tri_ab, ud, up, dn, st);
-    ud[c] = make_int2(up, dn);
+  for (int o = 1; o < TS; o <<= 1) {
+    up = min(up, __shfl_xor(up, o, 64));
+    dn = min(dn, __shfl_xor(dn, o, 64));
   }
+  if (valid && t == 0) ud[c] = make_int2(up, dn);
   if (st) atomicOr(status, st);
 }
 
@@ -203,11 +223,16 @@ __global__ void k_scan_finish(const unsigned long long* src, long long* dst) {
 // ------------------------------------------------------------ ET query
 
 // One query per wavefront (python/cch.py:675-732, python/cch_nb.py:577-615,
-// cpp/cch.cpp:549-575).  Labels live in LDS indexed by the position of a vertex on
-// its ancestor chain (position = depth(start) - depth(vertex)), so no per-vertex
-// scratch and no reset pass over n is needed; the walk over the s chain records
-// its vertices so the t walk can recognise common ancestors.
+// cpp/cch.cpp:549-575).  The two searches are independent until they meet, so
+// they run at the same time on the two halves of the wavefront: lanes 0-31 walk
+// the ancestors of s relaxing upward arcs, lanes 32-63 those of t relaxing the
+// reversed (down) arcs.  Labels live in LDS indexed by the position of a vertex
+// on its ancestor chain (depth(start) - depth(vertex)), so no per-vertex scratch
+// and no reset pass over n is needed.  The t half finds the lowest common
+// ancestor with Euler-tour intervals; afterwards every common ancestor (depth
+// 0..depth(lca)) is a meeting candidate, min(df + dr) over them.
 __global__ void __launch_bounds__(64) k_query(const VInfo* __restrict__ vinfo,
+                                              const int2* __restrict__ euler,
                                               const int32_t* __restrict__ adep,
                                               const int2* __restrict__ ud, i64 ud_stride,
                                               const int32_t* __restrict__ qs,
@@ -216,11 +241,10 @@ __global__ void __launch_bounds__(64) k_query(const VInfo* __restrict__ vinfo,
                                               int32_t* __restrict__ scanned,
                                               uint32_t* status) {
   extern __shared__ int32_t lds[];
-  int32_t* df = lds;
-  int32_t* dr = lds + maxlen;
-  int32_t* cs = lds + 2 * maxlen;
   const i64 q = blockIdx.x;
   const int lane = threadIdx.x;
+  const int half = lane >> 5;
+  const int hl = lane & 31;
   if (q >= nq) return;
   uint32_t st = 0;
   const int32_t rs = qs[q], rt = qt[q];
@@ -232,79 +256,58 @@ __global__ void __launch_bounds__(64) k_query(const VInfo* __restrict__ vinfo,
     }
     return;
   }
-  VInfo vs = vinfo[rs];
-  VInfo vt = vinfo[rt];
-  const int32_t ds = vs.depth, dt = vt.depth;
-  for (int32_t i = lane; i <= ds; i += 64) df[i] = kInf;
-  for (int32_t i = lane; i <= dt; i += 64) dr[i] = kInf;
+  int32_t* lab = lds + half * maxlen;  // df on the s half, dr on the t half
+  int32_t x = half ? rt : rs;
+  VInfo vi = vinfo[x];
+  const int32_t D = vi.depth;
+  const int32_t Ds = __shfl(D, 0, 64), Dt = __shfl(D, 32, 64);
+  for (int32_t i = hl; i <= D; i += 32) lab[i] = kInf;
   __syncthreads();
-  if (lane == 0) { df[0] = 0; dr[0] = 0; }
+  if (hl == 0) lab[0] = 0;
   __syncthreads();
-  int32_t nscan = 0;
-  // upward search from s
-  {
-    int32_t x = rs;
-    VInfo vi = vs;
-    int32_t p = 0;
-    while (true) {
-      const int32_t par = vi.parent;
-      VInfo nx;
-      if (par >= 0) nx = vinfo[par];
-      if (lane == 0) cs[p] = x;
-      const int32_t dx = df[p];
+  const int32_t tin_s = euler[rs].x;
+  int32_t lca_depth = -1;
+  int32_t nscan = 0, p = 0;
+  bool active = true;
+  while (__any(active)) {
+    const int32_t par = vi.parent;
+    VInfo nx;
+    if (active && par >= 0) nx = vinfo[par];
+    if (active) {
+      if (half && lca_depth < 0) {
+        const int2 e = euler[x];
+        if (e.x <= tin_s && tin_s <= e.y) lca_depth = vi.depth;
+      }
+      const int32_t dx = lab[p];
       if (dx != kInf) {
         nscan += vi.deg;
-        for (int32_t j = lane; j < vi.deg; j += 64) {
+        for (int32_t j = hl; j < vi.deg; j += 32) {
           const int32_t a = vi.first + j;
-          const int32_t pos = ds - adep[a];
-          const int32_t v = inf_add(dx, ud[(i64)a * ud_stride].x, st);
-          if (v < df[pos]) df[pos] = v;
+          const int32_t pos = D - adep[a];
+          const int2 u = ud[(i64)a * ud_stride];
+          const int32_t v = inf_add(dx, half ? u.y : u.x, st);
+          if (v < lab[pos]) lab[pos] = v;
         }
       }
-      __syncthreads();
-      if (par < 0) break;
-      x = par;
-      vi = nx;
-      p++;
+    }
+    __syncthreads();
+    if (active) {
+      if (par < 0) active = false;
+      else { x = par; vi = nx; p++; }
     }
   }
-  // downward search from t, meeting at common ancestors
+  const int32_t lcad = __shfl(lca_depth, 32, 64);
   int32_t best = kInf;
-  {
-    int32_t x = rt;
-    VInfo vi = vt;
-    int32_t p = 0;
-    while (true) {
-      const int32_t par = vi.parent;
-      VInfo nx;
-      if (par >= 0) nx = vinfo[par];
-      const int32_t dx = dr[p];
-      if (dx != kInf) {
-        const int32_t d = vi.depth;
-        if (d <= ds && cs[ds - d] == x) {
-          const int32_t fx = df[ds - d];
-          const int32_t v = inf_add(dx, fx, st);
-          if (v < best) best = v;
-        }
-        nscan += vi.deg;
-        for (int32_t j = lane; j < vi.deg; j += 64) {
-          const int32_t a = vi.first + j;
-          const int32_t pos = dt - adep[a];
-          const int32_t v = inf_add(dx, ud[(i64)a * ud_stride].y, st);
-          if (v < dr[pos]) dr[pos] = v;
-        }
-      }
-      __syncthreads();
-      if (par < 0) break;
-      x = par;
-      vi = nx;
-      p++;
-    }
+  for (int32_t d = lane; d <= lcad; d += 64) {
+    const int32_t v = inf_add(lds[Ds - d], lds[maxlen + Dt - d], st);
+    best = min(best, v);
   }
+  best = wave_min(best);
+  const int32_t total = __shfl(nscan, 0, 64) + __shfl(nscan, 32, 64);
   st = wave_or(st);
   if (lane == 0) {
     dist[q] = best;
-    if (scanned) scanned[q] = nscan;
+    if (scanned) scanned[q] = total;
     if (st) atomicOr(status, st);
   }
 }
@@ -363,10 +366,31 @@ extern "C" int nwcch_dev_create(const nwcch_topo* topo, int device, nwcch_dev** 
     d.tri_ab = dev_upload(d, reinterpret_cast<const int2*>(t.tri_ab.data()), (size_t)t.triangles);
     d.ae_off = dev_upload(d, t.ae_off.data(), t.ae_off.size());
     d.ae_idx = dev_upload(d, t.ae_idx.data(), t.ae_idx.size());
+    std::vector<PosRec> pr(t.mp);
+    for (i64 p = 0; p < t.mp; p++)
+      pr[p] = PosRec{t.tri_off[p], t.sched_arc[p], (int32_t)(t.tri_off[p + 1] - t.tri_off[p])};
+    d.prec = dev_upload(d, pr.data(), pr.size());
     std::vector<VInfo> vi(t.n);
     for (i64 x = 0; x < t.n; x++)
       vi[x] = VInfo{(int32_t)t.first[x], (int32_t)(t.first[x + 1] - t.first[x]), t.parent[x], t.vdepth[x]};
     d.vinfo = dev_upload(d, vi.data(), vi.size());
+    // Euler-tour (preorder) interval of every ET subtree: parents have higher
+    // ranks than their children, so sizes accumulate upward in rank order and
+    // preorder numbers are handed out downward
+    std::vector<int32_t> size(t.n, 1), nxt(t.n, 0);
+    std::vector<int2> eu(t.n);
+    for (i64 x = 0; x < t.n; x++)
+      if (t.parent[x] >= 0) size[t.parent[x]] += size[x];
+    int32_t next_root = 0;
+    for (i64 x = t.n - 1; x >= 0; x--) {
+      const int32_t pa = t.parent[x];
+      int32_t tin;
+      if (pa < 0) { tin = next_root; next_root += size[x]; }
+      else { tin = nxt[pa]; nxt[pa] += size[x]; }
+      eu[x] = make_int2(tin, tin + size[x] - 1);
+      nxt[x] = tin + 1;
+    }
+    d.euler = dev_upload(d, eu.data(), eu.size());
     std::vector<int32_t> adep(t.mp);
     for (i64 a = 0; a < t.mp; a++) adep[a] = t.vdepth[t.head[a]];
     d.adep = dev_upload(d, adep.data(), adep.size());
@@ -404,15 +428,19 @@ extern "C" int nwcch_customize(nwcch_dev* h, const int32_t* w, int32_t* ud, void
   try {
     DevTopo& d = h->d;
     hipStream_t s = (hipStream_t)stream;
+    int2* u = reinterpret_cast<int2*>(ud);
+    {
+      ProfScope ps(d, s, kKOther);
+      const i64 blocks = std::min<i64>(std::max<i64>((d.mp + 255) / 256, 1), 16384);
+      hipLaunchKernelGGL(k_base_all, dim3((unsigned)blocks), dim3(256), 0, s, d.mp, d.ae_off, d.ae_idx, w, u,
+                         d.status);
+    }
     for (const LevelRange& r : d.lv) {
-      const i64 nbig = r.big_end - r.begin, nsmall = r.end - r.big_end;
-      if (nbig + nsmall == 0) continue;
-      const int bb = (int)((nbig + 3) / 4);
-      const int bs = (int)((nsmall + 255) / 256);
+      if (r.end == r.begin || r.triangles == 0) continue;  // base values are final
+      const LevelLaunch L = level_launch(r);
       ProfScope ps(d, s, kKCustomize);
-      hipLaunchKernelGGL(k_customize_level, dim3(bb + bs), dim3(256), 0, s, r.begin, r.big_end, r.end,
-                         bb, d.sched_arc, d.tri_off, d.tri_ab, d.ae_off, d.ae_idx, w,
-                         reinterpret_cast<int2*>(ud), d.status);
+      hipLaunchKernelGGL(k_customize_level, dim3(L.blk[kTeamClasses]), dim3(256), 0, s, L, d.prec, d.tri_ab,
+                         u, d.status);
     }
     HIP_OK(hipGetLastError());
   } catch (const std::exception& e) {
@@ -472,10 +500,10 @@ extern "C" int nwcch_query_strided(nwcch_dev* h, const int32_t* ud, int64_t ud_s
     DevTopo& d = h->d;
     hipStream_t s = (hipStream_t)stream;
     const int32_t maxlen = d.depth + 1;
-    const size_t lds = (size_t)3 * maxlen * sizeof(int32_t);
+    const size_t lds = (size_t)2 * maxlen * sizeof(int32_t);
     if (lds > 160 * 1024) throw std::runtime_error("elimination tree too deep for the LDS query kernel");
     ProfScope ps(d, s, kKQuery);
-    hipLaunchKernelGGL(k_query, dim3((unsigned)nq), dim3(64), lds, s, d.vinfo, d.adep,
+    hipLaunchKernelGGL(k_query, dim3((unsigned)nq), dim3(64), lds, s, d.vinfo, d.euler, d.adep,
                        reinterpret_cast<const int2*>(ud), (i64)ud_stride, s_rank, t_rank, (i64)nq,
                        (int32_t)d.n, maxlen, dist, scanned, d.status);
     HIP_OK(hipGetLastError());

@@ -574,15 +574,20 @@ void build_index(Topology& t) {
   for (i64 p = 0; p < mp; p++) t.arc_pos[t.sched_arc[p]] = (i32)p;
   t.tri_off.assign(mp + 1, 0);
   for (i64 p = 0; p < mp; p++) t.tri_off[p + 1] = t.tri_off[p] + cnt[t.sched_arc[p]];
-  t.lv.assign(t.levels, LevelRange{0, 0, 0, 0});
+  t.lv.assign(t.levels, LevelRange{});
   {
     i64 p = 0;
     for (i32 L = 0; L < t.levels; L++) {
-      LevelRange r{p, p, p, 0};
+      LevelRange r{};
+      r.begin = p;
+      // team classes are contiguous: counts descend within the level
+      int k = 0;
       while (p < mp && t.vlevel[tail[t.sched_arc[p]]] == L) {
-        if (cnt[t.sched_arc[p]] >= kBigArc) r.big_end = p + 1;
+        const int kc = team_class(cnt[t.sched_arc[p]]);
+        while (k <= kc) r.cls[k++] = p;
         p++;
       }
+      while (k <= kTeamClasses) r.cls[k++] = p;
       r.end = p;
       r.triangles = t.tri_off[r.end] - t.tri_off[r.begin];
       t.lv[L] = r;

@@ -27,6 +27,14 @@ struct VInfo {
   int32_t first, deg, parent, depth;
 };
 
+// Packed per-schedule-position record of the level kernels: one 16-byte load
+// gives the arc and its slice of the triangle stream.
+struct alignas(16) PosRec {
+  i64 tri_begin;
+  int32_t arc;
+  int32_t tri_count;
+};
+
 struct DevTopo {
   int device = 0;
   i64 n = 0, m = 0, mp = 0, T = 0;
@@ -40,8 +48,10 @@ struct DevTopo {
   int2* tri_ab = nullptr;            // [T]
   int32_t* ae_off = nullptr;         // [mp+1] by arc id
   int32_t* ae_idx = nullptr;         // [#mapped original arcs]
+  PosRec* prec = nullptr;            // [mp] by schedule position
   // query
   VInfo* vinfo = nullptr;            // [n]
+  int2* euler = nullptr;             // [n] preorder interval of the ET subtree
   int32_t* adep = nullptr;           // [mp] ET depth of the arc's head
   // partial update (lazy)
   bool has_update = false;

@@ -15,12 +15,27 @@ namespace nwcch {
 using i32 = int32_t;
 using i64 = int64_t;
 
+// Lanes that evaluate one arc together ("team"): 64, 32, ..., 1 -- the smallest
+// power of two giving each lane at most kTriPerLane lower triangles (capped at a
+// whole wavefront).
+constexpr int kTeamClasses = 7;
+constexpr int kTriPerLane = 4;
+inline int team_class(i64 triangles) {  // 0 -> 64 lanes ... 6 -> 1 lane
+  int ts = 1;
+  while (ts < 64 && (i64)ts * kTriPerLane < triangles) ts <<= 1;
+  int k = 6;
+  while ((1 << (6 - k)) != ts) k--;
+  return k;
+}
+
 // One level of the customization schedule: schedule positions [begin, end) hold
-// the shortcuts whose lower endpoint has ET height `level`; positions
-// [begin, big_end) are the shortcuts with at least kBigArc lower triangles.
+// the shortcuts whose lower endpoint has ET height `level`, sorted by lower-
+// triangle count descending, so the arcs of team class k occupy positions
+// [cls[k], cls[k+1]).
 struct LevelRange {
-  i64 begin, big_end, end;
+  i64 begin, end;
   i64 triangles;  // lower triangles of all shortcuts in the level
+  i64 cls[kTeamClasses + 1];
 };
 
 struct Topology {
@@ -59,8 +74,6 @@ struct Topology {
   std::vector<i32> side_top;    // [2*T]
   std::vector<i32> arc_tail;    // [mp]
 };
-
-constexpr i64 kBigArc = 48;   // lower-triangle count from which one wavefront owns an arc
 
 // Builders (host_prep.cpp).  They throw std::runtime_error on bad input.
 void undirected_csr(i64 n, const i64* off, const i32* tgt, std::vector<i64>& noff,

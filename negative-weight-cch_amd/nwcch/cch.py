@@ -138,11 +138,11 @@ class CCH:
         self.ud = self.dev.customize(self.edge_w, out=self.ud)
         self._last = None
         if (self._loop_w < 0).any():
-            self.dev.check_status()
+            self.dev.check_status(negative_cycle=True)
             self.cycle_arc = -1
             return [-1]
         lst = self.dev.negcycle_list(self.ud)
-        self.dev.check_status()
+        self.dev.check_status(negative_cycle=bool(lst))
         if stop_on_negative_cycle and lst:
             # the reference checks the arcs of vertex x once a triangle with a
             # later "a" arc is reached (python/cch.py:270-278): vertices whose
@@ -180,7 +180,7 @@ class CCH:
                     self._loop_w[pos[k]] = v
         self._last = r
         self._last_changed = None
-        self.dev.check_status()
+        self.dev.check_status(negative_cycle=self.update_created_negative_cycle())
         return r[0]
 
     @property

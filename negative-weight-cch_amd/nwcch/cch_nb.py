@@ -118,7 +118,9 @@ def customize(n, first, head, e_arc, e_up, w):
     """(upw, dnw): customized shortcut weights (python/cch_nb.py:529-563)."""
     dt = device_topology(n, first, head, e_arc, e_up)
     ud = dt.customize(w)
-    dt.check_status()
+    st = dt.status(clear=False)
+    neg = bool(st & 1) and int(dt.negcycle_scan(ud)[0].item()) > 0
+    dt.check_status(negative_cycle=neg)
     return ud[:, 0], ud[:, 1]
 
 

@@ -42,6 +42,26 @@ def as_pairs(up, dn) -> torch.Tensor:
     return torch.stack([u, d], 1).contiguous()
 
 
+class StepGraph:
+    """A HIP graph of a fixed sequence of library calls on fixed device buffers
+    (e.g. the level launches of one customization plus the scan), captured once
+    and replayed with one launch.  `fn` must only enqueue work on the current
+    stream; the library records no memset nodes."""
+
+    def __init__(self, fn):
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):
+            fn()  # warm-up outside the capture
+        torch.cuda.current_stream().wait_stream(side)
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            fn()
+
+    def replay(self):
+        self.graph.replay()
+
+
 class DeviceTopology:
     """One topology uploaded to one GPU (owns the device copy)."""
 
@@ -222,8 +242,15 @@ class DeviceTopology:
         check(lib().nwcch_status(self._h, ptr(st), int(clear), _stream(stream)), "nwcch_status")
         return int(st[0])
 
-    def check_status(self, stream=None):
+    def check_status(self, stream=None, negative_cycle: bool = False):
+        """Raise on any sticky status bit.  With negative_cycle=True an int32
+        overflow is accepted: under a negative cycle the customized values run
+        away (the reference's float64 values would just keep falling), they are
+        undefined as distances, and the verdict -- computed with clamped values
+        that stay negative -- is what the caller gets."""
         st = self.status(clear=True, stream=stream)
+        if negative_cycle:
+            st &= ~1
         if st:
             bits = [v for k, v in _lib.STATUS_BITS.items() if st & k]
             raise NwcchError("device status: " + ", ".join(bits))
