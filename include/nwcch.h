@@ -124,23 +124,33 @@ int nwcch_query(nwcch_dev* h, const int32_t* ud, const int32_t* s_rank, const in
                 int64_t nq, int32_t* dist, int32_t* scanned, void* stream);
 
 /* The same on a metric stored with a stride: arc a's (up, down) pair is at
- * ud + 2 * a * ud_stride (ud_stride = K and ud = UD + 2k for metric k of a
- * batched [mp][K][2] customization). */
+ * ud + 2 * a * ud_stride (for metric k of a batched result with slab width SW:
+ * ud = UD + 2 * ((k / SW) * mp * SW + k % SW), ud_stride = SW).  `order`
+ * (device int32[nq] permutation, or NULL) is the query packing: wavefront i
+ * serves query order[i]; results stay at the query's own index. */
 int nwcch_query_strided(nwcch_dev* h, const int32_t* ud, int64_t ud_stride, const int32_t* s_rank,
-                        const int32_t* t_rank, int64_t nq, int32_t* dist, int32_t* scanned, void* stream);
+                        const int32_t* t_rank, int64_t nq, const int32_t* order, int32_t* dist,
+                        int32_t* scanned, void* stream);
+
+/* Slab width used for K metrics when `slab` (a power of two <= 64, 0 = the
+ * default 64) is requested: min(slab, K rounded up to a power of two). */
+int nwcch_batch_slab_width(int32_t K, int32_t slab);
 
 /* Batched customization of K metrics on one topology in one level sweep:
- * W[K][m] weights (device, metric-major) -> UD[mp][K][2] (device, arc-major:
- * the K pairs of one arc are contiguous).  Equals K calls of nwcch_customize,
- * i.e. K runs of cch_nb.customize (python/cch_nb.py:529-563) or of
- * CCH::customize (cpp/cch.cpp:437-463), which the reference runs one metric at a
- * time (python/run_nb.py:121-124; cpp/cch.cpp:805-816). */
-int nwcch_customize_batch(nwcch_dev* h, const int32_t* W, int32_t K, int32_t* UD, void* stream);
+ * W[K][m] weights (device, metric-major) -> UD (device) in metric slabs:
+ * int32 [ceil(K/SW)][mp][SW][2] with SW = nwcch_batch_slab_width(K, slab); with
+ * the default slab and K <= 64 this is [1][mp][K'][2], the K' >= K pairs of one
+ * arc contiguous.  Equals K calls of nwcch_customize, i.e. K runs of
+ * cch_nb.customize (python/cch_nb.py:529-563) or of CCH::customize
+ * (cpp/cch.cpp:437-463), which the reference runs one metric at a time
+ * (python/run_nb.py:121-124; cpp/cch.cpp:805-816). */
+int nwcch_customize_batch(nwcch_dev* h, const int32_t* W, int32_t K, int32_t slab, int32_t* UD, void* stream);
 
-/* Per-metric negative-cycle scan of UD[mp][K][2]: out (device int64[2K]) =
- * {count of metric 0..K-1, smallest witness arc of metric 0..K-1 or -1}
- * (python/cch_nb.py:566-572 per metric). */
-int nwcch_negcycle_scan_batch(const int32_t* UD, int64_t mp, int32_t K, int64_t* out, void* stream);
+/* Per-metric negative-cycle scan of a batched UD (same K and slab): out
+ * (device int64[2K]) = {count of metric 0..K-1, smallest witness arc of metric
+ * 0..K-1 or -1} (python/cch_nb.py:566-572 per metric). */
+int nwcch_negcycle_scan_batch(const int32_t* UD, int64_t mp, int32_t K, int32_t slab, int64_t* out,
+                              void* stream);
 
 /* Build (once, on the host) and upload the partial-update index of the topology
  * the handle was created from: for every arc the top arcs of the lower
@@ -172,6 +182,14 @@ int nwcch_update_changed(nwcch_dev* h, int32_t* out, int64_t cap, void* stream);
  * `stream`.  The reference raises Python exceptions / asserts instead
  * (python/cch.py:125, 218; python/preprocess.py:9-10). */
 int nwcch_status(nwcch_dev* h, uint32_t* out, int clear, void* stream);
+
+/* Tunables of the level kernels (no counterpart in the reference; set by the
+ * autotuner nwcch.tune): "tri_per_lane" (single-metric sweep: an arc gets the
+ * smallest power-of-two team of lanes, up to 64, with at most this many lower
+ * triangles per lane; default 4) and "batch_tri_per_lane" (batched sweep:
+ * wavefronts per arc, up to 16, likewise; default 8).  Values 1..256. */
+int nwcch_set_tuning(nwcch_dev* h, const char* key, int32_t value);
+int nwcch_get_tuning(const nwcch_dev* h, const char* key, int32_t* value);
 
 /* Launch profiling (the reference's Timing/now() role, cpp/cch.cpp:42-45,
  * 714-729, at kernel granularity): after nwcch_profile_begin every kernel launch

@@ -1,12 +1,15 @@
 // Batched customization: K metrics on one topology in one level sweep.
 //
-// Layout: UD[mp][K] int2 (arc-major, metric-minor), so the K values of one arc
-// are contiguous.  A wavefront owns one arc and a slice of KL = min(64, K
-// rounded up to a power of two) metrics: lane = (triangle sub-lane, metric),
-// each (a, b) pair of the shared triangle stream is read once per wavefront and
-// every metric lane gathers its own UD[a][k], UD[b][k] -- consecutive lanes read
-// consecutive 8-byte pairs, so the gathers are coalesced and the index stream is
-// amortised over the metrics.  Sub-lanes are reduced with cross-lane minima.
+// Layout: metric slabs.  With slab width SW = min(8, K rounded up to a power of
+// two) and S = ceil(K / SW) slabs, UD is int2 [S][mp][SW]: the SW values of one
+// arc in one slab are contiguous (64 bytes for SW = 8), and each slab is its
+// own contiguous array.  A wavefront holds the SW metrics of one slab x (64 /
+// SW) triangle sub-lanes, so each (a, b) pair of the shared triangle stream is
+// read once per wavefront-step and every lane gathers its metric's UD[a], UD[b]
+// -- SW consecutive lanes read one contiguous 8*SW-byte segment.  Blocks are
+// mapped slab = block mod S: workgroups are dispatched to the 8 XCDs round-
+// robin, so (for 8 slabs) each XCD works on one slab only and its L2 never
+// holds another XCD's data.
 // The level schedule, the no-atomics argument and the INF/overflow semantics
 // are those of the single-metric kernel (python/cch_nb.py:529-563 per metric).
 #include <hip/hip_runtime.h>
@@ -39,40 +42,130 @@ __device__ __forceinline__ int32_t inf_add_b(int32_t a, int32_t b, uint32_t& st)
   return (int32_t)s;
 }
 
-// W is metric-major [K][m]; UD arc-major [mp][K].
-template <int KL>
-__global__ void __launch_bounds__(256) k_customize_batch_level(
-    i64 pbeg, i64 pend, int32_t slices, const int32_t* __restrict__ sched_arc,
-    const i64* __restrict__ tri_off, const int2* __restrict__ tri_ab,
-    const int32_t* __restrict__ ae_off, const int32_t* __restrict__ ae_idx,
-    const int32_t* __restrict__ W, i64 m, int32_t K, int2* UD, uint32_t* status) {
-  constexpr int S = 64 / KL;  // triangle sub-lanes per metric
-  const int lane = threadIdx.x & 63;
-  const int km = lane % KL;
-  const int ts = lane / KL;
-  const i64 wv = (i64)blockIdx.x * 4 + (threadIdx.x >> 6);
-  const i64 p = pbeg + wv / slices;
-  if (p >= pend) return;
-  const int32_t metric = (int32_t)(wv % slices) * KL + km;
-  const bool live = metric < K;
-  const int32_t c = sched_arc[p];
+// Transpose of the weights W[K][m] -> WT[m][K] (64 x 64 tiles through LDS), so
+// the K weights of one original arc are contiguous like the K values of an arc.
+__global__ void __launch_bounds__(256) k_transpose_w(const int32_t* __restrict__ W, i64 m, int32_t K,
+                                                     int32_t* __restrict__ WT) {
+  __shared__ int32_t tile[64][65];
+  const i64 e0 = (i64)blockIdx.x * 64;
+  const int32_t k0 = blockIdx.y * 64;
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  for (int r = ty; r < 64; r += 4) {
+    const int32_t k = k0 + r;
+    const i64 e = e0 + tx;
+    tile[r][tx] = (k < K && e < m) ? W[(i64)k * m + e] : 0;
+  }
+  __syncthreads();
+  for (int r = ty; r < 64; r += 4) {
+    const i64 e = e0 + r;
+    const int32_t k = k0 + tx;
+    if (e < m && k < K) WT[e * K + k] = tile[tx][r];
+  }
+}
+
+// Base values of every (arc, metric) in the slab layout: UD[slab][c][km] = min
+// over the original arcs of c, per direction (python/cch_nb.py:534-542 per
+// metric).  Threads run metric-fastest, so reads of WT and writes of UD are
+// coalesced.
+__global__ void __launch_bounds__(256) k_base_batch(i64 mp, int32_t K, int32_t SW, int32_t slabs,
+                                                    const int32_t* __restrict__ ae_off,
+                                                    const int32_t* __restrict__ ae_idx,
+                                                    const int32_t* __restrict__ WT, int2* UD,
+                                                    uint32_t* status) {
   uint32_t st = 0;
-  int32_t up = kInf, dn = kInf;
-  const int32_t e0 = ae_off[c], e1 = ae_off[c + 1];
-  if (live) {
-    const int32_t* wk = W + (i64)metric * m;
-    for (int32_t j = e0 + ts; j < e1; j += S) {
+  const i64 total = (i64)slabs * mp * SW;
+  const i64 stride = (i64)gridDim.x * blockDim.x;
+  for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
+    const int32_t km = (int32_t)(i % SW);
+    const i64 rest = i / SW;
+    const i64 c = rest % mp;
+    const int32_t k = (int32_t)(rest / mp) * SW + km;
+    if (k >= K) continue;
+    int32_t up = kInf, dn = kInf;
+    const int32_t e1 = ae_off[c + 1];
+    for (int32_t j = ae_off[c]; j < e1; j++) {
       const int32_t e = ae_idx[j];
-      const int32_t x = wk[e >> 1];
+      const int32_t x = WT[(i64)(e >> 1) * K + k];
       if (x == (int32_t)0x80000000) st |= kStatusBadWeight;
       if (e & 1) up = min(up, x);
       else dn = min(dn, x);
     }
-    const i64 t1 = tri_off[p + 1];
-    for (i64 q = tri_off[p] + ts; q < t1; q += S) {
+    UD[i] = make_int2(up, dn);
+  }
+  if (st) atomicOr(status, st);
+}
+
+// Launch geometry of one level of the batched sweep: wave-team class j gives
+// WPA = 16 >> j wavefronts to each arc of positions [cls[j], cls[j+1]) (the
+// level's arcs are sorted by triangle count, descending) and owns blocks
+// [blk[j], blk[j+1]); every block is 16 wavefronts.
+constexpr int kWaveClasses = 5;
+constexpr int kSlabWidth = 64;  // default metrics per slab
+struct BatchLaunch {
+  i64 cls[kWaveClasses + 1];
+  int32_t blk[kWaveClasses + 1];
+};
+
+// One level of the batched sweep.  A wavefront holds KL metrics x (64 / KL)
+// triangle sub-lanes; WPA wavefronts of one block split an arc's triangles,
+// each lane keeping four (a, b) pairs -- eight gathers -- in flight per step;
+// sub-lanes meet through cross-lane minima and wavefronts through LDS.  Same
+// no-atomics argument as the single-metric level kernel.
+template <int KL>
+__global__ void __launch_bounds__(1024) k_customize_batch_level(BatchLaunch L, int32_t slices,
+                                                                const PosRec* __restrict__ prec,
+                                                                const int2* __restrict__ tri_ab, int32_t K,
+                                                                i64 mp, int2* UD, uint32_t* status) {
+  constexpr int S = 64 / KL;  // triangle sub-lanes per metric
+  __shared__ int2 red[16][64];
+  const int32_t b = blockIdx.x;
+  int j = 0;
+  while (j < kWaveClasses - 1 && b >= L.blk[j + 1]) j++;
+  const int wlog = 4 - j;
+  const int WPA = 1 << wlog;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int32_t local = b - L.blk[j];
+  const int32_t slice = local % slices;
+  const i64 grp = local / slices;
+  const i64 p = L.cls[j] + grp * (16 >> wlog) + (wave >> wlog);
+  const int chunk = wave & (WPA - 1);
+  const int km = lane % KL, ts = lane / KL;
+  const int32_t metric = slice * KL + km;
+  const bool live = p < L.cls[j + 1] && metric < K;
+  int2* slab = UD + (i64)slice * mp * KL + km;  // element of arc a: slab[a * KL]
+  uint32_t st = 0;
+  int32_t up = kInf, dn = kInf;
+  int32_t c = 0;
+  if (live) {
+    const PosRec r = prec[p];
+    c = r.arc;
+    if (chunk == 0 && ts == 0) {
+      const int2 base = slab[(i64)c * KL];
+      up = base.x;
+      dn = base.y;
+    }
+    const int32_t stride = WPA * S;
+    const i64 t1 = r.tri_begin + r.tri_count;
+    i64 q = r.tri_begin + chunk * S + ts;
+    for (; q + 3 * stride < t1; q += 4 * stride) {
+      int2 ab[4], A[4], B[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) ab[u] = tri_ab[q + u * stride];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        A[u] = slab[(i64)ab[u].x * KL];
+        B[u] = slab[(i64)ab[u].y * KL];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        up = min(up, inf_add_b(A[u].y, B[u].x, st));
+        dn = min(dn, inf_add_b(B[u].y, A[u].x, st));
+      }
+    }
+    for (; q < t1; q += stride) {
       const int2 ab = tri_ab[q];
-      const int2 A = UD[(i64)ab.x * K + metric];
-      const int2 B = UD[(i64)ab.y * K + metric];
+      const int2 A = slab[(i64)ab.x * KL];
+      const int2 B = slab[(i64)ab.y * KL];
       up = min(up, inf_add_b(A.y, B.x, st));
       dn = min(dn, inf_add_b(B.y, A.x, st));
     }
@@ -82,7 +175,18 @@ __global__ void __launch_bounds__(256) k_customize_batch_level(
     up = min(up, __shfl_xor(up, o, 64));
     dn = min(dn, __shfl_xor(dn, o, 64));
   }
-  if (live && ts == 0) UD[(i64)c * K + metric] = make_int2(up, dn);
+  if (WPA > 1) {  // uniform over the block
+    red[wave][lane] = make_int2(up, dn);
+    __syncthreads();
+    if (chunk == 0) {
+      for (int i = 1; i < WPA; i++) {
+        const int2 v = red[wave + i][lane];
+        up = min(up, v.x);
+        dn = min(dn, v.y);
+      }
+    }
+  }
+  if (live && chunk == 0 && ts == 0) slab[(i64)c * KL] = make_int2(up, dn);
   if (st) atomicOr(status, st);
 }
 
@@ -93,18 +197,20 @@ __global__ void k_batch_scan_init(long long* out, int32_t K) {
   }
 }
 
-// Per-metric negative-cycle scan over UD[mp][K]: out[k] = count, out[K + k] =
-// smallest witness arc (-1 if none).  Witnesses are rare, so they are counted
-// with atomics and the kernel is a pure stream.
-__global__ void __launch_bounds__(256) k_batch_scan(const int2* __restrict__ UD, i64 total, int32_t K,
-                                                    long long* out) {
+// Per-metric negative-cycle scan over the slab layout: out[k] = count,
+// out[K + k] = smallest witness arc (-1 if none).  Witnesses are rare, so they
+// are counted with atomics and the kernel is a pure stream.
+__global__ void __launch_bounds__(256) k_batch_scan(const int2* __restrict__ UD, i64 mp, int32_t K, int32_t SW,
+                                                    i64 total, long long* out) {
   const i64 stride = (i64)gridDim.x * blockDim.x;
   for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
     const int2 v = UD[i];
     if (v.x != kInf && v.y != kInf && (long long)v.x + (long long)v.y < 0) {
-      const int32_t k = (int32_t)(i % K);
+      const i64 rest = i / SW;
+      const int32_t k = (int32_t)(rest / mp) * SW + (int32_t)(i % SW);
+      if (k >= K) continue;
       atomicAdd((unsigned long long*)&out[k], 1ull);
-      atomicMin(&out[K + k], (long long)(i / K));
+      atomicMin(&out[K + k], (long long)(rest % mp));
     }
   }
 }
@@ -114,13 +220,47 @@ __global__ void k_batch_scan_finish(long long* out, int32_t K) {
     if (out[K + k] == 0x7fffffffffffffffll) out[K + k] = -1;
 }
 
+// Class boundaries of one level for S triangle sub-lanes per metric: the
+// smallest WPA (1..16) giving each lane at most R = batch_tri_per_lane
+// triangles.
+BatchLaunch batch_launch(const DevTopo& d, const LevelRange& r, int S, int32_t slices) {
+  BatchLaunch L;
+  const i64 R = d.batch_tri_per_lane;
+  // positions are sorted by count descending: class j holds counts in
+  // (limit(j+1), limit(j)] where limit(j) = (16 >> j) * S * R
+  i64 p = r.begin;
+  for (int j = 0; j < kWaveClasses; j++) {
+    L.cls[j] = p;
+    if (j == kWaveClasses - 1) { p = r.end; continue; }
+    const i64 lim = (i64)(16 >> (j + 1)) * S * R;  // upper bound of class j+1
+    // advance while count > lim (binary search over the descending counts)
+    i64 lo = p, hi = r.end;
+    while (lo < hi) {
+      const i64 mid = (lo + hi) / 2;
+      if (d.pos_cnt[mid] > lim) lo = mid + 1;
+      else hi = mid;
+    }
+    p = lo;
+  }
+  L.cls[kWaveClasses] = r.end;
+  int32_t b = 0;
+  for (int j = 0; j <= kWaveClasses; j++) {
+    L.blk[j] = b;
+    if (j < kWaveClasses) {
+      const i64 arcs = L.cls[j + 1] - L.cls[j];
+      const i64 per = 16 >> (4 - j);
+      b += (int32_t)(((arcs + per - 1) / per) * slices);
+    }
+  }
+  return L;
+}
+
 template <int KL>
-void launch_level(const DevTopo& d, const LevelRange& r, int32_t slices, const int32_t* W, int32_t K,
-                  int2* UD, hipStream_t s) {
-  const i64 waves = (r.end - r.begin) * slices;
-  const unsigned blocks = (unsigned)((waves + 3) / 4);
-  hipLaunchKernelGGL(k_customize_batch_level<KL>, dim3(blocks), dim3(256), 0, s, r.begin, r.end, slices,
-                     d.sched_arc, d.tri_off, d.tri_ab, d.ae_off, d.ae_idx, W, d.m, K, UD, d.status);
+void launch_level(const DevTopo& d, const LevelRange& r, int32_t slices, int32_t K, int2* UD, hipStream_t s) {
+  const BatchLaunch L = batch_launch(d, r, 64 / KL, slices);
+  if (L.blk[kWaveClasses] == 0) return;
+  hipLaunchKernelGGL(k_customize_batch_level<KL>, dim3(L.blk[kWaveClasses]), dim3(1024), 0, s, L, slices,
+                     d.prec, d.tri_ab, K, d.mp, UD, d.status);
 }
 
 }  // namespace
@@ -128,26 +268,64 @@ void launch_level(const DevTopo& d, const LevelRange& r, int32_t slices, const i
 
 using namespace nwcch;
 
-extern "C" int nwcch_customize_batch(nwcch_dev* h, const int32_t* W, int32_t K, int32_t* UD, void* stream) {
-  if (!h || !W || !UD || K <= 0) { set_error("bad argument"); return NWCCH_ERR_ARG; }
+namespace {
+// Effective slab width: the requested power of two (0 = the default), never
+// wider than K rounded up to a power of two.
+int slab_width(int32_t K, int32_t slab) {
+  if (slab <= 0) slab = kSlabWidth;
+  int SW = 1;
+  while (SW < K && SW < slab) SW <<= 1;
+  return SW;
+}
+}  // namespace
+
+extern "C" int nwcch_batch_slab_width(int32_t K, int32_t slab) {
+  if (K <= 0 || slab < 0 || slab > 64 || (slab & (slab - 1))) { set_error("bad slab width"); return NWCCH_ERR_ARG; }
+  return slab_width(K, slab);
+}
+
+extern "C" int nwcch_customize_batch(nwcch_dev* h, const int32_t* W, int32_t K, int32_t slab, int32_t* UD,
+                                     void* stream) {
+  if (!h || !W || !UD || K <= 0 || slab < 0 || slab > 64 || (slab & (slab - 1))) {
+    set_error("bad argument");
+    return NWCCH_ERR_ARG;
+  }
   try {
     DevTopo& d = h->d;
     hipStream_t s = (hipStream_t)stream;
-    int KL = 1;
-    while (KL < K && KL < 64) KL <<= 1;
+    const int KL = slab_width(K, slab);
     const int32_t slices = (K + KL - 1) / KL;
     int2* ud = reinterpret_cast<int2*>(UD);
+    // scratch for the transposed weights (grown on demand; not on a captured path)
+    const i64 need = d.m * (i64)K;
+    if (need > d.wt_cap) {
+      HIP_OK(hipStreamSynchronize(s));
+      int32_t* p = nullptr;
+      HIP_OK(hipMalloc((void**)&p, std::max<i64>(need, 1) * sizeof(int32_t)));
+      d.owned.push_back(p);
+      d.wt = p;
+      d.wt_cap = need;
+    }
+    {
+      ProfScope ps(d, s, kKOther);
+      dim3 grid((unsigned)((d.m + 63) / 64), (unsigned)((K + 63) / 64));
+      if (d.m > 0) hipLaunchKernelGGL(k_transpose_w, grid, dim3(256), 0, s, W, d.m, K, d.wt);
+      const i64 total = d.mp * (i64)KL * slices;
+      const i64 blocks = std::min<i64>(std::max<i64>((total + 255) / 256, 1), 65536);
+      hipLaunchKernelGGL(k_base_batch, dim3((unsigned)blocks), dim3(256), 0, s, d.mp, K, KL, slices, d.ae_off,
+                         d.ae_idx, d.wt, ud, d.status);
+    }
     for (const LevelRange& r : d.lv) {
-      if (r.end == r.begin) continue;
+      if (r.end == r.begin || r.triangles == 0) continue;
       ProfScope ps(d, s, kKBatch);
       switch (KL) {
-        case 1: launch_level<1>(d, r, slices, W, K, ud, s); break;
-        case 2: launch_level<2>(d, r, slices, W, K, ud, s); break;
-        case 4: launch_level<4>(d, r, slices, W, K, ud, s); break;
-        case 8: launch_level<8>(d, r, slices, W, K, ud, s); break;
-        case 16: launch_level<16>(d, r, slices, W, K, ud, s); break;
-        case 32: launch_level<32>(d, r, slices, W, K, ud, s); break;
-        default: launch_level<64>(d, r, slices, W, K, ud, s); break;
+        case 1: launch_level<1>(d, r, slices, K, ud, s); break;
+        case 2: launch_level<2>(d, r, slices, K, ud, s); break;
+        case 4: launch_level<4>(d, r, slices, K, ud, s); break;
+        case 8: launch_level<8>(d, r, slices, K, ud, s); break;
+        case 16: launch_level<16>(d, r, slices, K, ud, s); break;
+        case 32: launch_level<32>(d, r, slices, K, ud, s); break;
+        default: launch_level<64>(d, r, slices, K, ud, s); break;
       }
     }
     HIP_OK(hipGetLastError());
@@ -158,18 +336,23 @@ extern "C" int nwcch_customize_batch(nwcch_dev* h, const int32_t* W, int32_t K, 
   return NWCCH_OK;
 }
 
-extern "C" int nwcch_negcycle_scan_batch(const int32_t* UD, int64_t mp, int32_t K, int64_t* out,
+extern "C" int nwcch_negcycle_scan_batch(const int32_t* UD, int64_t mp, int32_t K, int32_t slab, int64_t* out,
                                          void* stream) {
-  if (!UD || !out || mp < 0 || K <= 0) { set_error("bad argument"); return NWCCH_ERR_ARG; }
+  if (!UD || !out || mp < 0 || K <= 0 || slab < 0 || slab > 64 || (slab & (slab - 1))) {
+    set_error("bad argument");
+    return NWCCH_ERR_ARG;
+  }
   try {
     hipStream_t s = (hipStream_t)stream;
     long long* o = (long long*)out;
+    const int SW = slab_width(K, slab);
+    const i64 slabs = (K + SW - 1) / SW;
     hipLaunchKernelGGL(k_batch_scan_init, dim3(1), dim3(256), 0, s, o, K);
-    const i64 total = mp * (i64)K;
+    const i64 total = mp * SW * slabs;
     if (total > 0) {
       const i64 blocks = std::min<i64>(std::max<i64>((total + 255) / 256, 1), 8192);
       hipLaunchKernelGGL(k_batch_scan, dim3((unsigned)blocks), dim3(256), 0, s,
-                         reinterpret_cast<const int2*>(UD), total, K, o);
+                         reinterpret_cast<const int2*>(UD), (i64)mp, K, SW, total, o);
     }
     hipLaunchKernelGGL(k_batch_scan_finish, dim3(1), dim3(256), 0, s, o, K);
     HIP_OK(hipGetLastError());

@@ -56,25 +56,44 @@ __device__ __forceinline__ uint32_t wave_or(uint32_t v) {
   return v;
 }
 
-// Launch geometry of one level: team class k (64 >> k lanes per arc) owns
-// schedule positions [cls[k], cls[k+1]) and blocks [blk[k], blk[k+1]).
-struct LevelLaunch {
-  i64 cls[kTeamClasses + 1];
-  int32_t blk[kTeamClasses + 1];
-};
-
-LevelLaunch level_launch(const LevelRange& r) {
+// Team classes of one level for R triangles per lane: class k (TS = 64 >> k
+// lanes) holds the arcs with (TS/2)*R < count <= TS*R (class 0 unbounded above,
+// class 6 down to 0).  A level's positions are sorted by count, descending, so
+// each boundary is a binary search.
+LevelLaunch level_launch(const DevTopo& d, const LevelRange& r, int32_t R) {
   LevelLaunch L;
+  i64 p = r.begin;
+  for (int k = 0; k < kTeamClasses; k++) {
+    L.cls[k] = p;
+    if (k == kTeamClasses - 1) break;
+    const i64 lim = (i64)(64 >> (k + 1)) * R;
+    i64 lo = p, hi = r.end;
+    while (lo < hi) {
+      const i64 mid = (lo + hi) / 2;
+      if (d.pos_cnt[mid] > lim) lo = mid + 1;
+      else hi = mid;
+    }
+    p = lo;
+  }
+  L.cls[kTeamClasses] = r.end;
   int32_t b = 0;
   for (int k = 0; k <= kTeamClasses; k++) {
-    L.cls[k] = r.cls[k];
     L.blk[k] = b;
     if (k < kTeamClasses) {
-      const i64 lanes = (r.cls[k + 1] - r.cls[k]) * (i64)(64 >> k);
+      const i64 lanes = (L.cls[k + 1] - L.cls[k]) * (i64)(64 >> k);
       b += (int32_t)((lanes + 255) / 256);
     }
   }
   return L;
+}
+
+const std::vector<LevelLaunch>& level_launches(DevTopo& d) {
+  if (d.launches_for != d.tri_per_lane) {
+    d.launches.resize(d.lv.size());
+    for (size_t i = 0; i < d.lv.size(); i++) d.launches[i] = level_launch(d, d.lv[i], d.tri_per_lane);
+    d.launches_for = d.tri_per_lane;
+  }
+  return d.launches;
 }
 
 // Base values of every arc (python/cch.py:234-247, python/cch_nb.py:534-542) in
@@ -236,12 +255,14 @@ __global__ void __launch_bounds__(64) k_query(const VInfo* __restrict__ vinfo,
                                               const int32_t* __restrict__ adep,
                                               const int2* __restrict__ ud, i64 ud_stride,
                                               const int32_t* __restrict__ qs,
-                                              const int32_t* __restrict__ qt, i64 nq, int32_t n,
+                                              const int32_t* __restrict__ qt, i64 nq,
+                                              const int32_t* __restrict__ order, int32_t n,
                                               int32_t maxlen, int32_t* __restrict__ dist,
                                               int32_t* __restrict__ scanned,
                                               uint32_t* status) {
   extern __shared__ int32_t lds[];
-  const i64 q = blockIdx.x;
+  // query packing: wavefront i serves query order[i] (e.g. longest chains first)
+  const i64 q = order ? (i64)order[blockIdx.x] : (i64)blockIdx.x;
   const int lane = threadIdx.x;
   const int half = lane >> 5;
   const int hl = lane & 31;
@@ -370,6 +391,8 @@ extern "C" int nwcch_dev_create(const nwcch_topo* topo, int device, nwcch_dev** 
     for (i64 p = 0; p < t.mp; p++)
       pr[p] = PosRec{t.tri_off[p], t.sched_arc[p], (int32_t)(t.tri_off[p + 1] - t.tri_off[p])};
     d.prec = dev_upload(d, pr.data(), pr.size());
+    d.pos_cnt.resize(t.mp);
+    for (i64 p = 0; p < t.mp; p++) d.pos_cnt[p] = pr[p].tri_count;
     std::vector<VInfo> vi(t.n);
     for (i64 x = 0; x < t.n; x++)
       vi[x] = VInfo{(int32_t)t.first[x], (int32_t)(t.first[x + 1] - t.first[x]), t.parent[x], t.vdepth[x]};
@@ -435,9 +458,11 @@ extern "C" int nwcch_customize(nwcch_dev* h, const int32_t* w, int32_t* ud, void
       hipLaunchKernelGGL(k_base_all, dim3((unsigned)blocks), dim3(256), 0, s, d.mp, d.ae_off, d.ae_idx, w, u,
                          d.status);
     }
-    for (const LevelRange& r : d.lv) {
+    const std::vector<LevelLaunch>& launches = level_launches(d);
+    for (size_t i = 0; i < d.lv.size(); i++) {
+      const LevelRange& r = d.lv[i];
       if (r.end == r.begin || r.triangles == 0) continue;  // base values are final
-      const LevelLaunch L = level_launch(r);
+      const LevelLaunch& L = launches[i];
       ProfScope ps(d, s, kKCustomize);
       hipLaunchKernelGGL(k_customize_level, dim3(L.blk[kTeamClasses]), dim3(256), 0, s, L, d.prec, d.tri_ab,
                          u, d.status);
@@ -489,13 +514,14 @@ extern "C" int nwcch_negcycle_list(const int32_t* ud, int64_t mp, int64_t* list,
 }
 
 extern "C" int nwcch_query_strided(nwcch_dev* h, const int32_t* ud, int64_t ud_stride, const int32_t* s_rank,
-                                   const int32_t* t_rank, int64_t nq, int32_t* dist, int32_t* scanned,
-                                   void* stream) {
+                                   const int32_t* t_rank, int64_t nq, const int32_t* order, int32_t* dist,
+                                   int32_t* scanned, void* stream) {
   if (!h || !ud || !s_rank || !t_rank || !dist || nq < 0 || ud_stride < 1) {
     set_error("null argument");
     return NWCCH_ERR_ARG;
   }
   if (nq == 0) return NWCCH_OK;
+  if (nq > 0x7fffffffll) { set_error("at most 2^31-1 queries per launch"); return NWCCH_ERR_CAPACITY; }
   try {
     DevTopo& d = h->d;
     hipStream_t s = (hipStream_t)stream;
@@ -504,7 +530,7 @@ extern "C" int nwcch_query_strided(nwcch_dev* h, const int32_t* ud, int64_t ud_s
     if (lds > 160 * 1024) throw std::runtime_error("elimination tree too deep for the LDS query kernel");
     ProfScope ps(d, s, kKQuery);
     hipLaunchKernelGGL(k_query, dim3((unsigned)nq), dim3(64), lds, s, d.vinfo, d.euler, d.adep,
-                       reinterpret_cast<const int2*>(ud), (i64)ud_stride, s_rank, t_rank, (i64)nq,
+                       reinterpret_cast<const int2*>(ud), (i64)ud_stride, s_rank, t_rank, (i64)nq, order,
                        (int32_t)d.n, maxlen, dist, scanned, d.status);
     HIP_OK(hipGetLastError());
   } catch (const std::exception& e) {
@@ -515,7 +541,26 @@ extern "C" int nwcch_query_strided(nwcch_dev* h, const int32_t* ud, int64_t ud_s
 
 extern "C" int nwcch_query(nwcch_dev* h, const int32_t* ud, const int32_t* s_rank, const int32_t* t_rank,
                            int64_t nq, int32_t* dist, int32_t* scanned, void* stream) {
-  return nwcch_query_strided(h, ud, 1, s_rank, t_rank, nq, dist, scanned, stream);
+  return nwcch_query_strided(h, ud, 1, s_rank, t_rank, nq, nullptr, dist, scanned, stream);
+}
+
+extern "C" int nwcch_set_tuning(nwcch_dev* h, const char* key, int32_t value) {
+  if (!h || !key) { set_error("null argument"); return NWCCH_ERR_ARG; }
+  const std::string k(key);
+  if (value < 1 || value > 256) { set_error("tuning value out of range [1, 256]"); return NWCCH_ERR_ARG; }
+  if (k == "tri_per_lane") h->d.tri_per_lane = value;
+  else if (k == "batch_tri_per_lane") h->d.batch_tri_per_lane = value;
+  else { set_error("unknown tuning key: " + k); return NWCCH_ERR_ARG; }
+  return NWCCH_OK;
+}
+
+extern "C" int nwcch_get_tuning(const nwcch_dev* h, const char* key, int32_t* value) {
+  if (!h || !key || !value) { set_error("null argument"); return NWCCH_ERR_ARG; }
+  const std::string k(key);
+  if (k == "tri_per_lane") *value = h->d.tri_per_lane;
+  else if (k == "batch_tri_per_lane") *value = h->d.batch_tri_per_lane;
+  else { set_error("unknown tuning key: " + k); return NWCCH_ERR_ARG; }
+  return NWCCH_OK;
 }
 
 extern "C" int nwcch_profile_begin(nwcch_dev* h, int64_t max_launches) {

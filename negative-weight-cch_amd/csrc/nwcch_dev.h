@@ -27,6 +27,14 @@ struct VInfo {
   int32_t first, deg, parent, depth;
 };
 
+// Launch geometry of one level of the single-metric sweep: team class k
+// (64 >> k lanes per arc) owns schedule positions [cls[k], cls[k+1]) and blocks
+// [blk[k], blk[k+1]).
+struct LevelLaunch {
+  i64 cls[kTeamClasses + 1];
+  int32_t blk[kTeamClasses + 1];
+};
+
 // Packed per-schedule-position record of the level kernels: one 16-byte load
 // gives the arc and its slice of the triangle stream.
 struct alignas(16) PosRec {
@@ -49,6 +57,14 @@ struct DevTopo {
   int32_t* ae_off = nullptr;         // [mp+1] by arc id
   int32_t* ae_idx = nullptr;         // [#mapped original arcs]
   PosRec* prec = nullptr;            // [mp] by schedule position
+  std::vector<int32_t> pos_cnt;      // [mp] host: triangles per schedule position
+  // tunables (nwcch_set_tuning) and the launch geometry derived from them
+  int32_t tri_per_lane = kTriPerLane;        // single-metric level kernel
+  int32_t batch_tri_per_lane = 8;            // batched level kernel
+  std::vector<LevelLaunch> launches;         // per level, for tri_per_lane
+  int32_t launches_for = -1;
+  int32_t* wt = nullptr;             // batched weights transposed to [m][K] (scratch)
+  i64 wt_cap = 0;
   // query
   VInfo* vinfo = nullptr;            // [n]
   int2* euler = nullptr;             // [n] preorder interval of the ET subtree

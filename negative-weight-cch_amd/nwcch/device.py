@@ -124,19 +124,24 @@ class DeviceTopology:
         return sorted(lst.cpu().tolist())
 
     def query(self, ud: torch.Tensor, s_rank, t_rank, want_scanned=True, stream=None, metric: int = 0,
-              dist: torch.Tensor | None = None):
+              dist: torch.Tensor | None = None, order: torch.Tensor | None = None):
         """ET queries between ranks -> (dist int32 [nq], scanned int32 [nq] or None).
-        ud is one metric [mp, 2] or a batch [mp, K, 2] (then `metric` picks one)."""
+        ud is one metric [mp, 2] or a batched result [slabs, mp, SW, 2] (then
+        `metric` picks one).  `order` (device int32 permutation) packs the
+        queries: wavefront i serves query order[i]."""
         qs = torch.as_tensor(s_rank).to(self.dev, torch.int32).contiguous()
         qt = torch.as_tensor(t_rank).to(self.dev, torch.int32).contiguous()
         if qs.shape != qt.shape or qs.dim() != 1:
             raise ValueError("s_rank and t_rank must be 1-d of equal length")
-        if ud.dim() == 3:
-            K = ud.shape[1]
-            if not 0 <= metric < K:
+        if order is not None:
+            assert order.dtype == torch.int32 and order.is_cuda and order.shape == qs.shape
+        if ud.dim() == 4:
+            slabs, _, SW, _ = ud.shape
+            if not 0 <= metric < slabs * SW:
                 raise ValueError("metric out of range")
-            assert ud.shape == (self.mp, K, 2) and ud.is_contiguous()
-            base, stride = ud.data_ptr() + 8 * metric, K
+            assert ud.shape == (slabs, self.mp, SW, 2) and ud.is_contiguous()
+            base = ud.data_ptr() + 8 * ((metric // SW) * self.mp * SW + metric % SW)
+            stride = SW
         else:
             assert ud.shape == (self.mp, 2) and ud.is_contiguous()
             base, stride = ud.data_ptr(), 1
@@ -144,9 +149,27 @@ class DeviceTopology:
         if dist is None:
             dist = torch.empty(nq, dtype=torch.int32, device=self.dev)
         sc = torch.empty(nq, dtype=torch.int32, device=self.dev) if want_scanned else None
-        check(lib().nwcch_query_strided(self._h, base, stride, ptr(qs), ptr(qt), nq, ptr(dist), ptr(sc),
-                                        _stream(stream)), "nwcch_query_strided")
+        check(lib().nwcch_query_strided(self._h, base, stride, ptr(qs), ptr(qt), nq, ptr(order), ptr(dist),
+                                        ptr(sc), _stream(stream)), "nwcch_query_strided")
         return dist, sc
+
+    def chain_order(self, s_rank, t_rank) -> torch.Tensor:
+        """Query packing: the permutation serving the longest ancestor chains
+        (depth(s) + depth(t)) first, so the longest queries start earliest."""
+        if not hasattr(self, "_depth"):
+            self._depth = torch.as_tensor(self.topo.export()["depth"]).to(self.dev)
+        qs = torch.as_tensor(s_rank).to(self.dev, torch.long)
+        qt = torch.as_tensor(t_rank).to(self.dev, torch.long)
+        key = self._depth[qs] + self._depth[qt]
+        return torch.argsort(key, descending=True).to(torch.int32)
+
+    def set_tuning(self, key: str, value: int):
+        check(lib().nwcch_set_tuning(self._h, key.encode(), int(value)), "nwcch_set_tuning")
+
+    def get_tuning(self, key: str) -> int:
+        v = np.zeros(1, np.int32)
+        check(lib().nwcch_get_tuning(self._h, key.encode(), ptr(v)), "nwcch_get_tuning")
+        return int(v[0])
 
     # ---------------------------------------------------------------- batched
     def weights_batch(self, W) -> torch.Tensor:
@@ -160,22 +183,40 @@ class DeviceTopology:
             W = W64.to(self.dev, torch.int32)
         return W.contiguous()
 
-    def customize_batch(self, W, out: torch.Tensor | None = None, stream=None) -> torch.Tensor:
-        """K metrics at once: W [K, m] -> UD [mp, K, 2] (arc-major)."""
+    def batch_shape(self, K: int, slab: int = 0):
+        """Shape of the batched result: int32 [slabs, mp, SW, 2] with slab width
+        SW = min(slab or 64, K rounded up to a power of two), slabs = ceil(K/SW)."""
+        sw = lib().nwcch_batch_slab_width(int(K), int(slab))
+        if sw < 0:
+            raise ValueError("slab must be 0 or a power of two <= 64")
+        return ((K + sw - 1) // sw, self.mp, sw, 2)
+
+    @staticmethod
+    def metric_view(UD: torch.Tensor, k: int) -> torch.Tensor:
+        """The [mp, 2] (up, down) values of metric k of a batched result."""
+        sw = UD.shape[2]
+        return UD[k // sw, :, k % sw, :]
+
+    def customize_batch(self, W, out: torch.Tensor | None = None, slab: int = 0, stream=None) -> torch.Tensor:
+        """K metrics in one level sweep: W [K, m] -> UD [slabs, mp, SW, 2]."""
         W = self.weights_batch(W)
         K = W.shape[0]
+        shape = self.batch_shape(K, slab)
         if out is None:
-            out = torch.empty((self.mp, K, 2), dtype=torch.int32, device=self.dev)
-        assert out.shape == (self.mp, K, 2) and out.dtype == torch.int32 and out.is_contiguous()
-        check(lib().nwcch_customize_batch(self._h, ptr(W), K, ptr(out), _stream(stream)), "nwcch_customize_batch")
+            out = torch.empty(shape, dtype=torch.int32, device=self.dev)
+        assert tuple(out.shape) == shape and out.dtype == torch.int32 and out.is_contiguous()
+        check(lib().nwcch_customize_batch(self._h, ptr(W), K, int(slab), ptr(out), _stream(stream)),
+              "nwcch_customize_batch")
         return out
 
-    def negcycle_scan_batch(self, UD: torch.Tensor, out: torch.Tensor | None = None, stream=None):
-        """Device int64 [2, K]: per-metric witness counts and smallest witness (or -1)."""
-        K = UD.shape[1]
+    def negcycle_scan_batch(self, UD: torch.Tensor, K: int, out: torch.Tensor | None = None, stream=None):
+        """Device int64 [2, K]: per-metric witness counts and smallest witness (or
+        -1).  K is the number of metrics (the last slab may be partly unused)."""
+        slab = UD.shape[2]
+        assert tuple(UD.shape) == self.batch_shape(K, slab)
         if out is None:
             out = torch.empty((2, K), dtype=torch.int64, device=self.dev)
-        check(lib().nwcch_negcycle_scan_batch(ptr(UD), UD.shape[0], K, ptr(out), _stream(stream)),
+        check(lib().nwcch_negcycle_scan_batch(ptr(UD), self.mp, K, int(slab), ptr(out), _stream(stream)),
               "nwcch_negcycle_scan_batch")
         return out
 

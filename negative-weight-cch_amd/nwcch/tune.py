@@ -1,0 +1,84 @@
+"""Autotuner of the launch shapes of the level kernels and of the query packing.
+
+Every candidate is timed on the caller's own topology, metric(s) and queries
+with HIP events, and its output is compared with the first candidate's: a shape
+changes only how work is spread over lanes and wavefronts, never a result.
+The winning values are set on the handle (``DeviceTopology.set_tuning``); the
+slab width of a batched layout is returned for the caller to pass on.
+"""
+from __future__ import annotations
+
+import statistics
+
+import torch
+
+from .device import DeviceTopology
+
+
+def _ms(fn, reps: int) -> float:
+    fn()
+    evs = []
+    for _ in range(reps):
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record()
+        fn()
+        b.record()
+        evs.append((a, b))
+    torch.cuda.synchronize()
+    return statistics.median(a.elapsed_time(b) for a, b in evs)
+
+
+def tune_customize(dt: DeviceTopology, w, candidates=(1, 2, 4, 8, 16), reps: int = 5) -> dict:
+    """Triangles per lane of the single-metric level kernel."""
+    w = dt.weights(w) if not (isinstance(w, torch.Tensor) and w.is_cuda) else w
+    ref = None
+    times = {}
+    for r in candidates:
+        dt.set_tuning("tri_per_lane", r)
+        out = torch.empty((dt.mp, 2), dtype=torch.int32, device=dt.dev)
+        times[r] = _ms(lambda: dt.customize(w, out=out), reps)
+        if ref is None:
+            ref = out.clone()
+        elif not torch.equal(ref, out):
+            raise AssertionError(f"tri_per_lane={r} changed the customization")
+    best = min(times, key=times.get)
+    dt.set_tuning("tri_per_lane", best)
+    dt.check_status()
+    return {"tri_per_lane": best, "ms": times}
+
+
+def tune_batch(dt: DeviceTopology, W, slabs=(64, 16, 8), tris=(4, 8, 16), reps: int = 3) -> dict:
+    """Slab width and triangles per lane of the batched sweep."""
+    W = dt.weights_batch(W)
+    K = W.shape[0]
+    times = {}
+    ref = None
+    for sw in slabs:
+        for r in tris:
+            dt.set_tuning("batch_tri_per_lane", r)
+            out = torch.empty(dt.batch_shape(K, sw), dtype=torch.int32, device=dt.dev)
+            times[(sw, r)] = _ms(lambda: dt.customize_batch(W, out=out, slab=sw), reps)
+            check = [DeviceTopology.metric_view(out, k) for k in (0, K - 1)]
+            if ref is None:
+                ref = [c.clone() for c in check]
+            elif not all(torch.equal(a, b) for a, b in zip(ref, check)):
+                raise AssertionError(f"slab={sw} batch_tri_per_lane={r} changed the customization")
+    (sw, r) = min(times, key=times.get)
+    dt.set_tuning("batch_tri_per_lane", r)
+    return {"slab": sw, "batch_tri_per_lane": r, "ms": {f"{a}/{b}": v for (a, b), v in times.items()}}
+
+
+def tune_queries(dt: DeviceTopology, ud, s_rank, t_rank, reps: int = 3) -> dict:
+    """Query packing: launch order as given, or longest ancestor chains first."""
+    qs = torch.as_tensor(s_rank).to(dt.dev, torch.int32)
+    qt = torch.as_tensor(t_rank).to(dt.dev, torch.int32)
+    dist = torch.empty(qs.shape[0], dtype=torch.int32, device=dt.dev)
+    order = dt.chain_order(qs, qt)
+    t_plain = _ms(lambda: dt.query(ud, qs, qt, want_scanned=False, dist=dist), reps)
+    ref = dist.clone()
+    t_pack = _ms(lambda: dt.query(ud, qs, qt, want_scanned=False, dist=dist, order=order), reps)
+    if not torch.equal(ref, dist):
+        raise AssertionError("query packing changed a distance")
+    best = "longest_first" if t_pack < t_plain else "none"
+    return {"packing": best, "ms": {"none": t_plain, "longest_first": t_pack}}

@@ -13,10 +13,11 @@ pytestmark = pytest.mark.gpu
 INF = O.INF
 
 
+@pytest.mark.parametrize("slab", [0, 8])
 @pytest.mark.parametrize("K", [1, 5, 16, 64, 70])
-def test_batch_equals_single(c0, K):
+def test_batch_equals_single(c0, K, slab):
     g, rank = c0
-    print(f"case: batch K={K} on config 0", flush=True)
+    print(f"case: batch K={K} slab={slab} on config 0", flush=True)
     W, _ = graphs.metric_family(g, K, seed=K, hmax=400)
     bad = K // 2
     tri = graphs.find_directed_triangle(g)
@@ -29,15 +30,16 @@ def test_batch_equals_single(c0, K):
         W[bad] = graphs.inject_negative_cycle(W[bad], tri)
     topo = _lib.Topology.from_graph(g.n, g.off, g.tgt, rank)
     dt = DeviceTopology(topo)
-    UD = dt.customize_batch(W)
-    sc = dt.negcycle_scan_batch(UD).cpu().numpy()
+    UD = dt.customize_batch(W, slab=slab)
+    sc = dt.negcycle_scan_batch(UD, K).cpu().numpy()
     st = O.Structure(g.n, g.off, g.tgt, rank)
-    got = UD.cpu().numpy()
+    assert tuple(UD.shape) == dt.batch_shape(K, slab)
     s, t = graphs.random_pairs(g.n, 300, seed=K)
     for k in range(K):
         up, dn, _ = st.customize(W[k])
-        assert np.array_equal(got[:, k, 0], up), k
-        assert np.array_equal(got[:, k, 1], dn), k
+        got = dt.metric_view(UD, k).cpu().numpy()
+        assert np.array_equal(got[:, 0], up), k
+        assert np.array_equal(got[:, 1], dn), k
         cnt, first = st.negcycle(up, dn)
         assert (sc[0, k], sc[1, k]) == (cnt, first), k
         assert (cnt > 0) == (k == bad)
