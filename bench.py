@@ -299,6 +299,51 @@ def batched(args, out):
     del dt
 
 
+def serving(args, out):
+    """Config 3: 16 resident metrics on the 1.1M-point graph (seed 3), Poisson
+    query streams with interleaved 1/10/1000-arc updates; p50/p99 latency and
+    throughput per arrival rate, every answer re-checked alone."""
+    import numpy as np
+
+    from nwcch import graphs, serve
+    from nwcch.device import DeviceTopology
+
+    g, rank, topo, host = prepare(3)
+    K = 16
+    W, _ = graphs.metric_family(g, K, seed=3, hmax=800)
+    dt = DeviceTopology(topo, 0)
+    srv = serve.Server(dt, rank, metric_budget=K * (4 * topo.m + 8 * topo.mp), slot_budget=64 << 20,
+                       max_batch=65536, n_streams=2)
+    w0 = {}
+    for k in range(K):
+        if not srv.add_metric(k, W[k]):
+            raise RuntimeError(f"metric {k} was not admitted")
+        w0[k] = W[k].copy()
+    runs = []
+    all_reqs = []
+    for i, rate in enumerate(args.serve_rates):
+        reqs = serve.poisson_workload(g, K, rate=rate, duration=args.serve_seconds, seed=30 + i,
+                                      update_rate=args.serve_update_rate)
+        reqs = serve.absolute_updates(reqs, {k: srv.weights_at(k, srv.metrics[k].version, w0[k])
+                                             for k in range(K)})
+        st = srv.run(reqs, max_seconds=60 + 10 * args.serve_seconds)
+        st["arrival_rate"] = rate
+        runs.append(st)
+        all_reqs += reqs
+        log(f"serving {rate:.0f}/s: {st['queries_per_s']:.0f} q/s, p50 {st['query_p50_ms']:.3f} ms, "
+            f"p99 {st['query_p99_ms']:.3f} ms, batch {st['mean_batch']:.0f}, updates {st['updates']} "
+            f"(rejected {st['rejected_updates']}), update p99 {st['update_p99_ms']:.2f} ms")
+    ver = srv.verify_alone(all_reqs, w0) if not args.no_check else None
+    if ver is not None:
+        log(f"serving: every answer vs alone recompute: {ver}")
+    out["serving"] = {
+        "workload": f"config 3: n={g.n} m={g.m} m+={topo.mp} T={topo.triangles} depth={topo.depth}; {K} "
+                    f"resident metrics; Poisson queries (20% within 5 km) + 1/10/1000-arc updates",
+        "runs": runs, "verify_alone": ver, "host": host,
+    }
+    del srv, dt
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -313,6 +358,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (library defaults)")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--serve-rates", type=float, nargs="+", default=[200_000, 1_000_000, 2_000_000])
+    ap.add_argument("--serve-seconds", type=float, default=0.25)
+    ap.add_argument("--serve-update-rate", type=float, default=40.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU design is "
@@ -327,6 +376,8 @@ def main():
     plain(args, out)
     if not args.no_batched:
         batched(args, out)
+    if not args.no_serving:
+        serving(args, out)
     print(json.dumps(out), flush=True)
 
 

@@ -136,6 +136,15 @@ int nwcch_query_strided(nwcch_dev* h, const int32_t* ud, int64_t ud_stride, cons
  * default 64) is requested: min(slab, K rounded up to a power of two). */
 int nwcch_batch_slab_width(int32_t K, int32_t slab);
 
+/* Queries on several resident one-metric customizations in one launch: query i
+ * uses metric qmetric[i] (device int32), whose ud[mp][2] device address is
+ * ud_table[qmetric[i]] (device int64[n_metrics]).  A metric index outside
+ * [0, n_metrics) gives INF and NWCCH_STATUS_BAD_QUERY.  The serving loop packs
+ * all metrics' admitted queries into one such launch. */
+int nwcch_query_multi(nwcch_dev* h, const int64_t* ud_table, int32_t n_metrics, const int32_t* qmetric,
+                      const int32_t* s_rank, const int32_t* t_rank, int64_t nq, int32_t* dist, int32_t* scanned,
+                      void* stream);
+
 /* Batched customization of K metrics on one topology in one level sweep:
  * W[K][m] weights (device, metric-major) -> UD (device) in metric slabs:
  * int32 [ceil(K/SW)][mp][SW][2] with SW = nwcch_batch_slab_width(K, slab); with

@@ -253,7 +253,9 @@ __global__ void k_scan_finish(const unsigned long long* src, long long* dst) {
 __global__ void __launch_bounds__(64) k_query(const VInfo* __restrict__ vinfo,
                                               const int2* __restrict__ euler,
                                               const int32_t* __restrict__ adep,
-                                              const int2* __restrict__ ud, i64 ud_stride,
+                                              const int2* __restrict__ ud_one, i64 ud_stride,
+                                              const long long* __restrict__ ud_table,
+                                              const int32_t* __restrict__ qmetric, int32_t n_metrics,
                                               const int32_t* __restrict__ qs,
                                               const int32_t* __restrict__ qt, i64 nq,
                                               const int32_t* __restrict__ order, int32_t n,
@@ -269,7 +271,11 @@ __global__ void __launch_bounds__(64) k_query(const VInfo* __restrict__ vinfo,
   if (q >= nq) return;
   uint32_t st = 0;
   const int32_t rs = qs[q], rt = qt[q];
-  if (rs < 0 || rs >= n || rt < 0 || rt >= n) {
+  // one metric for the launch, or a per-query metric out of a table of metrics
+  const int32_t mq = ud_table ? qmetric[q] : 0;
+  const int2* __restrict__ ud =
+      ud_table ? ((mq >= 0 && mq < n_metrics) ? reinterpret_cast<const int2*>(ud_table[mq]) : nullptr) : ud_one;
+  if (rs < 0 || rs >= n || rt < 0 || rt >= n || !ud) {
     if (lane == 0) {
       dist[q] = kInf;
       if (scanned) scanned[q] = 0;
@@ -435,6 +441,8 @@ extern "C" void nwcch_dev_destroy(nwcch_dev* h) {
   (void)hipSetDevice(h->d.device);
   (void)hipDeviceSynchronize();
   for (hipEvent_t e : h->d.ev_pool) (void)hipEventDestroy(e);
+  if (h->d.stage_ev) (void)hipEventDestroy(h->d.stage_ev);
+  for (void* p : h->d.owned_host) (void)hipHostFree(p);
   for (void* p : h->d.owned) (void)hipFree(p);
   delete h;
 }
@@ -513,13 +521,10 @@ extern "C" int nwcch_negcycle_list(const int32_t* ud, int64_t mp, int64_t* list,
   return NWCCH_OK;
 }
 
-extern "C" int nwcch_query_strided(nwcch_dev* h, const int32_t* ud, int64_t ud_stride, const int32_t* s_rank,
-                                   const int32_t* t_rank, int64_t nq, const int32_t* order, int32_t* dist,
-                                   int32_t* scanned, void* stream) {
-  if (!h || !ud || !s_rank || !t_rank || !dist || nq < 0 || ud_stride < 1) {
-    set_error("null argument");
-    return NWCCH_ERR_ARG;
-  }
+namespace {
+int launch_queries(nwcch_dev* h, const int32_t* ud, int64_t ud_stride, const int64_t* ud_table,
+                   const int32_t* qmetric, int32_t n_metrics, const int32_t* s_rank, const int32_t* t_rank,
+                   int64_t nq, const int32_t* order, int32_t* dist, int32_t* scanned, void* stream) {
   if (nq == 0) return NWCCH_OK;
   if (nq > 0x7fffffffll) { set_error("at most 2^31-1 queries per launch"); return NWCCH_ERR_CAPACITY; }
   try {
@@ -530,13 +535,36 @@ extern "C" int nwcch_query_strided(nwcch_dev* h, const int32_t* ud, int64_t ud_s
     if (lds > 160 * 1024) throw std::runtime_error("elimination tree too deep for the LDS query kernel");
     ProfScope ps(d, s, kKQuery);
     hipLaunchKernelGGL(k_query, dim3((unsigned)nq), dim3(64), lds, s, d.vinfo, d.euler, d.adep,
-                       reinterpret_cast<const int2*>(ud), (i64)ud_stride, s_rank, t_rank, (i64)nq, order,
-                       (int32_t)d.n, maxlen, dist, scanned, d.status);
+                       reinterpret_cast<const int2*>(ud), (i64)ud_stride, (const long long*)ud_table, qmetric,
+                       n_metrics, s_rank, t_rank, (i64)nq, order, (int32_t)d.n, maxlen, dist, scanned,
+                       d.status);
     HIP_OK(hipGetLastError());
   } catch (const std::exception& e) {
     return fail(e);
   }
   return NWCCH_OK;
+}
+}  // namespace
+
+extern "C" int nwcch_query_strided(nwcch_dev* h, const int32_t* ud, int64_t ud_stride, const int32_t* s_rank,
+                                   const int32_t* t_rank, int64_t nq, const int32_t* order, int32_t* dist,
+                                   int32_t* scanned, void* stream) {
+  if (!h || !ud || !s_rank || !t_rank || !dist || nq < 0 || ud_stride < 1) {
+    set_error("null argument");
+    return NWCCH_ERR_ARG;
+  }
+  return launch_queries(h, ud, ud_stride, nullptr, nullptr, 0, s_rank, t_rank, nq, order, dist, scanned, stream);
+}
+
+extern "C" int nwcch_query_multi(nwcch_dev* h, const int64_t* ud_table, int32_t n_metrics, const int32_t* qmetric,
+                                 const int32_t* s_rank, const int32_t* t_rank, int64_t nq, int32_t* dist,
+                                 int32_t* scanned, void* stream) {
+  if (!h || !ud_table || n_metrics <= 0 || !qmetric || !s_rank || !t_rank || !dist || nq < 0) {
+    set_error("null argument");
+    return NWCCH_ERR_ARG;
+  }
+  return launch_queries(h, nullptr, 1, ud_table, qmetric, n_metrics, s_rank, t_rank, nq, nullptr, dist, scanned,
+                        stream);
 }
 
 extern "C" int nwcch_query(nwcch_dev* h, const int32_t* ud, const int32_t* s_rank, const int32_t* t_rank,

@@ -87,7 +87,11 @@ struct DevTopo {
   long long* upd_res = nullptr;      // [4] evaluated, changed, negative changed, first negative
   i64* stage_e = nullptr;            // staging of the changes (grown on demand)
   int32_t* stage_w = nullptr;
+  i64* hstage_e = nullptr;           // pinned host side of the staging
+  int32_t* hstage_w = nullptr;
   i64 stage_cap = 0;
+  hipEvent_t stage_ev = nullptr;     // the last staging copy
+  std::vector<void*> owned_host;     // pinned host allocations
   int32_t epoch = 0;
   // scratch
   uint32_t* status = nullptr;        // [1]

@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstring>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -302,6 +303,11 @@ extern "C" int nwcch_update(nwcch_dev* h, int32_t* w, int32_t* ud, const int64_t
       const int32_t a = d.e_arc_host[e];
       if (a >= 0) L0 = std::min(L0, d.arc_lvl_host[a]);
     }
+    // the changes travel through pinned host staging, so the caller's arrays
+    // may be released as soon as this call returns; the staging is reused only
+    // after the previous call's copies have completed
+    if (d.stage_ev) HIP_OK(hipEventSynchronize(d.stage_ev));
+    else HIP_OK(hipEventCreateWithFlags(&d.stage_ev, hipEventDisableTiming));
     if (k > d.stage_cap) {
       i64 cap = std::max<i64>(k, 2 * d.stage_cap);
       HIP_OK(hipStreamSynchronize(s));
@@ -311,13 +317,24 @@ extern "C" int nwcch_update(nwcch_dev* h, int32_t* w, int32_t* ud, const int64_t
       HIP_OK(hipMalloc((void**)&w2, cap * sizeof(int32_t)));
       d.owned.push_back(e2);
       d.owned.push_back(w2);
+      void* he = nullptr;
+      void* hw = nullptr;
+      HIP_OK(hipHostMalloc(&he, cap * sizeof(i64), hipHostMallocDefault));
+      HIP_OK(hipHostMalloc(&hw, cap * sizeof(int32_t), hipHostMallocDefault));
+      d.owned_host.push_back(he);
+      d.owned_host.push_back(hw);
       d.stage_e = e2;
       d.stage_w = w2;
+      d.hstage_e = (i64*)he;
+      d.hstage_w = (int32_t*)hw;
       d.stage_cap = cap;
     }
     if (k > 0) {
-      HIP_OK(hipMemcpyAsync(d.stage_e, edge_idx, k * sizeof(i64), hipMemcpyHostToDevice, s));
-      HIP_OK(hipMemcpyAsync(d.stage_w, new_w, k * sizeof(int32_t), hipMemcpyHostToDevice, s));
+      std::memcpy(d.hstage_e, edge_idx, k * sizeof(i64));
+      std::memcpy(d.hstage_w, new_w, k * sizeof(int32_t));
+      HIP_OK(hipMemcpyAsync(d.stage_e, d.hstage_e, k * sizeof(i64), hipMemcpyHostToDevice, s));
+      HIP_OK(hipMemcpyAsync(d.stage_w, d.hstage_w, k * sizeof(int32_t), hipMemcpyHostToDevice, s));
+      HIP_OK(hipEventRecord(d.stage_ev, s));
     }
     d.epoch++;
     UpdArgs A = make_args(d, w, ud, changed, changed_cap);

@@ -54,6 +54,7 @@ def lib():
             "nwcch_update_changed": ([vp, p, i64, vp], C.c_int),
             "nwcch_profile_end": ([vp, p, p, i64, p], C.c_int),
             "nwcch_query_strided": ([vp, p, i64, p, p, i64, p, p, p, vp], C.c_int),
+            "nwcch_query_multi": ([vp, p, i32, p, p, p, i64, p, p, vp], C.c_int),
             "nwcch_customize_batch": ([vp, p, i32, i32, p, vp], C.c_int),
             "nwcch_negcycle_scan_batch": ([p, i64, i32, i32, p, vp], C.c_int),
             "nwcch_batch_slab_width": ([i32, i32], C.c_int),
@@ -102,7 +103,7 @@ def exported_symbols():
             "nwcch_query", "nwcch_status", "nwcch_profile_begin", "nwcch_profile_end",
             "nwcch_update_prepare", "nwcch_update", "nwcch_update_changed", "nwcch_query_strided",
             "nwcch_customize_batch", "nwcch_negcycle_scan_batch", "nwcch_batch_slab_width",
-            "nwcch_set_tuning", "nwcch_get_tuning"]
+            "nwcch_set_tuning", "nwcch_get_tuning", "nwcch_query_multi"]
 
 
 # ------------------------------------------------------------ host topology

@@ -153,6 +153,22 @@ class DeviceTopology:
                                         ptr(sc), _stream(stream)), "nwcch_query_strided")
         return dist, sc
 
+    def query_raw(self, ud: torch.Tensor, qs: torch.Tensor, qt: torch.Tensor, n: int, dist: torch.Tensor,
+                  stream) -> None:
+        """The first n queries of device int32 rank arrays qs/qt on a one-metric ud,
+        into dist, with no argument conversion (the serving loop's launch path)."""
+        check(lib().nwcch_query_strided(self._h, ud.data_ptr(), 1, qs.data_ptr(), qt.data_ptr(), n, None,
+                                        dist.data_ptr(), None, stream.cuda_stream), "nwcch_query_strided")
+
+    def query_multi(self, table: torch.Tensor, qmetric: torch.Tensor, qs: torch.Tensor, qt: torch.Tensor, n: int,
+                    dist: torch.Tensor, stream=None) -> None:
+        """The first n queries, query i on the metric whose ud address is
+        table[qmetric[i]] (device int64 table, int32 metric indices)."""
+        assert table.dtype == torch.int64 and qmetric.dtype == torch.int32
+        check(lib().nwcch_query_multi(self._h, table.data_ptr(), table.numel(), qmetric.data_ptr(), qs.data_ptr(),
+                                      qt.data_ptr(), n, dist.data_ptr(), None, _stream(stream)),
+              "nwcch_query_multi")
+
     def chain_order(self, s_rank, t_rank) -> torch.Tensor:
         """Query packing: the permutation serving the longest ancestor chains
         (depth(s) + depth(t)) first, so the longest queries start earliest."""

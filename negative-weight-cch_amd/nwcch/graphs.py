@@ -312,9 +312,9 @@ def inject_negative_cycle(w: np.ndarray, arcs, total=-1):
 CONFIGS = {
     0: dict(kind="grid", side=64, seed=0),
     1: dict(kind="road", n_points=264_000, seed=1, hmax=800),
-    2: dict(kind="road", n_points=321_000, seed=2, hmax=800),
-    3: dict(kind="road", n_points=1_100_000, seed=3, hmax=800),
-    4: dict(kind="road", n_points=23_900_000, seed=4, hmax=3000),
+    2: dict(kind="road", n_points=321_000, seed=2, hmax=800, out_degree=2.5),
+    3: dict(kind="road", n_points=1_100_000, seed=3, hmax=800, out_degree=2.45),
+    4: dict(kind="road", n_points=23_900_000, seed=4, hmax=3000, out_degree=2.44),
 }
 
 
