@@ -312,8 +312,8 @@ def serving(args, out):
     K = 16
     W, _ = graphs.metric_family(g, K, seed=3, hmax=800)
     dt = DeviceTopology(topo, 0)
-    srv = serve.Server(dt, rank, metric_budget=K * (4 * topo.m + 8 * topo.mp), slot_budget=64 << 20,
-                       max_batch=65536, n_streams=2)
+    srv = serve.Server(dt, rank, metric_budget=K * 2 * (4 * topo.m + 8 * topo.mp), slot_budget=64 << 20,
+                       max_batch=65536, n_streams=2, snapshot=True)
     w0 = {}
     for k in range(K):
         if not srv.add_metric(k, W[k]):
@@ -339,6 +339,7 @@ def serving(args, out):
     out["serving"] = {
         "workload": f"config 3: n={g.n} m={g.m} m+={topo.mp} T={topo.triangles} depth={topo.depth}; {K} "
                     f"resident metrics; Poisson queries (20% within 5 km) + 1/10/1000-arc updates",
+        "update_mode": "snapshot (updates on a second copy, committed by swap)",
         "runs": runs, "verify_alone": ver, "host": host,
     }
     del srv, dt

@@ -8,14 +8,18 @@ cycle-creating update by applying the inverse update (docs/technical_note.md:147
 Here every admitted request becomes GPU work that overlaps with the rest:
 
 * metrics are admitted while their device footprint fits a memory budget;
-* queries that have arrived are packed into one launch per metric on a pool of
-  streams, as long as query slots (device + pinned host buffers, sized by a
-  memory budget) are free -- requests arriving while others run join the next
+* the queries of all metrics that have arrived are packed into one launch
+  (each query names its metric through a device table of metric addresses),
+  with at most `max_inflight` launches on the device and launch buffers sized
+  by a memory budget -- requests arriving while others run join the next
   launch (continuous batching);
-* an update of metric k runs on the update stream after the in-flight queries
-  of metric k (stream events) and blocks new launches for metric k until its
-  changed-arc negative-cycle verdict is known; a cycle-creating update is
-  undone by the inverse update and reported as rejected;
+* an update of metric k runs on the update stream after the query launches
+  already issued.  In snapshot mode (the default) it works on a second copy of
+  the metric while queries keep reading the committed version; an accepted
+  update is committed by swapping the copies, a cycle-creating one is simply
+  dropped.  In strict mode it works in place, metric k takes no new queries
+  until the changed-arc negative-cycle verdict is known, and a cycle-creating
+  update is undone by the inverse update;
 * completion times come from HIP events (device clock), so latency is
   arrival -> result on the device.
 
@@ -61,56 +65,64 @@ class _Metric:
     w: torch.Tensor                # device int32 [m]
     ud: torch.Tensor               # device int32 [mp, 2]
     w_host: np.ndarray             # host mirror of w
+    w_spare: torch.Tensor | None = None    # snapshot mode: the copy an update works on
+    ud_spare: torch.Tensor | None = None
     version: int = 0
     history: list = dataclasses.field(default_factory=list)  # accepted updates, in order
     busy_update: object = None     # in-flight update state
 
 
 class _Launch:
-    """Buffers of one in-flight query launch (queries of any metrics)."""
+    """Buffers of one in-flight query launch (queries of any metrics) and the
+    metric address table it was issued with."""
 
-    def __init__(self, cap: int, dev):
+    def __init__(self, cap: int, dev, n_table: int):
         self.cap = cap
         self.h_q = torch.empty((3, cap), dtype=torch.int32, pin_memory=True)   # s, t, metric slot
         self.d_q = torch.empty((3, cap), dtype=torch.int32, device=dev)
+        self.h_tab = torch.empty(n_table, dtype=torch.int64, pin_memory=True)  # metric addresses
+        self.d_tab = torch.empty(n_table, dtype=torch.int64, device=dev)
         self.d_dist = torch.empty(cap, dtype=torch.int32, device=dev)
         self.h_dist = torch.empty(cap, dtype=torch.int32, pin_memory=True)
         self.event = torch.cuda.Event(enable_timing=True)
         self.ids = np.zeros(0, np.int64)   # request indices of the launch
 
     @staticmethod
-    def nbytes(cap: int) -> int:
-        return cap * (12 + 4) * 2
+    def nbytes(cap: int, n_table: int = 64) -> int:
+        return cap * (12 + 4) * 2 + 16 * n_table
 
 
 class Server:
     """metric_budget: device bytes for resident metrics (weights + customized
-    pairs); slot_budget: bytes for query launch buffers (device + pinned host),
-    which bounds the launches and queries in flight.  Query launches alternate
-    over `n_streams` streams, updates run on their own stream (all streams at
-    the default priority)."""
+    pairs, twice in snapshot mode); slot_budget: bytes for query launch buffers
+    (device + pinned host), which bounds the launches and queries in flight.
+    Query launches alternate over `n_streams` streams, updates run on their own
+    stream (all streams at the default priority)."""
 
     def __init__(self, dt: DeviceTopology, rank: np.ndarray, metric_budget: int, slot_budget: int,
-                 max_batch: int = 65536, n_streams: int = 2, max_metrics: int = 64):
+                 max_batch: int = 65536, n_streams: int = 2, max_metrics: int = 64, snapshot: bool = True):
         self.dt = dt
         self.rank = np.asarray(rank, np.int32)
         self.metric_budget = int(metric_budget)
+        self.snapshot = bool(snapshot)
         self.metrics: dict[int, _Metric] = {}
         self.used = 0
         dt.prepare_update()
-        n_launch = max(1, int(slot_budget) // _Launch.nbytes(max_batch))
-        self.launch_bufs = [_Launch(max_batch, dt.dev) for _ in range(n_launch)]
+        n_launch = max(1, int(slot_budget) // _Launch.nbytes(max_batch, max_metrics))
+        self.launch_bufs = [_Launch(max_batch, dt.dev, max_metrics) for _ in range(n_launch)]
         self.free_bufs = list(range(n_launch))
         self.streams = [torch.cuda.Stream() for _ in range(n_streams)]
         self.upd_stream = torch.cuda.Stream()
-        self.table = torch.zeros(max_metrics, dtype=torch.int64, device=dt.dev)   # metric addresses
+        # committed address of every metric's customization, copied into each
+        # launch with its queries (a commit never races with a running launch)
+        self.h_table = np.zeros(max_metrics, np.int64)
         self.free_table = list(range(max_metrics))
         self._rr = 0
         self.last_query_events: list = []
 
     # ----------------------------------------------------------- metrics
     def metric_bytes(self) -> int:
-        return 4 * self.dt.m + 8 * self.dt.mp
+        return (2 if self.snapshot else 1) * (4 * self.dt.m + 8 * self.dt.mp)
 
     def add_metric(self, mid: int, w) -> bool:
         """Customize and keep metric `mid` resident; False if it does not fit the
@@ -125,8 +137,11 @@ class Server:
         if neg or st:
             return False
         slot = self.free_table.pop(0)
-        self.table[slot] = ud.data_ptr()
-        self.metrics[mid] = _Metric(mid, slot, wd, ud, np.asarray(w, np.int32).copy())
+        self.h_table[slot] = ud.data_ptr()
+        m = _Metric(mid, slot, wd, ud, np.asarray(w, np.int32).copy())
+        if self.snapshot:
+            m.w_spare, m.ud_spare = torch.empty_like(wd), torch.empty_like(ud)
+        self.metrics[mid] = m
         self.used += need
         return True
 
@@ -134,7 +149,7 @@ class Server:
         m = self.metrics.pop(mid, None)
         if m is not None:
             torch.cuda.synchronize()
-            self.table[m.slot] = 0
+            self.h_table[m.slot] = 0
             self.free_table.append(m.slot)
             self.used -= self.metric_bytes()
 
@@ -145,8 +160,9 @@ class Server:
         Request objects.  At most `max_inflight` query launches are on the
         device; queries of all metrics that arrive meanwhile wait and go into
         the next launch together, so launches grow with the load (continuous
-        batching).  A metric with an update in flight takes no new queries
-        until its verdict is known."""
+        batching).  A metric runs one update at a time; in strict mode it also
+        takes no new queries until that update's verdict is known, in snapshot
+        mode its queries go on at the committed version meanwhile."""
         reqs = sorted(requests, key=lambda r: r.arrival)
         unknown = {r.metric for r in reqs} - set(self.metrics)
         if unknown:
@@ -203,7 +219,8 @@ class Server:
                     self._poll_update(m, start, done)
                     if m.busy_update is not None:
                         busy = True
-                        continue
+                        if not self.snapshot:
+                            continue
                 ids, arr, upd_pos, head = queues[k]
                 if head >= len(ids):
                     continue
@@ -212,6 +229,8 @@ class Server:
                 if head >= adm:
                     continue
                 if not is_q[ids[head]]:
+                    if m.busy_update is not None:
+                        continue
                     r = reqs[ids[head]]
                     r._idx = int(ids[head])
                     queues[k][3] = head + 1
@@ -261,11 +280,13 @@ class Server:
         hq[0, :n] = s_rank[batch]
         hq[1, :n] = t_rank[batch]
         hq[2, :n] = slot_of[batch]
+        L.h_tab.numpy()[:] = self.h_table
         stream = self.streams[self._rr % len(self.streams)]
         self._rr += 1
         with torch.cuda.stream(stream):
             L.d_q[:, :n].copy_(L.h_q[:, :n], non_blocking=True)
-            self.dt.query_multi(self.table, L.d_q[2], L.d_q[0], L.d_q[1], n, L.d_dist, stream)
+            L.d_tab.copy_(L.h_tab, non_blocking=True)
+            self.dt.query_multi(L.d_tab, L.d_q[2], L.d_q[0], L.d_q[1], n, L.d_dist, stream)
             L.h_dist[:n].copy_(L.d_dist[:n], non_blocking=True)
             L.event.record(stream)
         L.ids = batch
@@ -279,7 +300,14 @@ class Server:
             s.wait_event(ev)
         old = m.w_host[r.edges].copy()
         with torch.cuda.stream(s):
-            res = self.dt.update(m.w, m.ud, r.edges, r.weights, stream=s)
+            if self.snapshot:
+                # the spare copy is free: every launch that read it as the
+                # committed version was issued before the waits above
+                m.w_spare.copy_(m.w)
+                m.ud_spare.copy_(m.ud)
+                res = self.dt.update(m.w_spare, m.ud_spare, r.edges, r.weights, stream=s)
+            else:
+                res = self.dt.update(m.w, m.ud, r.edges, r.weights, stream=s)
             host = torch.empty(4, dtype=torch.int64, pin_memory=True)
             host.copy_(res, non_blocking=True)
             ev = torch.cuda.Event(enable_timing=True)
@@ -298,15 +326,24 @@ class Server:
             if done is not None and hasattr(restore_of, "_idx"):
                 done[restore_of._idx] = restore_of.done
             return
-        m.w_host[r.edges] = r.weights
         r.evaluated, r.changed = evaluated, changed
         if neg > 0:
-            # reject: apply the inverse update (Theorem 4 restores bitwise)
             r.rejected = True
-            inv = Request("update", m.mid, r.arrival, edges=r.edges, weights=old)
-            m.w_host[r.edges] = old
-            self._launch_update(m, inv, restore_of=r)
+            if not self.snapshot:
+                # apply the inverse update in place (Theorem 4 restores bitwise)
+                inv = Request("update", m.mid, r.arrival, edges=r.edges, weights=old)
+                self._launch_update(m, inv, restore_of=r)
+                return
+            # the committed copy was never touched: drop the spare
+            r.done = start.elapsed_time(ev) / 1000.0
+            if done is not None and hasattr(r, "_idx"):
+                done[r._idx] = r.done
             return
+        m.w_host[r.edges] = r.weights
+        if self.snapshot:
+            m.w, m.w_spare = m.w_spare, m.w
+            m.ud, m.ud_spare = m.ud_spare, m.ud
+            self.h_table[m.slot] = m.ud.data_ptr()
         m.version += 1
         m.history.append((r.edges.copy(), np.asarray(r.weights, np.int32).copy()))
         r.version = m.version

@@ -1,7 +1,8 @@
 """Serving: continuous batching of queries with interleaved updates.  Every
 served answer equals the answer computed alone on a from-scratch customization
 of the metric version it was served on, and matches the CPU oracle; a
-cycle-creating update is rejected and undone; admission respects the budgets."""
+cycle-creating update is rejected (dropped in snapshot mode, undone in strict
+mode); admission respects the budgets."""
 import numpy as np
 import pytest
 import torch
@@ -18,15 +19,17 @@ def _setup(g, rank):
     return DeviceTopology(topo)
 
 
-def test_served_answers_equal_alone(c0):
+@pytest.mark.parametrize("snapshot", [True, False])
+def test_served_answers_equal_alone(c0, snapshot):
     g, rank = c0
-    print("case: serving on config 0, 4 metrics", flush=True)
+    print(f"case: serving on config 0, 4 metrics, snapshot={snapshot}", flush=True)
     dt = _setup(g, rank)
     K = 4
     W, _ = graphs.metric_family(g, K, seed=9, hmax=60)
     # +-500 changes on 1 m grid arcs close negative cycles now and then: those
     # updates are rejected and undone, the rest become new metric versions
-    srv = serve.Server(dt, rank, metric_budget=1 << 30, slot_budget=1 << 24, max_batch=512, n_streams=2)
+    srv = serve.Server(dt, rank, metric_budget=1 << 30, slot_budget=1 << 24, max_batch=512, n_streams=2,
+                       snapshot=snapshot)
     w0 = {}
     for k in range(K):
         assert srv.add_metric(k, W[k])
@@ -49,11 +52,12 @@ def test_served_answers_equal_alone(c0):
         assert int(d[0]) == r.dist
 
 
-def test_cycle_creating_update_is_rejected(c0):
+@pytest.mark.parametrize("snapshot", [True, False])
+def test_cycle_creating_update_is_rejected(c0, snapshot):
     g, rank = c0
-    print("case: serving reject-and-restore", flush=True)
+    print(f"case: serving reject-and-restore, snapshot={snapshot}", flush=True)
     dt = _setup(g, rank)
-    srv = serve.Server(dt, rank, metric_budget=1 << 30, slot_budget=1 << 22, max_batch=256)
+    srv = serve.Server(dt, rank, metric_budget=1 << 30, slot_budget=1 << 22, max_batch=256, snapshot=snapshot)
     assert srv.add_metric(0, g.w)
     before = srv.metrics[0].ud.clone()
     k = int(g.off[100])
@@ -78,8 +82,9 @@ def test_cycle_creating_update_is_rejected(c0):
 def test_admission_by_memory(c0):
     g, rank = c0
     dt = _setup(g, rank)
-    one = 4 * dt.m + 8 * dt.mp
+    one = 2 * (4 * dt.m + 8 * dt.mp)          # snapshot mode keeps two copies
     srv = serve.Server(dt, rank, metric_budget=2 * one + 10, slot_budget=1, max_batch=64)
+    assert srv.metric_bytes() == one
     assert srv.add_metric(0, g.w) and srv.add_metric(1, g.w)
     assert not srv.add_metric(2, g.w)        # over the metric budget
     srv.drop_metric(1)
