@@ -49,6 +49,28 @@ def cust_bytes(m, mp, T, mapped, K=1):
     return 8 * T + 16 * mp + 4 * mapped + 4 * (mp + 1) + K * (4 * m + 16 * mp)
 
 
+def level_bytes(mp, T):
+    """Compulsory HBM bytes of all level launches of one customization: every
+    lower triangle's two operand arc ids (8 B), and per arc its base (up, down)
+    pair read and its final pair written (16 B).  Schedule records and per-
+    triangle arc ids are layout overhead and not counted."""
+    return 8 * T + 16 * mp
+
+
+def read_traffic(kernel, algorithmic):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary (FETCH_SIZE + WRITE_SIZE, scaled by a plain copy of known size);
+    None when there is no summary for it."""
+    p = ROOT / "profiles" / f"{kernel}_traffic.md"
+    if not p.exists():
+        return None
+    for line in p.read_text().splitlines():
+        if line.startswith("traffic_bytes_per_launch:"):
+            v = float(line.split(":", 1)[1])
+            return {"bytes_per_launch": v, "vs_algorithmic": v / algorithmic, "source": str(p.relative_to(ROOT))}
+    return None
+
+
 def timed(fn, reps):
     import torch
     torch.cuda.synchronize()
@@ -115,19 +137,50 @@ def plain(args, out):
     # timed region: K steps between two synchronisations
     t_step = timed(step, args.steps)
     # the same step replayed from one HIP graph
-    graph = StepGraph(step)
-    timed(graph.replay, args.warmup)
-    t_graph = timed(graph.replay, args.steps)
+    if args.no_graph:
+        t_graph = float("inf")
+    else:
+        graph = StepGraph(step)
+        timed(graph.replay, args.warmup)
+        t_graph = timed(graph.replay, args.steps)
     dt.check_status()
     # device time of the customization alone, per call (back-to-back launches)
     cust_ms = event_ms(lambda: dt.customize(w, out=ud), args.steps)
     scan_ms = event_ms(lambda: dt.negcycle_scan(ud, out=scan), args.steps)
-    # the base pass + one launch per level that has lower triangles
-    launches = 1 + int(np.count_nonzero(np.asarray(level_triangles(topo)) > 0))
+    # per-launch durations of the level kernel: HIP events recorded by the
+    # library around each launch, on the stream the launch goes to
+    n_lv = int(np.count_nonzero(np.asarray(level_triangles(topo)) > 0))
+    dt.profile_begin(args.steps * (n_lv + 2))
+    for _ in range(args.steps):
+        dt.customize(w, out=ud)
+    kid, kms = dt.profile_end()
+    base_ms = float(np.median(kms[kid == 5]))
+    launches = 1 + n_lv          # the base pass + one launch per level with triangles
     best = min(t_step, t_graph)
     B = cust_bytes(g.m, topo.mp, topo.triangles, topo.n_mapped)
+    B_lv = level_bytes(topo.mp, topo.triangles)
     cmed = statistics.median(cust_ms)
-    achieved = B / (cmed / 1000.0) / 1e9
+    # average level launch: the back-to-back customization minus its base pass
+    # (events around every launch would add their own gaps)
+    lv_avg_us = 1000.0 * (cmed - base_ms) / n_lv
+    achieved = (B_lv / n_lv) / (lv_avg_us * 1e-6) / 1e9
+    traffic = read_traffic("k_customize_level", B_lv / n_lv)
+    if args.copy_mib:
+        # a plain device copy of known size, for scaling the HBM byte counters
+        src = torch.ones(args.copy_mib << 18, dtype=torch.int32, device=dt.dev)
+        torch.empty_like(src).copy_(src)
+        torch.cuda.synchronize()
+        del src
+    # the drop-in path on host buffers: weights uploaded, customized, scanned,
+    # the (up, down) pairs copied back (PCIe included)
+    w_host = g.w.astype(np.int32)
+
+    def host_step():
+        u = dt.customize(w_host)
+        dt.negcycle_scan(u, out=scan)
+        return u.cpu()
+    host_step()
+    t_host = timed(host_step, max(1, args.steps // 2))
 
     # ET queries
     qs_h, qt_h = graphs.random_pairs(g.n, args.queries, seed=7)
@@ -173,11 +226,21 @@ def plain(args, out):
                    "n": g.n, "m": g.m, "m_plus": topo.mp, "triangles": topo.triangles, "depth": topo.depth,
                    "levels": topo.levels, "metrics": 1, "negative_arc_fraction": round(g.negative_fraction(), 4)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_customize_level",
-                     "algorithmic_bytes": B, "launches_per_customization": launches,
-                     "avg_launch_us": 1000.0 * cmed / launches, "customize_ms": cmed},
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic["bytes_per_launch"] if traffic else None,
+                     "traffic_vs_algorithmic": traffic["vs_algorithmic"] if traffic else None,
+                     "traffic_source": traffic["source"] if traffic else None, "kernel": "k_customize_level",
+                     "algorithmic_bytes_per_launch": B_lv / n_lv, "avg_launch_us": lv_avg_us,
+                     "avg_launch_us_isolated": 1000.0 * float(kms[kid == 0].astype(np.float64).mean()),
+                     "base_pass_us": 1000.0 * base_ms,
+                     "level_launches": n_lv, "launches_per_customization": launches,
+                     "customization_algorithmic_bytes": B, "customize_ms": cmed,
+                     "customization_GB_s": B / (cmed / 1000.0) / 1e9},
         "cpu_baseline": cpu,
-        "step_ms": {"eager": 1000.0 * t_step, "hip_graph": 1000.0 * t_graph},
+        "host_buffers": {"ms": 1000.0 * t_host, "triangles_per_s": topo.triangles / t_host,
+                         "what": "customize + scan with int32 weights from host memory and the (up, down) "
+                                 "pairs copied back to host memory (PCIe included)"},
+        "step_ms": {"eager": 1000.0 * t_step, "hip_graph": None if args.no_graph else 1000.0 * t_graph},
         "scan": {"ms": statistics.median(scan_ms),
                  "GB_s": 8 * topo.mp / (statistics.median(scan_ms) / 1000.0) / 1e9},
         "queries": {"batch": args.queries, "batch_ms": qms, "queries_per_s": args.queries / (qms / 1000.0),
@@ -271,6 +334,15 @@ def batched(args, out):
     t_s = timed(single_all, 2)
     bms = statistics.median(event_ms(lambda: dt.customize_batch(Wd, out=UD, slab=slab), args.batch_steps))
     B = cust_bytes(g.m, topo.mp, topo.triangles, topo.n_mapped, K)
+    n_lv = int(np.count_nonzero(np.asarray(level_triangles(topo)) > 0))
+    dt.profile_begin(args.batch_steps * (n_lv + 4))
+    for _ in range(args.batch_steps):
+        dt.customize_batch(Wd, out=UD, slab=slab)
+    kid, kms = dt.profile_end()
+    bbase_ms = float(np.median(kms[kid == 5]))
+    blv_us = 1000.0 * (bms - bbase_ms) / n_lv
+    B_blv = 8 * topo.triangles + K * 16 * topo.mp   # operand ids shared, 16 B per arc per metric
+    b_ach = (B_blv / n_lv) / (blv_us * 1e-6) / 1e9
     flagged = [int(k) for k in np.nonzero(sc[0].cpu().numpy() > 0)[0]]
     parity = None
     if not args.no_check:
@@ -290,9 +362,13 @@ def batched(args, out):
         "metrics": K, "ms_per_step": 1000 * t_b, "triangles_per_s": K * topo.triangles / t_b,
         "plain_ms_for_all_metrics": 1000 * t_s, "plain_triangles_per_s": K * topo.triangles / t_s,
         "speedup_vs_plain": t_s / t_b,
-        "roofline": {"kernel": "k_customize_batch_level", "achieved": B / (bms / 1000.0) / 1e9,
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": B / (bms / 1000.0) / 1e9 / HBM_PEAK_GBS,
-                     "algorithmic_bytes": B, "customize_ms": bms},
+        "roofline": {"kernel": "k_customize_batch_level", "bound": "hbm", "achieved": b_ach,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": b_ach / HBM_PEAK_GBS,
+                     "algorithmic_bytes_per_launch": B_blv / n_lv, "avg_launch_us": blv_us, "level_launches": n_lv,
+                     "avg_launch_us_isolated": 1000.0 * float(kms[kid == 4].astype(np.float64).mean()),
+                     "base_pass_us": 1000.0 * bbase_ms,
+                     "customization_algorithmic_bytes": B, "customize_ms": bms,
+                     "customization_GB_s": B / (bms / 1000.0) / 1e9},
         "injected_cycle_metric": bad if tri is not None else None, "flagged_metrics": flagged,
         "parity_vs_oracle": parity, "tuning": btune, "host": host,
     }
@@ -355,6 +431,9 @@ def main():
     ap.add_argument("--single", type=int, default=500, help="single-query launches for latency")
     ap.add_argument("--metrics", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=5)
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="also run one device copy of this many MiB (scales HBM byte counters)")
+    ap.add_argument("--no-graph", action="store_true", help="skip the HIP graph replay of the step")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")

@@ -193,10 +193,17 @@ int nwcch_update_changed(nwcch_dev* h, int32_t* out, int64_t cap, void* stream);
 int nwcch_status(nwcch_dev* h, uint32_t* out, int clear, void* stream);
 
 /* Tunables of the level kernels (no counterpart in the reference; set by the
- * autotuner nwcch.tune): "tri_per_lane" (single-metric sweep: an arc gets the
- * smallest power-of-two team of lanes, up to 64, with at most this many lower
- * triangles per lane; default 4) and "batch_tri_per_lane" (batched sweep:
- * wavefronts per arc, up to 16, likewise; default 8).  Values 1..256. */
+ * autotuner nwcch.tune):
+ *   "flat_min_tri"  single-metric sweep: a level whose heaviest arc has at least
+ *                   this many lower triangles runs triangle-parallel (one lane
+ *                   per triangle); 0 = every level (default), 2^30 = none.
+ *   "flat_xcd"      triangle-parallel levels: 1 = each XCD takes one contiguous
+ *                   slice of the level, 0 = workgroups interleaved (default).
+ *   "tri_per_lane"  the other levels: an arc gets the smallest power-of-two
+ *                   team of lanes, up to 64, with at most this many lower
+ *                   triangles per lane; default 4, values 1..256.
+ *   "batch_tri_per_lane"  batched sweep: wavefronts per arc, up to 16, likewise;
+ *                   default 8, values 1..256. */
 int nwcch_set_tuning(nwcch_dev* h, const char* key, int32_t value);
 int nwcch_get_tuning(const nwcch_dev* h, const char* key, int32_t* value);
 

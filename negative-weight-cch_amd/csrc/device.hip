@@ -5,10 +5,12 @@
 //   ud[mp]  int2 {up, dn}: the customized shortcut weights of one metric, pair per
 //           arc, so one 8-byte gather fetches both directions of an arc.
 //   tri_ab[T] int2 {a, b}: for each lower triangle (x; y, z) of arc c=(y,z) the arcs
-//           a=(x,y), b=(x,z); stored grouped by c in schedule order (level of the
-//           tail of c, then lower-triangle count descending) so a level is one
-//           contiguous slice streamed front to back.
-//   tri_off[mp+1] int64: triangle slice of each schedule position.
+//           a=(x,y), b=(x,z); grouped by c in schedule order (level = ET height
+//           of the tail of c, then lower-triangle count descending), so a level
+//           is one contiguous slice streamed front to back.  tri_c[T]: the c of
+//           each triangle.
+//   prec[mp] PosRec {tri_begin, arc, tri_count}: the schedule, each record
+//           pointing at its slice.
 //   ae_off/ae_idx: the original arcs of each shortcut, entry (edge << 1) | up.
 //   vinfo[n] {first, deg, parent, depth}, adep[mp] = ET depth of head[a].
 //
@@ -171,6 +173,67 @@ __global__ void __launch_bounds__(256) k_customize_level(LevelLaunch L, const Po
   }
   if (valid && t == 0) ud[c] = make_int2(up, dn);
   if (st) atomicOr(status, st);
+}
+
+// The same level in triangle-parallel form: one lane per lower triangle of the
+// level (a level's triangles are contiguous, grouped by the arc they improve),
+// so a heavy arc is spread over as many lanes as it has triangles and every
+// lane does one dependent gather.  Lanes of one arc are combined by a segmented
+// suffix minimum over the wavefront; the first lane of each segment folds it
+// into the arc with an atomic minimum (the arc holds its base value; min is
+// order independent, so the result is exact and deterministic).  The atomics
+// are performed beyond the XCD's L2 and leave no dirty lines for the end of
+// the launch to write back; with the level's arcs in count order, consecutive
+// segments target scattered arcs rather than one cache line.
+//
+// With xcd_slices, workgroup b takes slice (b % 8) * per + b / 8 of the level
+// (grid = 8 * per): workgroups are dealt to the 8 XCDs in turn, so each XCD
+// works through one contiguous eighth of the level.
+__global__ void __launch_bounds__(256) k_customize_flat(i64 q0, i64 nt, const int2* __restrict__ tri_ab,
+                                                        const int32_t* __restrict__ tri_c, int2* ud,
+                                                        uint32_t* status, int xcd_slices) {
+  const int lane = threadIdx.x & 63;
+  const unsigned per = gridDim.x >> 3;
+  const unsigned slice = xcd_slices ? (blockIdx.x & 7u) * per + (blockIdx.x >> 3) : blockIdx.x;
+  const i64 i = (i64)slice * blockDim.x + threadIdx.x;
+  const bool valid = i < nt;
+  uint32_t st = 0;
+  int32_t c = -1, up = kInf, dn = kInf;
+  if (valid) {
+    const i64 q = q0 + i;
+    c = tri_c[q];
+    const int2 ab = tri_ab[q];
+    const int2 A = ud[ab.x], B = ud[ab.y];
+    up = inf_add(A.y, B.x, st);  // y -> x -> z
+    dn = inf_add(B.y, A.x, st);  // z -> x -> y
+  }
+  // an arc's lanes are contiguous: if lane+o has the same arc, so has every lane between
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int32_t c2 = __shfl_down(c, o, 64);
+    const int32_t u2 = __shfl_down(up, o, 64);
+    const int32_t d2 = __shfl_down(dn, o, 64);
+    if (lane + o < 64 && c2 == c) {
+      up = min(up, u2);
+      dn = min(dn, d2);
+    }
+  }
+  const int32_t cprev = __shfl_up(c, 1, 64);
+  if (valid && (lane == 0 || cprev != c)) {
+    int32_t* v = reinterpret_cast<int32_t*>(ud) + 2 * (i64)c;
+    if (up != kInf) atomicMin(v, up);
+    if (dn != kInf) atomicMin(v + 1, dn);
+  }
+  if (st) atomicOr(status, st);
+}
+
+// tri_c[q] = the arc that triangle q improves (expanded from the schedule records)
+__global__ void __launch_bounds__(256) k_fill_tri_c(const PosRec* __restrict__ prec, i64 mp, int32_t* tri_c) {
+  const i64 stride = (i64)gridDim.x * blockDim.x;
+  for (i64 p = (i64)blockIdx.x * blockDim.x + threadIdx.x; p < mp; p += stride) {
+    const PosRec r = prec[p];
+    for (int32_t j = 0; j < r.tri_count; j++) tri_c[r.tri_begin + j] = r.arc;
+  }
 }
 
 // -------------------------------------------------------- negative-cycle scan
@@ -389,16 +452,24 @@ extern "C" int nwcch_dev_create(const nwcch_topo* topo, int device, nwcch_dev** 
     d.lv = t.lv;
     d.loop_edges = t.loop_edges;
     d.sched_arc = dev_upload(d, t.sched_arc.data(), t.sched_arc.size());
-    d.tri_off = dev_upload(d, t.tri_off.data(), t.tri_off.size());
-    d.tri_ab = dev_upload(d, reinterpret_cast<const int2*>(t.tri_ab.data()), (size_t)t.triangles);
     d.ae_off = dev_upload(d, t.ae_off.data(), t.ae_off.size());
     d.ae_idx = dev_upload(d, t.ae_idx.data(), t.ae_idx.size());
+    d.tri_ab = dev_upload(d, reinterpret_cast<const int2*>(t.tri_ab.data()), (size_t)t.triangles);
     std::vector<PosRec> pr(t.mp);
     for (i64 p = 0; p < t.mp; p++)
       pr[p] = PosRec{t.tri_off[p], t.sched_arc[p], (int32_t)(t.tri_off[p + 1] - t.tri_off[p])};
     d.prec = dev_upload(d, pr.data(), pr.size());
     d.pos_cnt.resize(t.mp);
     for (i64 p = 0; p < t.mp; p++) d.pos_cnt[p] = pr[p].tri_count;
+    d.lv_tri0.resize(d.lv.size() + 1);
+    for (size_t i = 0; i < d.lv.size(); i++) d.lv_tri0[i] = t.tri_off[d.lv[i].begin];
+    d.lv_tri0[d.lv.size()] = t.triangles;
+    d.tri_c = dev_alloc<int32_t>(d, (size_t)std::max<i64>(t.triangles, 1));
+    if (t.mp > 0) {
+      const i64 blocks = std::min<i64>((t.mp + 255) / 256, 16384);
+      hipLaunchKernelGGL(k_fill_tri_c, dim3((unsigned)blocks), dim3(256), 0, 0, d.prec, t.mp, d.tri_c);
+      HIP_OK(hipGetLastError());
+    }
     std::vector<VInfo> vi(t.n);
     for (i64 x = 0; x < t.n; x++)
       vi[x] = VInfo{(int32_t)t.first[x], (int32_t)(t.first[x + 1] - t.first[x]), t.parent[x], t.vdepth[x]};
@@ -470,8 +541,16 @@ extern "C" int nwcch_customize(nwcch_dev* h, const int32_t* w, int32_t* ud, void
     for (size_t i = 0; i < d.lv.size(); i++) {
       const LevelRange& r = d.lv[i];
       if (r.end == r.begin || r.triangles == 0) continue;  // base values are final
-      const LevelLaunch& L = launches[i];
       ProfScope ps(d, s, kKCustomize);
+      if (d.pos_cnt[r.begin] >= d.flat_min_tri) {  // positions of a level: heaviest first
+        const i64 q0 = d.lv_tri0[i], nt = d.lv_tri0[i + 1] - q0;
+        const i64 nb = (nt + 255) / 256;
+        const i64 grid = d.flat_xcd ? 8 * ((nb + 7) / 8) : nb;  // XCD slices: 8 equal shares
+        hipLaunchKernelGGL(k_customize_flat, dim3((unsigned)grid), dim3(256), 0, s, q0, nt, d.tri_ab, d.tri_c, u,
+                           d.status, d.flat_xcd);
+        continue;
+      }
+      const LevelLaunch& L = launches[i];
       hipLaunchKernelGGL(k_customize_level, dim3(L.blk[kTeamClasses]), dim3(256), 0, s, L, d.prec, d.tri_ab,
                          u, d.status);
     }
@@ -575,6 +654,16 @@ extern "C" int nwcch_query(nwcch_dev* h, const int32_t* ud, const int32_t* s_ran
 extern "C" int nwcch_set_tuning(nwcch_dev* h, const char* key, int32_t value) {
   if (!h || !key) { set_error("null argument"); return NWCCH_ERR_ARG; }
   const std::string k(key);
+  if (k == "flat_xcd") {
+    if (value < 0 || value > 1) { set_error("flat_xcd is 0 or 1"); return NWCCH_ERR_ARG; }
+    h->d.flat_xcd = value;
+    return NWCCH_OK;
+  }
+  if (k == "flat_min_tri") {
+    if (value < 0 || value > kFlatOff) { set_error("flat_min_tri out of range [0, 2^30]"); return NWCCH_ERR_ARG; }
+    h->d.flat_min_tri = value;
+    return NWCCH_OK;
+  }
   if (value < 1 || value > 256) { set_error("tuning value out of range [1, 256]"); return NWCCH_ERR_ARG; }
   if (k == "tri_per_lane") h->d.tri_per_lane = value;
   else if (k == "batch_tri_per_lane") h->d.batch_tri_per_lane = value;
@@ -587,6 +676,8 @@ extern "C" int nwcch_get_tuning(const nwcch_dev* h, const char* key, int32_t* va
   const std::string k(key);
   if (k == "tri_per_lane") *value = h->d.tri_per_lane;
   else if (k == "batch_tri_per_lane") *value = h->d.batch_tri_per_lane;
+  else if (k == "flat_min_tri") *value = h->d.flat_min_tri;
+  else if (k == "flat_xcd") *value = h->d.flat_xcd;
   else { set_error("unknown tuning key: " + k); return NWCCH_ERR_ARG; }
   return NWCCH_OK;
 }

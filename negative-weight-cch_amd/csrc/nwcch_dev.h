@@ -43,6 +43,8 @@ struct alignas(16) PosRec {
   int32_t tri_count;
 };
 
+constexpr int32_t kFlatOff = 1 << 30;
+
 struct DevTopo {
   int device = 0;
   i64 n = 0, m = 0, mp = 0, T = 0;
@@ -52,15 +54,20 @@ struct DevTopo {
 
   // customization
   int32_t* sched_arc = nullptr;      // [mp]
-  i64* tri_off = nullptr;            // [mp+1] by schedule position
-  int2* tri_ab = nullptr;            // [T]
+  int2* tri_ab = nullptr;            // [T] in schedule order
   int32_t* ae_off = nullptr;         // [mp+1] by arc id
   int32_t* ae_idx = nullptr;         // [#mapped original arcs]
   PosRec* prec = nullptr;            // [mp] by schedule position
   std::vector<int32_t> pos_cnt;      // [mp] host: triangles per schedule position
+  int32_t* tri_c = nullptr;          // [T] arc improved by each triangle (triangle-parallel levels)
+  std::vector<i64> lv_tri0;          // [levels+1] host: first triangle of each level
   // tunables (nwcch_set_tuning) and the launch geometry derived from them
   int32_t tri_per_lane = kTriPerLane;        // single-metric level kernel
   int32_t batch_tri_per_lane = 8;            // batched level kernel
+  int32_t flat_min_tri = 0;                  // levels whose heaviest arc has >= this many
+                                             // triangles run triangle-parallel
+  int32_t flat_xcd = 0;                      // triangle-parallel levels: one contiguous
+                                             // slice of the level per XCD (else interleaved)
   std::vector<LevelLaunch> launches;         // per level, for tri_per_lane
   int32_t launches_for = -1;
   int32_t* wt = nullptr;             // batched weights transposed to [m][K] (scratch)

@@ -49,7 +49,7 @@ struct UpdArgs {
   const int32_t* ae_off;
   const int32_t* ae_idx;
   const int32_t* arc_pos;
-  const i64* tri_off;
+  const PosRec* prec;
   const int2* tri_ab;
   const i64* side_off;
   const int32_t* side_top;
@@ -89,9 +89,9 @@ __device__ void wave_update_arc(const UpdArgs& A, int32_t c, int lane, uint32_t&
     if (e & 1) up = min(up, wk);
     else dn = min(dn, wk);
   }
-  const int32_t p = A.arc_pos[c];
-  const i64 t1 = A.tri_off[p + 1];
-  for (i64 q = A.tri_off[p] + lane; q < t1; q += 64) {
+  const PosRec r = A.prec[A.arc_pos[c]];
+  const i64 t1 = r.tri_begin + r.tri_count;
+  for (i64 q = r.tri_begin + lane; q < t1; q += 64) {
     const int2 ab = A.tri_ab[q];
     const int2 X = A.ud[ab.x];
     const int2 Y = A.ud[ab.y];
@@ -233,7 +233,7 @@ T* up_alloc(DevTopo& d, size_t count) {
 
 UpdArgs make_args(DevTopo& d, int32_t* w, int32_t* ud, int32_t* chg, i64 chg_cap) {
   UpdArgs A;
-  A.ae_off = d.ae_off; A.ae_idx = d.ae_idx; A.arc_pos = d.arc_pos; A.tri_off = d.tri_off;
+  A.ae_off = d.ae_off; A.ae_idx = d.ae_idx; A.arc_pos = d.arc_pos; A.prec = d.prec;
   A.tri_ab = d.tri_ab; A.side_off = d.side_off; A.side_top = d.side_top; A.arc_lvl = d.arc_lvl;
   A.lvl_cap_off = d.lvl_cap_off; A.e_arc32 = d.e_arc32; A.w = w;
   A.ud = reinterpret_cast<int2*>(ud); A.qstamp = d.qstamp; A.qarcs = d.qarcs; A.qcnt = d.qcnt;

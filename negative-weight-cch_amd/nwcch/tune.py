@@ -29,23 +29,44 @@ def _ms(fn, reps: int) -> float:
     return statistics.median(a.elapsed_time(b) for a, b in evs)
 
 
-def tune_customize(dt: DeviceTopology, w, candidates=(1, 2, 4, 8, 16), reps: int = 5) -> dict:
-    """Triangles per lane of the single-metric level kernel."""
+FLAT_OFF = 1 << 30
+
+
+def tune_customize(dt: DeviceTopology, w, candidates=(1, 2, 4, 8, 16),
+                   flat=(FLAT_OFF, 256, 128, 64, 0), reps: int = 5) -> dict:
+    """Single-metric level sweep: triangles per lane of the team kernel, then
+    the heaviest-arc size from which a level runs triangle-parallel instead
+    (FLAT_OFF = never, 0 = every level)."""
     w = dt.weights(w) if not (isinstance(w, torch.Tensor) and w.is_cuda) else w
     ref = None
-    times = {}
-    for r in candidates:
-        dt.set_tuning("tri_per_lane", r)
-        out = torch.empty((dt.mp, 2), dtype=torch.int32, device=dt.dev)
-        times[r] = _ms(lambda: dt.customize(w, out=out), reps)
+    out = torch.empty((dt.mp, 2), dtype=torch.int32, device=dt.dev)
+
+    def run(key, value):
+        nonlocal ref
+        dt.set_tuning(key, value)
+        out.fill_(0)
+        t = _ms(lambda: dt.customize(w, out=out), reps)
         if ref is None:
             ref = out.clone()
         elif not torch.equal(ref, out):
-            raise AssertionError(f"tri_per_lane={r} changed the customization")
+            raise AssertionError(f"{key}={value} changed the customization")
+        return t
+
+    dt.set_tuning("flat_min_tri", FLAT_OFF)
+    times = {r: run("tri_per_lane", r) for r in candidates}
     best = min(times, key=times.get)
     dt.set_tuning("tri_per_lane", best)
+    ftimes = {f: run("flat_min_tri", f) for f in flat}
+    fbest = min(ftimes, key=ftimes.get)
+    dt.set_tuning("flat_min_tri", fbest)
+    xtimes = {}
+    if fbest != FLAT_OFF:
+        xtimes = {x: run("flat_xcd", x) for x in (0, 1)}
+        dt.set_tuning("flat_xcd", min(xtimes, key=xtimes.get))
     dt.check_status()
-    return {"tri_per_lane": best, "ms": times}
+    return {"tri_per_lane": best, "flat_min_tri": fbest, "flat_xcd": dt.get_tuning("flat_xcd"), "ms": times,
+            "flat_ms": {("off" if f == FLAT_OFF else str(f)): v for f, v in ftimes.items()},
+            "xcd_ms": xtimes}
 
 
 def tune_batch(dt: DeviceTopology, W, slabs=(64, 16, 8), tris=(4, 8, 16), reps: int = 3) -> dict:

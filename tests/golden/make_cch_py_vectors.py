@@ -1,0 +1,131 @@
+"""Write golden vectors of the reference's partial update and early-stop witness.
+
+The reference's ``python/cch.py`` (standard library only) is run here, in the
+build container, on small seeded graphs with an explicit order.  It is used
+for the outputs that no closed form or other reference test fixes:
+
+* ``CCH.update_edges`` (python/cch.py:349-402): the number of arcs evaluated
+  and ``last_changed`` (the changed arcs in pop order);
+* ``CCH.customize(g, stop_on_negative_cycle=True)`` (python/cch.py:264-278):
+  the witness arc;
+* ``CCH.customize(g)``: the full witness list (python/cch.py:287).
+
+Only data is written: ``cch_py_vectors.npz`` with the inputs (graph CSR, order,
+weights, update batches) and the reference's outputs.  The graphs come from
+this repository's seeded generators.  Run from the repository root:
+
+    python tests/golden/make_cch_py_vectors.py
+"""
+from __future__ import annotations
+
+import sys
+import types
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parents[2]
+REF = Path("/root/reference/python")
+sys.path.insert(0, str(ROOT / "negative-weight-cch_amd"))
+
+from nwcch import _lib, graphs  # noqa: E402
+
+INF32 = 2 ** 31 - 1
+
+
+def ref_graph(g):
+    """The reference's Graph shape: n, adj[u] = [(v, w), ...], coords."""
+    adj = []
+    for u in range(g.n):
+        s, e = int(g.off[u]), int(g.off[u + 1])
+        adj.append([(int(v), float(w)) for v, w in zip(g.tgt[s:e], g.w[s:e])])
+    return types.SimpleNamespace(n=g.n, adj=adj, coords=list(zip(g.x.tolist(), g.y.tolist())))
+
+
+def set_w(rg, g, k, w):
+    """Write weight w into original arc k of the reference graph (CSR order)."""
+    u = int(np.searchsorted(g.off, k, side="right") - 1)
+    j = k - int(g.off[u])
+    v, _ = rg.adj[u][j]
+    rg.adj[u][j] = (v, float(w))
+
+
+def as_i32(a):
+    a = np.asarray(a, np.float64)
+    out = np.where(np.isinf(a), INF32, a)
+    return out.astype(np.int32)
+
+
+def main():
+    sys.path.insert(0, str(REF))
+    import cch as refcch  # the reference's pure-Python CCH
+
+    out = {}
+    cases = [("grid", graphs.make_config(0))] + [(f"small{i}", g) for i, g in enumerate(graphs_small())]
+    for name, g in cases:
+        rank = _lib.order(g.n, g.off, g.tgt, g.x, g.y, 8 if g.n < 1000 else 64)
+        rg = ref_graph(g)
+        c = refcch.CCH(rg, order=rank.tolist(), log=lambda *_: None)
+        wit = c.customize(rg)
+        assert wit == [], (name, wit)
+        out[f"{name}/off"], out[f"{name}/tgt"] = g.off, g.tgt
+        out[f"{name}/w"], out[f"{name}/rank"] = g.w, rank
+        out[f"{name}/up0"], out[f"{name}/dn0"] = as_i32(c.up), as_i32(c.down)
+        rng = np.random.default_rng(17)
+        w = g.w.astype(np.int64).copy()
+        for u, k in enumerate([1, 3, 10, 40, 100, 10]):
+            k = min(k, g.m)
+            edges = rng.choice(g.m, k, replace=False).astype(np.int64)
+            if u % 2 == 0:
+                new = w[edges] + rng.integers(1, 200, k)       # raise (traffic)
+            else:
+                new = np.maximum(w[edges] - rng.integers(1, 60, k), w[edges] // 2)  # lower
+            # stay conservative: skip arcs whose lowering would close a negative cycle
+            trial = w.copy()
+            trial[edges] = new
+            from oracle import oracle as O
+            if O.has_negative_cycle(g.n, g.off, g.tgt, trial.astype(np.int32)):
+                new = w[edges] + rng.integers(1, 200, k)
+                trial[edges] = new
+            for e, v in zip(edges, new):
+                set_w(rg, g, int(e), int(v))
+            ev = c.update_edges([(int(e), float(v)) for e, v in zip(edges, new)])
+            w = trial
+            out[f"{name}/u{u}/edges"] = edges
+            out[f"{name}/u{u}/new_w"] = new.astype(np.int64)
+            out[f"{name}/u{u}/evaluated"] = np.int64(ev)
+            out[f"{name}/u{u}/last_changed"] = np.asarray(c.last_changed, np.int32)
+            out[f"{name}/u{u}/negative"] = np.int64(bool(c.update_created_negative_cycle()))
+        out[f"{name}/n_updates"] = np.int64(6)
+        out[f"{name}/up_end"], out[f"{name}/dn_end"] = as_i32(c.up), as_i32(c.down)
+        # a negative cycle: early-stop witness and the full witness list
+        tri = graphs.find_directed_triangle(g, np.random.default_rng(3))
+        if tri is not None:
+            wn = graphs.inject_negative_cycle(g.w, tri)
+        else:
+            k = int(g.off[10])
+            v = int(g.tgt[k])
+            back = [j for j in range(g.off[v], g.off[v + 1]) if g.tgt[j] == 10][0]
+            wn = g.w.copy()
+            wn[k], wn[back] = 0, -1
+        rgn = ref_graph(g.with_weights(wn))
+        cn = refcch.CCH(rgn, order=rank.tolist(), log=lambda *_: None)
+        out[f"{name}/neg_w"] = wn
+        out[f"{name}/neg_witnesses"] = np.asarray(cn.customize(rgn), np.int64)
+        cs = refcch.CCH(rgn, order=rank.tolist(), log=lambda *_: None)
+        out[f"{name}/neg_stop_witness"] = np.asarray(cs.customize(rgn, stop_on_negative_cycle=True), np.int64)
+        print(name, g.n, g.m, "mp", c.m, "witnesses", len(out[f"{name}/neg_witnesses"]),
+              "stop", out[f"{name}/neg_stop_witness"].tolist(), flush=True)
+    out["cases"] = np.array([n for n, _ in cases])
+    dst = Path(__file__).resolve().parent / "cch_py_vectors.npz"
+    np.savez_compressed(dst, **out)
+    print("wrote", dst, dst.stat().st_size, "bytes")
+
+
+def graphs_small():
+    return [graphs.ev_grid(12, seed=1), graphs.ev_geometric(200, seed=2), graphs.shifted_random(150, seed=3)]
+
+
+if __name__ == "__main__":
+    sys.path.insert(0, str(ROOT))
+    main()

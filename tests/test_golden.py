@@ -1,0 +1,59 @@
+"""The oracle against golden vectors written by the reference's own python/cch.py
+(tests/golden/make_cch_py_vectors.py): full customization, the partial update's
+evaluated count and changed list (python/cch.py:349-402), and the witnesses of
+customize with and without the early stop (python/cch.py:249-287)."""
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+GOLD = Path(__file__).resolve().parent / "golden" / "cch_py_vectors.npz"
+
+
+def cases():
+    with np.load(GOLD) as d:
+        return [str(c) for c in d["cases"]]
+
+
+@pytest.fixture(scope="module")
+def gold():
+    with np.load(GOLD) as d:
+        return {k: d[k] for k in d.files}
+
+
+@pytest.mark.parametrize("name", cases())
+def test_customization_equals_reference_python(gold, name):
+    off, tgt, w, rank = (gold[f"{name}/{k}"] for k in ("off", "tgt", "w", "rank"))
+    st = O.Structure(len(rank), off, tgt, rank)
+    up, dn, ovf = st.customize(w)
+    assert not ovf
+    assert np.array_equal(up, gold[f"{name}/up0"]) and np.array_equal(dn, gold[f"{name}/dn0"])
+
+
+@pytest.mark.parametrize("name", cases())
+def test_update_counts_equal_reference_python(gold, name):
+    off, tgt, w, rank = (gold[f"{name}/{k}"] for k in ("off", "tgt", "w", "rank"))
+    st = O.Structure(len(rank), off, tgt, rank)
+    up, dn, _ = st.customize(w)
+    upd = O.Updater(st, w, up, dn)
+    for u in range(int(gold[f"{name}/n_updates"])):
+        ev, changed, ovf = upd.update(gold[f"{name}/u{u}/edges"], gold[f"{name}/u{u}/new_w"])
+        assert not ovf
+        assert ev == int(gold[f"{name}/u{u}/evaluated"]), (name, u)
+        assert changed.tolist() == gold[f"{name}/u{u}/last_changed"].tolist(), (name, u)
+        neg = st.negcycle(upd.up[changed], upd.dn[changed])[0] > 0
+        assert neg == bool(gold[f"{name}/u{u}/negative"])
+    assert np.array_equal(upd.up, gold[f"{name}/up_end"]) and np.array_equal(upd.dn, gold[f"{name}/dn_end"])
+
+
+@pytest.mark.parametrize("name", cases())
+def test_witnesses_equal_reference_python(gold, name):
+    off, tgt, rank = (gold[f"{name}/{k}"] for k in ("off", "tgt", "rank"))
+    wn = gold[f"{name}/neg_w"]
+    st = O.Structure(len(rank), off, tgt, rank)
+    up, dn, _ = st.customize(wn)
+    neg = np.nonzero((up != O.INF) & (dn != O.INF) & (up.astype(np.int64) + dn < 0))[0]
+    assert neg.tolist() == gold[f"{name}/neg_witnesses"].tolist()
+    assert [st.customize_stop(wn)] == gold[f"{name}/neg_stop_witness"].tolist()

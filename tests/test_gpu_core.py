@@ -13,9 +13,15 @@ pytestmark = pytest.mark.gpu
 INF = O.INF
 
 
-def _setup(g, rank):
+FLAT_OFF = 1 << 30
+# level sweep forms: team kernel everywhere, triangle-parallel everywhere, mixed
+FLAT = [FLAT_OFF, 0, 64]
+
+
+def _setup(g, rank, flat=FLAT_OFF):
     topo = _lib.Topology.from_graph(g.n, g.off, g.tgt, rank)
     dt = DeviceTopology(topo)
+    dt.set_tuning("flat_min_tri", flat)
     st = O.Structure(g.n, g.off, g.tgt, rank)
     return dt, st
 
@@ -32,10 +38,11 @@ def _check_customize(dt, st, w):
     return ud, up, dn
 
 
-def test_customize_scan_query_c0(c0):
+@pytest.mark.parametrize("flat", FLAT)
+def test_customize_scan_query_c0(c0, flat):
     g, rank = c0
-    print("case: config 0 grid", g.n, g.m, flush=True)
-    dt, st = _setup(g, rank)
+    print("case: config 0 grid", g.n, g.m, "flat_min_tri", flat, flush=True)
+    dt, st = _setup(g, rank, flat)
     ud, up, dn = _check_customize(dt, st, g.w)
     s, t = graphs.random_pairs(g.n, 1000, seed=2)
     dist, sc = dt.query(ud, rank[s], rank[t])
@@ -52,12 +59,13 @@ def test_customize_scan_query_c0(c0):
     assert int(sc.max()) <= st.depth * (st.depth + 1)
 
 
+@pytest.mark.parametrize("flat", [FLAT_OFF, 0])
 @pytest.mark.parametrize("idx", range(12))
-def test_small_family(idx):
+def test_small_family(idx, flat):
     g = small_graphs()[idx]
-    print("case: small family", idx, g.name, flush=True)
+    print("case: small family", idx, g.name, "flat_min_tri", flat, flush=True)
     rank = order_of(g)
-    dt, st = _setup(g, rank)
+    dt, st = _setup(g, rank, flat)
     ud, up, dn = _check_customize(dt, st, g.w)
     qs = np.repeat(np.arange(g.n, dtype=np.int32), 3)
     qt = np.random.default_rng(idx).integers(0, g.n, len(qs)).astype(np.int32)
@@ -73,6 +81,11 @@ def test_road_config1_bit_exact():
     rank = _lib.order(g.n, g.off, g.tgt, g.x, g.y, 64)
     dt, st = _setup(g, rank)
     ud, up, dn = _check_customize(dt, st, g.w)
+    for flat in (0, 128):
+        print("case: config 1 road graph, flat_min_tri", flat, flush=True)
+        dt.set_tuning("flat_min_tri", flat)
+        _check_customize(dt, st, g.w)
+    dt.set_tuning("flat_min_tri", FLAT_OFF)
     s, t = graphs.random_pairs(g.n, 2000, seed=11)
     dist, sc = dt.query(ud, rank[s], rank[t])
     d0, s0, _ = st.query(up, dn, s, t, threads=8)
@@ -85,7 +98,8 @@ def test_road_config1_bit_exact():
         assert dist[i].item() == dj(s[i], t[i])
 
 
-def test_injected_negative_cycle_is_flagged(c0):
+@pytest.mark.parametrize("flat", [FLAT_OFF, 0])
+def test_injected_negative_cycle_is_flagged(c0, flat):
     """A 3-arc loop of total -1 (the batched config's injected cycle) is found by
     the scan, with the oracle's count and first witness."""
     g, rank = c0
@@ -98,7 +112,7 @@ def test_injected_negative_cycle_is_flagged(c0):
         w[k], w[back] = 0, -1
     else:
         w = graphs.inject_negative_cycle(g.w, tri)
-    dt, st = _setup(g, rank)
+    dt, st = _setup(g, rank, flat)
     ud, up, dn = _check_customize(dt, st, w)
     cnt = int(dt.negcycle_scan(ud)[0])
     assert cnt > 0 and O.has_negative_cycle(g.n, g.off, g.tgt, w)
@@ -119,11 +133,13 @@ def test_unreachable_pairs_and_inf():
     assert dist.cpu().tolist() == [1, INF, -7, 2, INF]
 
 
-def test_overflow_is_reported_not_saturated():
+@pytest.mark.parametrize("flat", [FLAT_OFF, 0])
+def test_overflow_is_reported_not_saturated(flat):
     g = graphs.from_arcs(3, [(0, 1, 2 ** 30 + 5), (1, 2, 2 ** 30 + 5)])
     rank = np.array([1, 0, 2], np.int32)
     topo = _lib.Topology.from_graph(g.n, g.off, g.tgt, rank)
     dt = DeviceTopology(topo)
+    dt.set_tuning("flat_min_tri", flat)
     dt.customize(g.w)
     assert dt.status() & 1
     st = O.Structure(g.n, g.off, g.tgt, rank)

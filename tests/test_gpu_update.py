@@ -133,3 +133,38 @@ def test_stop_on_negative_cycle_witness(c0):
     early = c.customize(g.with_weights(w), stop_on_negative_cycle=True)
     wit = st.customize_stop(w)   # the reference's elimination loop with early checks
     assert early == ([wit] if wit >= 0 else neg.tolist())
+
+
+def _golden():
+    from pathlib import Path
+    p = Path(__file__).resolve().parent / "golden" / "cch_py_vectors.npz"
+    with np.load(p) as d:
+        return {k: d[k] for k in d.files}
+
+
+@pytest.mark.parametrize("name", ["grid", "small0", "small1", "small2"])
+def test_dropin_matches_reference_python_vectors(name):
+    """The CCH drop-in on the GPU against vectors the reference's python/cch.py
+    wrote: customized values, every update's evaluated count and changed list,
+    the final values, and both witness forms of a negative cycle."""
+    G = _golden()
+    print(f"case: golden vectors {name}", flush=True)
+    off, tgt, w, rank = (G[f"{name}/{k}"] for k in ("off", "tgt", "w", "rank"))
+    n = len(rank)
+    g = graphs.RoadGraph(n, off, tgt, w.astype(np.int32), np.zeros(n), np.zeros(n))
+    c = CCH(g, order=rank, log=lambda *_: None)
+    assert c.customize(g) == []
+    ud = c.ud.cpu().numpy()
+    assert np.array_equal(ud[:, 0], G[f"{name}/up0"]) and np.array_equal(ud[:, 1], G[f"{name}/dn0"])
+    for u in range(int(G[f"{name}/n_updates"])):
+        edges, new = G[f"{name}/u{u}/edges"], G[f"{name}/u{u}/new_w"]
+        ev = c.update_edges(list(zip(edges.tolist(), new.tolist())))
+        assert ev == int(G[f"{name}/u{u}/evaluated"]), (u, ev)
+        assert c.last_changed == G[f"{name}/u{u}/last_changed"].tolist(), u
+        assert c.update_created_negative_cycle() == bool(G[f"{name}/u{u}/negative"])
+    ud = c.ud.cpu().numpy()
+    assert np.array_equal(ud[:, 0], G[f"{name}/up_end"]) and np.array_equal(ud[:, 1], G[f"{name}/dn_end"])
+    gn = g.with_weights(G[f"{name}/neg_w"])
+    c2 = CCH(gn, order=rank, log=lambda *_: None)
+    assert c2.customize(gn) == G[f"{name}/neg_witnesses"].tolist()
+    assert c2.customize(gn, stop_on_negative_cycle=True) == G[f"{name}/neg_stop_witness"].tolist()
