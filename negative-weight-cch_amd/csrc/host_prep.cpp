@@ -87,9 +87,16 @@ struct CellFlow {
   // Only "active" vertices take part: interior vertices of the source or sink
   // set (all of whose cell neighbours share their role) cannot lie on a useful
   // augmenting path; the caller assigns them to their side directly.
+  //
+  // Local vertex l is network node pair 2*nid[l], 2*nid[l]+1, with nid a
+  // space-filling-curve numbering of the cell: the searches below then touch
+  // nearby memory for nearby vertices.  Numbering changes no adjacency order,
+  // so the flow, and the residual reachability that defines the cut, are the
+  // same as with any other numbering.
   void build(const std::vector<i32>& cell, const Member& loc,
              const std::vector<i64>& noff, const std::vector<i32>& nbr,
-             const std::vector<int8_t>& role, const std::vector<char>& active) {
+             const std::vector<int8_t>& role, const std::vector<char>& active,
+             const std::vector<int>& nid) {
     s = (int)cell.size();
     S = 2 * s;
     T = 2 * s + 1;
@@ -99,13 +106,14 @@ struct CellFlow {
     for (int l = 0; l < s; l++) {
       if (!active[l]) continue;
       i32 v = cell[l];
-      add_deg(2 * l, 2 * l + 1);
+      const int nl = nid[l];
+      add_deg(2 * nl, 2 * nl + 1);
       for (i64 j = noff[v]; j < noff[v + 1]; j++) {
         i32 u = nbr[j];
-        if (loc[u] >= 0 && active[loc[u]]) add_deg(2 * l + 1, 2 * loc[u]);
+        if (loc[u] >= 0 && active[loc[u]]) add_deg(2 * nl + 1, 2 * nid[loc[u]]);
       }
-      if (role[l] == 1) add_deg(S, 2 * l);
-      if (role[l] == 2) add_deg(2 * l + 1, T);
+      if (role[l] == 1) add_deg(S, 2 * nl);
+      if (role[l] == 2) add_deg(2 * nl + 1, T);
     }
     for (int i = 0; i < N; i++) deg[i + 1] += deg[i];
     off = deg;
@@ -121,83 +129,80 @@ struct CellFlow {
     for (int l = 0; l < s; l++) {
       if (!active[l]) continue;
       i32 v = cell[l];
+      const int nl = nid[l];
       // sources and sinks cannot be cut (infinite vertex capacity)
-      add(2 * l, 2 * l + 1, role[l] ? kInfCap : 1);
+      add(2 * nl, 2 * nl + 1, role[l] ? kInfCap : 1);
       for (i64 j = noff[v]; j < noff[v + 1]; j++) {
         i32 u = nbr[j];
-        if (loc[u] >= 0 && active[loc[u]]) add(2 * l + 1, 2 * loc[u], kInfCap);
+        if (loc[u] >= 0 && active[loc[u]]) add(2 * nl + 1, 2 * nid[loc[u]], kInfCap);
       }
-      if (role[l] == 1) add(S, 2 * l, kInfCap);
-      if (role[l] == 2) add(2 * l + 1, T, kInfCap);
+      if (role[l] == 1) add(S, 2 * nl, kInfCap);
+      if (role[l] == 2) add(2 * nl + 1, T, kInfCap);
     }
     level.assign(N, -1);
     it.assign(N, 0);
     queue.assign(N, 0);
   }
 
-  bool bfs() {
-    std::fill(level.begin(), level.end(), -1);
-    int qh = 0, qt = 0;
-    queue[qt++] = S;
-    level[S] = 0;
-    while (qh < qt) {
-      int x = queue[qh++];
-      // nodes at or beyond T's level cannot be on a shortest augmenting path
-      if (level[T] >= 0 && level[x] >= level[T]) break;
-      for (int e = off[x]; e < off[x + 1]; e++)
-        if (cap[e] > 0 && level[to[e]] < 0) {
-          level[to[e]] = level[x] + 1;
-          queue[qt++] = to[e];
-        }
-    }
-    return level[T] >= 0;
-  }
-
-  // One augmenting path of one unit (every S-T path crosses a unit vertex arc
-  // unless a source touches a sink, which the caller excludes).
-  int augment() {
+  // Maximum flow in rounds of one breadth-first search each: the search tree
+  // from S is traced back from every node with a residual arc into T, and each
+  // traced path that shares no node with a path already taken this round is
+  // augmented.  On these networks one search yields many disjoint paths (a
+  // blocking-flow phase restricted to shortest paths yields about one, since
+  // path lengths across a clustered point set all differ).  Any maximum flow
+  // gives the same residual reachability from S, hence the same cut.
+  long maxflow_rounds(long limit) {
+    std::vector<int>& seen = level;  // round stamp of the search
+    std::vector<int>& par = it;      // tree arc into each reached node
+    std::vector<int> used(N, 0);     // round stamp of nodes on augmented paths
+    std::fill(seen.begin(), seen.end(), 0);
     std::vector<int>& path = stack_;
-    path.clear();
-    int x = S;
-    while (true) {
-      if (x == T) {
-        int f = kInfCap;
-        for (int e : path) f = std::min(f, cap[e]);
-        for (int e : path) { cap[e] -= f; cap[rev[e]] += f; }
-        return f;
-      }
-      bool moved = false;
-      for (int& e = it[x]; e < off[x + 1]; e++) {
-        int y = to[e];
-        if (cap[e] > 0 && level[y] == level[x] + 1) {
-          path.push_back(e);
-          x = y;
-          moved = true;
-          break;
+    long flow = 0;
+    for (int round = 1; flow < limit; round++) {
+      int qh = 0, qt = 0;
+      queue[qt++] = S;
+      seen[S] = round;
+      while (qh < qt) {
+        const int x = queue[qh++];
+        for (int e = off[x]; e < off[x + 1]; e++) {
+          const int y = to[e];
+          if (cap[e] > 0 && seen[y] != round) {
+            seen[y] = round;
+            par[y] = e;
+            if (y != T) queue[qt++] = y;
+          }
         }
       }
-      if (!moved) {
-        if (path.empty()) return 0;
-        level[x] = -1;  // dead end
-        int e = path.back();
-        path.pop_back();
-        x = to[rev[e]];
-        it[x]++;
+      if (seen[T] != round) break;
+      long got = 0;
+      for (int e = off[T]; e < off[T + 1] && flow < limit; e++) {
+        const int u = to[e];  // arc u -> T is rev[e]
+        if (seen[u] != round || used[u] == round || cap[rev[e]] <= 0) continue;
+        path.clear();
+        path.push_back(rev[e]);
+        bool ok = true;
+        for (int y = u; y != S; y = to[rev[par[y]]]) {
+          if (used[y] == round) { ok = false; break; }
+          path.push_back(par[y]);
+        }
+        if (!ok) continue;
+        int f = kInfCap;
+        for (int a : path) f = std::min(f, cap[a]);
+        for (int a : path) {
+          cap[a] -= f;
+          cap[rev[a]] += f;
+          used[to[a]] = round;
+        }
+        used[T] = 0;
+        flow += f;
+        got += f;
       }
-    }
-  }
-
-  long maxflow(long limit) {
-    long flow = 0;
-    while (flow < limit && bfs()) {
-      for (int i = 0; i < N; i++) it[i] = off[i];
-      int f;
-      while (flow < limit && (f = augment()) > 0) flow += f;
+      if (got == 0) break;  // cannot happen while T is reachable; guards the loop
     }
     return flow;
   }
 
-  // residual reachability from S (after maxflow)
+  // residual reachability from S (after the maximum flow)
   void reach(std::vector<char>& seen) {
     seen.assign(N, 0);
     int qh = 0, qt = 0;
@@ -275,7 +280,7 @@ Split geometric_split(const std::vector<i32>& cell, const Member& loc,
 // touches a sink (no finite vertex cut exists).
 Split direction_cut(int d, const std::vector<i32>& cell, const Member& loc,
                     const std::vector<i64>& noff, const std::vector<i32>& nbr,
-                    const double* x, const double* y) {
+                    const double* x, const double* y, const std::vector<int>& nid) {
   Split cur;
   const int s = (int)cell.size();
   const int k = std::max(1, (int)(s * 0.25));
@@ -313,18 +318,48 @@ Split direction_cut(int d, const std::vector<i32>& cell, const Member& loc,
     if (inner) active[l] = 0;
   }
   CellFlow net;
-  net.build(cell, loc, noff, nbr, role, active);
-  net.maxflow((long)s + 1);
+  net.build(cell, loc, noff, nbr, role, active, nid);
+  net.maxflow_rounds((long)s + 1);
   net.reach(seen);
   cur.ok = true;
   for (int l = 0; l < s; l++) {
-    bool in_r = seen[2 * l], out_r = seen[2 * l + 1];
+    bool in_r = seen[2 * nid[l]], out_r = seen[2 * nid[l] + 1];
     if (!active[l]) in_r = out_r = role[l] == 1;
     if (out_r) cur.A.push_back(cell[l]);
     else if (in_r) cur.sep.push_back(cell[l]);
     else cur.B.push_back(cell[l]);
   }
   return cur;
+}
+
+// Morton (Z-order) numbering of a cell's vertices: coordinates scaled to 16
+// bits over the cell's bounding box, bits interleaved; nid[l] = position of
+// local vertex l in that order.
+std::vector<int> curve_numbering(const std::vector<i32>& cell, const double* x, const double* y) {
+  const int s = (int)cell.size();
+  double x0 = 1e300, x1 = -1e300, y0 = 1e300, y1 = -1e300;
+  for (i32 v : cell) {
+    x0 = std::min(x0, x[v]); x1 = std::max(x1, x[v]);
+    y0 = std::min(y0, y[v]); y1 = std::max(y1, y[v]);
+  }
+  const double sx = x1 > x0 ? 65535.0 / (x1 - x0) : 0.0, sy = y1 > y0 ? 65535.0 / (y1 - y0) : 0.0;
+  auto spread = [](uint64_t v) {
+    v &= 0xffff;
+    v = (v | (v << 8)) & 0x00ff00ffull;
+    v = (v | (v << 4)) & 0x0f0f0f0full;
+    v = (v | (v << 2)) & 0x33333333ull;
+    v = (v | (v << 1)) & 0x55555555ull;
+    return v;
+  };
+  std::vector<uint64_t> key(s);
+  for (int l = 0; l < s; l++) {
+    const uint64_t qx = (uint64_t)((x[cell[l]] - x0) * sx), qy = (uint64_t)((y[cell[l]] - y0) * sy);
+    key[l] = ((spread(qx) | (spread(qy) << 1)) << 32) | (uint64_t)l;
+  }
+  std::sort(key.begin(), key.end());
+  std::vector<int> nid(s);
+  for (int i = 0; i < s; i++) nid[(int)(key[i] & 0xffffffffull)] = i;
+  return nid;
 }
 
 // Best of the 4 directions: smallest separator, then most balanced, then the
@@ -334,13 +369,14 @@ Split inertial_split(const std::vector<i32>& cell, const Member& loc,
                      const std::vector<i64>& noff, const std::vector<i32>& nbr,
                      const double* x, const double* y, bool parallel) {
   Split cut[4];
+  const std::vector<int> nid = curve_numbering(cell, x, y);
   if (parallel) {
     std::vector<std::thread> th;
     for (int d = 0; d < 4; d++)
-      th.emplace_back([&, d] { cut[d] = direction_cut(d, cell, loc, noff, nbr, x, y); });
+      th.emplace_back([&, d] { cut[d] = direction_cut(d, cell, loc, noff, nbr, x, y, nid); });
     for (auto& t : th) t.join();
   } else {
-    for (int d = 0; d < 4; d++) cut[d] = direction_cut(d, cell, loc, noff, nbr, x, y);
+    for (int d = 0; d < 4; d++) cut[d] = direction_cut(d, cell, loc, noff, nbr, x, y, nid);
   }
   int best = -1;
   auto bal = [](const Split& q) { return std::min(q.A.size(), q.B.size()); };
