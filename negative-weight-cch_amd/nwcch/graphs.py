@@ -202,14 +202,19 @@ def road_graph(n_points, seed, box=60000.0, clusters=200, sigma=2000.0, out_degr
         p = p[(p >= 0).all(1) & (p <= box).all(1)]
         pts = np.concatenate([pts, p])
         need = n_points - pts.shape[0]
-    pts = pts[:n_points]
-    pts, uniq = np.unique(np.round(pts, 3), axis=0, return_index=True)
+    pts = np.round(pts[:n_points], 3)
+    # distinct points in lexicographic (x, y) order, through integer keys
+    # (millimetre coordinates, at most 2^26 in the box) rather than row sorting
+    kxy = (np.rint(pts[:, 0] * 1000).astype(np.int64) << 32) | np.rint(pts[:, 1] * 1000).astype(np.int64)
+    _, first = np.unique(kxy, return_index=True)
+    pts = pts[first]
     pts = pts[rng.permutation(pts.shape[0])]
     n0 = pts.shape[0]
     tri = Delaunay(pts).simplices
     e = np.concatenate([tri[:, [0, 1]], tri[:, [1, 2]], tri[:, [2, 0]]])
     e.sort(axis=1)
-    e = np.unique(e, axis=0)
+    ke = np.unique(e[:, 0].astype(np.int64) * n0 + e[:, 1])   # distinct edges, lexicographic
+    e = np.stack([ke // n0, ke % n0], 1).astype(tri.dtype)
     ln = np.hypot(*(pts[e[:, 0]] - pts[e[:, 1]]).T)
     mst = minimum_spanning_tree(coo_matrix((ln + 1e-9, (e[:, 0], e[:, 1])), shape=(n0, n0))).tocoo()
     mst_e = np.stack([np.minimum(mst.row, mst.col), np.maximum(mst.row, mst.col)], 1)
