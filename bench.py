@@ -164,7 +164,10 @@ def plain(args, out):
     # (events around every launch would add their own gaps)
     lv_avg_us = 1000.0 * (cmed - base_ms) / n_lv
     achieved = (B_lv / n_lv) / (lv_avg_us * 1e-6) / 1e9
-    traffic = read_traffic("k_customize_level", B_lv / n_lv)
+    flat = dt.get_tuning("flat_min_tri")
+    kernel = "k_customize_flat" if flat == 0 else ("k_customize_level" if flat >= 1 << 30 else
+                                                   "k_customize_flat+k_customize_level")
+    traffic = read_traffic(kernel, B_lv / n_lv)
     if args.copy_mib:
         # a plain device copy of known size, for scaling the HBM byte counters
         src = torch.ones(args.copy_mib << 18, dtype=torch.int32, device=dt.dev)
@@ -229,7 +232,7 @@ def plain(args, out):
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic["bytes_per_launch"] if traffic else None,
                      "traffic_vs_algorithmic": traffic["vs_algorithmic"] if traffic else None,
-                     "traffic_source": traffic["source"] if traffic else None, "kernel": "k_customize_level",
+                     "traffic_source": traffic["source"] if traffic else None, "kernel": kernel,
                      "algorithmic_bytes_per_launch": B_lv / n_lv, "avg_launch_us": lv_avg_us,
                      "avg_launch_us_isolated": 1000.0 * float(kms[kid == 0].astype(np.float64).mean()),
                      "base_pass_us": 1000.0 * base_ms,

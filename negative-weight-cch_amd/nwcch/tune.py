@@ -33,10 +33,11 @@ FLAT_OFF = 1 << 30
 
 
 def tune_customize(dt: DeviceTopology, w, candidates=(1, 2, 4, 8, 16),
-                   flat=(FLAT_OFF, 256, 128, 64, 0), reps: int = 5) -> dict:
+                   flat=(0, 64, 128, 256, FLAT_OFF), reps: int = 5, margin: float = 0.02) -> dict:
     """Single-metric level sweep: triangles per lane of the team kernel, then
     the heaviest-arc size from which a level runs triangle-parallel instead
-    (FLAT_OFF = never, 0 = every level)."""
+    (FLAT_OFF = never, 0 = every level).  A later candidate replaces an
+    earlier one only if it is faster by more than `margin` (timing noise)."""
     w = dt.weights(w) if not (isinstance(w, torch.Tensor) and w.is_cuda) else w
     ref = None
     out = torch.empty((dt.mp, 2), dtype=torch.int32, device=dt.dev)
@@ -57,7 +58,10 @@ def tune_customize(dt: DeviceTopology, w, candidates=(1, 2, 4, 8, 16),
     best = min(times, key=times.get)
     dt.set_tuning("tri_per_lane", best)
     ftimes = {f: run("flat_min_tri", f) for f in flat}
-    fbest = min(ftimes, key=ftimes.get)
+    fbest = flat[0]
+    for f in flat[1:]:
+        if ftimes[f] < ftimes[fbest] * (1.0 - margin):
+            fbest = f
     dt.set_tuning("flat_min_tri", fbest)
     xtimes = {}
     if fbest != FLAT_OFF:
