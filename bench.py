@@ -424,6 +424,140 @@ def serving(args, out):
     del srv, dt
 
 
+def full_scale(args, out):
+    """Config 4: the 23.9M-point road graph (seed 4, continental fractal heights
+    0-3,000 m).  One metric and K = 8 metrics customized on the GPU, the
+    negative-cycle scans, 1M random ET queries; every shortcut value checked
+    against the reference's own C++ customization (cpp/cch.cpp, compiled from
+    its sources) and a query sample against Johnson + Dijkstra; the reference's
+    serial and OpenMP customization timed on the host beside it."""
+    import concurrent.futures as cf
+
+    import numpy as np
+    import torch
+
+    from nwcch import graphs
+    from nwcch.device import DeviceTopology
+
+    g, rank, topo, host = prepare(4)
+    dt = DeviceTopology(topo, 0)
+    log(f"device topology uploaded ({torch.cuda.memory_allocated() / 1e9:.1f} GB)")
+    w = dt.weights(g.w)
+    ud = torch.empty((topo.mp, 2), dtype=torch.int32, device=dt.dev)
+    scan = torch.empty(2, dtype=torch.int64, device=dt.dev)
+    dt.customize(w, out=ud)
+    dt.check_status()
+    cust_ms = event_ms(lambda: dt.customize(w, out=ud), args.steps)
+    scan_ms = event_ms(lambda: dt.negcycle_scan(ud, out=scan), args.steps)
+    n_lv = int(np.count_nonzero(np.asarray(level_triangles(topo)) > 0))
+    dt.profile_begin(n_lv + 2)
+    dt.customize(w, out=ud)
+    kid, kms = dt.profile_end()
+    base_ms = float(np.median(kms[kid == 5]))
+    cmed = statistics.median(cust_ms)
+    B_lv = level_bytes(topo.mp, topo.triangles)
+    lv_avg_us = 1000.0 * (cmed - base_ms) / n_lv
+    achieved = (B_lv / n_lv) / (lv_avg_us * 1e-6) / 1e9
+    log(f"config 4 customize {cmed:.1f} ms ({topo.triangles / (cmed / 1e3):.3g} tri/s), level launches "
+        f"{n_lv}, {achieved:.0f} GB/s algorithmic; scan {statistics.median(scan_ms):.2f} ms")
+    # K = 8 metrics in one sweep
+    K = 8
+    W, _ = graphs.metric_family(g, K, seed=4, hmax=3000)
+    Wd = dt.weights_batch(W)
+    del W
+    UD = torch.empty(dt.batch_shape(K, 0), dtype=torch.int32, device=dt.dev)
+    sc = torch.empty((2, K), dtype=torch.int64, device=dt.dev)
+    dt.customize_batch(Wd, out=UD)
+    dt.check_status()
+    bms = statistics.median(event_ms(lambda: dt.customize_batch(Wd, out=UD), max(2, args.steps // 2)))
+    dt.negcycle_scan_batch(UD, K, out=sc)
+    flagged = [int(k) for k in np.nonzero(sc[0].cpu().numpy() > 0)[0]]
+    log(f"config 4 K={K}: {bms:.1f} ms ({K * topo.triangles / (bms / 1e3):.3g} tri/s), flagged {flagged}")
+    del UD, Wd
+    # 1M random ET queries on metric 0
+    nq = args.full_queries
+    qs_h, qt_h = graphs.random_pairs(g.n, nq, seed=41)
+    qs = torch.as_tensor(rank[qs_h]).to(dt.dev)
+    qt = torch.as_tensor(rank[qt_h]).to(dt.dev)
+    dist = torch.empty(nq, dtype=torch.int32, device=dt.dev)
+    order = dt.chain_order(qs, qt)
+    dt.query(ud, qs, qt, want_scanned=False, dist=dist, order=order)
+    qms = statistics.median(event_ms(lambda: dt.query(ud, qs, qt, want_scanned=False, dist=dist, order=order), 3))
+    single = [event_ms(lambda i=i: dt.query(ud, qs[i:i + 1], qt[i:i + 1], want_scanned=False), 1)[0]
+              for i in range(50)]
+    dt.check_status()
+    log(f"config 4 queries: {nq} in {qms:.1f} ms ({nq / (qms / 1e3):.3g} q/s); single p50 "
+        f"{np.percentile(single, 50):.2f} ms")
+    res = {
+        "workload": f"config 4: n={g.n} m={g.m} m+={topo.mp} T={topo.triangles} ET depth={topo.depth}",
+        "n": g.n, "m": g.m, "m_plus": topo.mp, "triangles": topo.triangles, "depth": topo.depth,
+        "levels": topo.levels, "negative_arc_fraction": round(g.negative_fraction(), 4), "host": host,
+        "customize_ms": cmed, "triangles_per_s": topo.triangles / (cmed / 1e3),
+        "roofline": {"kernel": "level sweep", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "algorithmic_bytes_per_launch": B_lv / n_lv,
+                     "avg_launch_us": lv_avg_us, "level_launches": n_lv},
+        "scan_ms": statistics.median(scan_ms),
+        "batched": {"metrics": K, "ms": bms, "triangles_per_s": K * topo.triangles / (bms / 1e3),
+                    "flagged_metrics": flagged},
+        "queries": {"count": nq, "ms": qms, "queries_per_s": nq / (qms / 1e3),
+                    "single_p50_ms": float(np.percentile(single, 50)),
+                    "single_p99_ms": float(np.percentile(single, 99))},
+    }
+    if not args.no_check or not args.no_cpu:
+        from oracle import oracle as O
+        got = ud.cpu().numpy()
+        d_gpu = dist[:args.full_check].cpu().numpy()
+        del ud
+        torch.cuda.empty_cache()
+        if O.have_ref():
+            t0 = time.perf_counter()
+            ref = O.Reference(g.n, g.off, g.tgt, rank)
+            t_build = time.perf_counter() - t0
+            t0 = time.perf_counter()
+            up, dn = ref.customize(g.w)
+            t_ser = time.perf_counter() - t0
+            ok = True
+            for col, r in ((0, up), (1, dn)):
+                r = np.where(np.isinf(r), O.INF, r).astype(np.int64)
+                ok &= bool(np.array_equal(got[:, col].astype(np.int64), r))
+            del up, dn
+            thr = os.cpu_count() or 1
+            t0 = time.perf_counter()
+            ref.customize(g.w, threads=thr)
+            t_par = time.perf_counter() - t0
+            res["parity_vs_reference_customization"] = ok
+            res["cpu_baseline"] = [
+                {"value": topo.triangles / t_ser, "unit": "triangles/s", "cores": 1, "kind": "reference",
+                 "sample": "cpp/cch.cpp CCH::customize (serial, float64), one full customization", "ms": 1e3 * t_ser},
+                {"value": topo.triangles / t_par, "unit": "triangles/s", "cores": thr, "kind": "reference",
+                 "sample": "cpp/cch.cpp CCH::customize_parallel (OpenMP, level buckets), one full customization",
+                 "ms": 1e3 * t_par}]
+            res["reference_build_s"] = t_build
+            log(f"config 4 reference: build {t_build:.1f}s, serial {t_ser:.1f}s, {thr} threads {t_par:.1f}s; "
+                f"parity {ok}")
+            del ref
+        if not args.no_check:
+            t0 = time.perf_counter()
+            p, neg = O.johnson(g.n, g.off, g.tgt, g.w)
+            t_j = time.perf_counter() - t0
+            nchk = args.full_check
+            workers = min(8, os.cpu_count() or 1)
+            djs = [O.Dijkstra(g.n, g.off, g.tgt, g.w, p) for _ in range(workers)]
+
+            def run(i):
+                return djs[i % workers](int(qs_h[i]), int(qt_h[i]))
+            t0 = time.perf_counter()
+            with cf.ThreadPoolExecutor(workers) as ex:
+                want = np.array(list(ex.map(run, range(nchk), chunksize=1)), np.int64)
+            t_d = time.perf_counter() - t0
+            want = np.where(want >= O.INF, O.INF, want)
+            okq = bool(np.array_equal(d_gpu.astype(np.int64), want))
+            res["query_check"] = {"pairs": nchk, "agree_with_johnson_dijkstra": okq, "johnson_s": t_j,
+                                  "dijkstra_s": t_d, "negative_cycle": neg}
+            log(f"config 4 queries vs Johnson+Dijkstra on {nchk}: {okq} (Johnson {t_j:.1f}s, Dijkstra {t_d:.1f}s)")
+    out["full_scale"] = res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -437,6 +571,10 @@ def main():
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="also run one device copy of this many MiB (scales HBM byte counters)")
     ap.add_argument("--no-graph", action="store_true", help="skip the HIP graph replay of the step")
+    ap.add_argument("--full-scale", action="store_true",
+                    help="run the 23.9M-point road graph (config 4) instead of the default workloads")
+    ap.add_argument("--full-queries", type=int, default=1_000_000)
+    ap.add_argument("--full-check", type=int, default=100, help="queries checked against Johnson + Dijkstra")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
@@ -456,6 +594,10 @@ def main():
         "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "int32", "data": "synthetic",
     }
+    if args.full_scale:
+        full_scale(args, out)
+        print(json.dumps(out), flush=True)
+        return
     plain(args, out)
     if not args.no_batched:
         batched(args, out)

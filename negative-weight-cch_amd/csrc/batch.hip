@@ -65,32 +65,32 @@ __global__ void __launch_bounds__(256) k_transpose_w(const int32_t* __restrict__
 
 // Base values of every (arc, metric) in the slab layout: UD[slab][c][km] = min
 // over the original arcs of c, per direction (python/cch_nb.py:534-542 per
-// metric).  Threads run metric-fastest, so reads of WT and writes of UD are
-// coalesced.
-__global__ void __launch_bounds__(256) k_base_batch(i64 mp, int32_t K, int32_t SW, int32_t slabs,
+// metric).  A block holds 256 / SW arcs x SW metrics, metric fastest, so reads
+// of WT and writes of UD are coalesced and the index math is shifts and masks;
+// the arc's original-arc list is read once for all slabs.
+__global__ void __launch_bounds__(256) k_base_batch(i64 mp, int32_t K, int32_t sw_log2, int32_t slabs,
                                                     const int32_t* __restrict__ ae_off,
                                                     const int32_t* __restrict__ ae_idx,
                                                     const int32_t* __restrict__ WT, int2* UD,
                                                     uint32_t* status) {
+  const int32_t SW = 1 << sw_log2;
+  const int32_t km = threadIdx.x & (SW - 1);
+  const i64 c = (i64)blockIdx.x * (256 >> sw_log2) + (threadIdx.x >> sw_log2);
+  if (c >= mp) return;
   uint32_t st = 0;
-  const i64 total = (i64)slabs * mp * SW;
-  const i64 stride = (i64)gridDim.x * blockDim.x;
-  for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
-    const int32_t km = (int32_t)(i % SW);
-    const i64 rest = i / SW;
-    const i64 c = rest % mp;
-    const int32_t k = (int32_t)(rest / mp) * SW + km;
-    if (k >= K) continue;
+  const int32_t e0 = ae_off[c], e1 = ae_off[c + 1];
+  for (int32_t sl = 0; sl < slabs; sl++) {
+    const int32_t k = sl * SW + km;
+    if (k >= K) break;
     int32_t up = kInf, dn = kInf;
-    const int32_t e1 = ae_off[c + 1];
-    for (int32_t j = ae_off[c]; j < e1; j++) {
+    for (int32_t j = e0; j < e1; j++) {
       const int32_t e = ae_idx[j];
       const int32_t x = WT[(i64)(e >> 1) * K + k];
       if (x == (int32_t)0x80000000) st |= kStatusBadWeight;
       if (e & 1) up = min(up, x);
       else dn = min(dn, x);
     }
-    UD[i] = make_int2(up, dn);
+    UD[((i64)sl * mp + c) * SW + km] = make_int2(up, dn);
   }
   if (st) atomicOr(status, st);
 }
@@ -310,9 +310,10 @@ extern "C" int nwcch_customize_batch(nwcch_dev* h, const int32_t* W, int32_t K, 
       ProfScope ps(d, s, kKOther);
       dim3 grid((unsigned)((d.m + 63) / 64), (unsigned)((K + 63) / 64));
       if (d.m > 0) hipLaunchKernelGGL(k_transpose_w, grid, dim3(256), 0, s, W, d.m, K, d.wt);
-      const i64 total = d.mp * (i64)KL * slices;
-      const i64 blocks = std::min<i64>(std::max<i64>((total + 255) / 256, 1), 65536);
-      hipLaunchKernelGGL(k_base_batch, dim3((unsigned)blocks), dim3(256), 0, s, d.mp, K, KL, slices, d.ae_off,
+      int32_t sw_log2 = 0;
+      while ((1 << sw_log2) < KL) sw_log2++;
+      const i64 blocks = std::max<i64>((d.mp + (256 >> sw_log2) - 1) / (256 >> sw_log2), 1);
+      hipLaunchKernelGGL(k_base_batch, dim3((unsigned)blocks), dim3(256), 0, s, d.mp, K, sw_log2, slices, d.ae_off,
                          d.ae_idx, d.wt, ud, d.status);
     }
     for (const LevelRange& r : d.lv) {

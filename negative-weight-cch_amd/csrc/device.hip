@@ -359,10 +359,28 @@ __global__ void __launch_bounds__(64) k_query(const VInfo* __restrict__ vinfo,
   int32_t lca_depth = -1;
   int32_t nscan = 0, p = 0;
   bool active = true;
+  // Software pipeline over the chain: while the row of the current vertex is
+  // relaxed, the first 32 arcs of the next vertex's row and the chain entry of
+  // the vertex after it are already being loaded (neither depends on labels).
+  VInfo nx = VInfo{0, 0, -1, 0};
+  if (vi.parent >= 0) nx = vinfo[vi.parent];
+  int32_t ra = 0;
+  int2 ru = make_int2(kInf, kInf);
+  if (hl < vi.deg) {
+    ra = adep[vi.first + hl];
+    ru = ud[(i64)(vi.first + hl) * ud_stride];
+  }
   while (__any(active)) {
-    const int32_t par = vi.parent;
-    VInfo nx;
-    if (active && par >= 0) nx = vinfo[par];
+    VInfo nn = VInfo{0, 0, -1, 0};
+    int32_t na = 0;
+    int2 nu = make_int2(kInf, kInf);
+    if (active && vi.parent >= 0) {
+      if (nx.parent >= 0) nn = vinfo[nx.parent];
+      if (hl < nx.deg) {
+        na = adep[nx.first + hl];
+        nu = ud[(i64)(nx.first + hl) * ud_stride];
+      }
+    }
     if (active) {
       if (half && lca_depth < 0) {
         const int2 e = euler[x];
@@ -371,7 +389,12 @@ __global__ void __launch_bounds__(64) k_query(const VInfo* __restrict__ vinfo,
       const int32_t dx = lab[p];
       if (dx != kInf) {
         nscan += vi.deg;
-        for (int32_t j = hl; j < vi.deg; j += 32) {
+        if (hl < vi.deg) {
+          const int32_t v = inf_add(dx, half ? ru.y : ru.x, st);
+          const int32_t pos = D - ra;
+          if (v < lab[pos]) lab[pos] = v;
+        }
+        for (int32_t j = hl + 32; j < vi.deg; j += 32) {  // rows longer than 32 arcs
           const int32_t a = vi.first + j;
           const int32_t pos = D - adep[a];
           const int2 u = ud[(i64)a * ud_stride];
@@ -382,8 +405,8 @@ __global__ void __launch_bounds__(64) k_query(const VInfo* __restrict__ vinfo,
     }
     __syncthreads();
     if (active) {
-      if (par < 0) active = false;
-      else { x = par; vi = nx; p++; }
+      if (vi.parent < 0) active = false;
+      else { x = vi.parent; vi = nx; nx = nn; ra = na; ru = nu; p++; }
     }
   }
   const int32_t lcad = __shfl(lca_depth, 32, 64);

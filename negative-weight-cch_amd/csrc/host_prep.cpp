@@ -14,6 +14,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <limits>
 #include <memory>
 #include <cmath>
 #include <condition_variable>
@@ -151,7 +152,10 @@ struct CellFlow {
   // blocking-flow phase restricted to shortest paths yields about one, since
   // path lengths across a clustered point set all differ).  Any maximum flow
   // gives the same residual reachability from S, hence the same cut.
-  long maxflow_rounds(long limit) {
+  // Stops early, returning a value above `*bound`, once the flow exceeds
+  // *bound (the smallest cut another direction has already completed): such
+  // a cut can no longer be the chosen one.
+  long maxflow_rounds(long limit, const std::atomic<long>* bound) {
     std::vector<int>& seen = level;  // round stamp of the search
     std::vector<int>& par = it;      // tree arc into each reached node
     std::vector<int> used(N, 0);     // round stamp of nodes on augmented paths
@@ -198,6 +202,7 @@ struct CellFlow {
         got += f;
       }
       if (got == 0) break;  // cannot happen while T is reachable; guards the loop
+      if (flow > bound->load(std::memory_order_relaxed)) break;
     }
     return flow;
   }
@@ -280,7 +285,8 @@ Split geometric_split(const std::vector<i32>& cell, const Member& loc,
 // touches a sink (no finite vertex cut exists).
 Split direction_cut(int d, const std::vector<i32>& cell, const Member& loc,
                     const std::vector<i64>& noff, const std::vector<i32>& nbr,
-                    const double* x, const double* y, const std::vector<int>& nid) {
+                    const double* x, const double* y, const std::vector<int>& nid,
+                    std::atomic<long>* bound) {
   Split cur;
   const int s = (int)cell.size();
   const int k = std::max(1, (int)(s * 0.25));
@@ -319,7 +325,11 @@ Split direction_cut(int d, const std::vector<i32>& cell, const Member& loc,
   }
   CellFlow net;
   net.build(cell, loc, noff, nbr, role, active, nid);
-  net.maxflow_rounds((long)s + 1);
+  const long flow = net.maxflow_rounds((long)s + 1, bound);
+  long b = bound->load(std::memory_order_relaxed);
+  if (flow > b) return cur;  // loses to a finished direction
+  while (flow < b && !bound->compare_exchange_weak(b, flow, std::memory_order_relaxed)) {
+  }
   net.reach(seen);
   cur.ok = true;
   for (int l = 0; l < s; l++) {
@@ -370,13 +380,16 @@ Split inertial_split(const std::vector<i32>& cell, const Member& loc,
                      const double* x, const double* y, bool parallel) {
   Split cut[4];
   const std::vector<int> nid = curve_numbering(cell, x, y);
+  // smallest cut completed so far: a direction whose flow exceeds it stops
+  // (it cannot win), so the choice below is the same in any completion order
+  std::atomic<long> bound{std::numeric_limits<long>::max()};
   if (parallel) {
     std::vector<std::thread> th;
     for (int d = 0; d < 4; d++)
-      th.emplace_back([&, d] { cut[d] = direction_cut(d, cell, loc, noff, nbr, x, y, nid); });
+      th.emplace_back([&, d] { cut[d] = direction_cut(d, cell, loc, noff, nbr, x, y, nid, &bound); });
     for (auto& t : th) t.join();
   } else {
-    for (int d = 0; d < 4; d++) cut[d] = direction_cut(d, cell, loc, noff, nbr, x, y, nid);
+    for (int d = 0; d < 4; d++) cut[d] = direction_cut(d, cell, loc, noff, nbr, x, y, nid, &bound);
   }
   int best = -1;
   auto bal = [](const Split& q) { return std::min(q.A.size(), q.B.size()); };
