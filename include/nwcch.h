@@ -203,7 +203,10 @@ int nwcch_status(nwcch_dev* h, uint32_t* out, int clear, void* stream);
  *                   team of lanes, up to 64, with at most this many lower
  *                   triangles per lane; default 4, values 1..256.
  *   "batch_tri_per_lane"  batched sweep: wavefronts per arc, up to 16, likewise;
- *                   default 8, values 1..256. */
+ *                   default 8, values 1..256.
+ *   "query_deep_max"  query launches of at most this many queries prefetch 128
+ *                   arcs of the next chain vertex's row (shorter latency),
+ *                   larger launches 32 (more wavefronts per CU); default 32768. */
 int nwcch_set_tuning(nwcch_dev* h, const char* key, int32_t value);
 int nwcch_get_tuning(const nwcch_dev* h, const char* key, int32_t* value);
 

@@ -313,6 +313,7 @@ __global__ void k_scan_finish(const unsigned long long* src, long long* dst) {
 // and no reset pass over n is needed.  The t half finds the lowest common
 // ancestor with Euler-tour intervals; afterwards every common ancestor (depth
 // 0..depth(lca)) is a meeting candidate, min(df + dr) over them.
+template <int kPre>
 __global__ void __launch_bounds__(64) k_query(const VInfo* __restrict__ vinfo,
                                               const int2* __restrict__ euler,
                                               const int32_t* __restrict__ adep,
@@ -360,25 +361,39 @@ __global__ void __launch_bounds__(64) k_query(const VInfo* __restrict__ vinfo,
   int32_t nscan = 0, p = 0;
   bool active = true;
   // Software pipeline over the chain: while the row of the current vertex is
-  // relaxed, the first 32 arcs of the next vertex's row and the chain entry of
-  // the vertex after it are already being loaded (neither depends on labels).
+  // relaxed, the first kPre * 32 arcs of the next vertex's row and the chain
+  // entry of the vertex after it are already being loaded (neither depends on
+  // labels).  Rows near the top of the hierarchy are the long ones.  A deep
+  // prefetch shortens one query; a shallow one leaves more wavefronts per CU
+  // for large batches.
   VInfo nx = VInfo{0, 0, -1, 0};
   if (vi.parent >= 0) nx = vinfo[vi.parent];
-  int32_t ra = 0;
-  int2 ru = make_int2(kInf, kInf);
-  if (hl < vi.deg) {
-    ra = adep[vi.first + hl];
-    ru = ud[(i64)(vi.first + hl) * ud_stride];
+  int32_t ra[kPre];
+  int2 ru[kPre];
+#pragma unroll
+  for (int c = 0; c < kPre; c++) {
+    const int32_t j = hl + 32 * c;
+    ra[c] = 0;
+    ru[c] = make_int2(kInf, kInf);
+    if (j < vi.deg) {
+      ra[c] = adep[vi.first + j];
+      ru[c] = ud[(i64)(vi.first + j) * ud_stride];
+    }
   }
   while (__any(active)) {
     VInfo nn = VInfo{0, 0, -1, 0};
-    int32_t na = 0;
-    int2 nu = make_int2(kInf, kInf);
-    if (active && vi.parent >= 0) {
-      if (nx.parent >= 0) nn = vinfo[nx.parent];
-      if (hl < nx.deg) {
-        na = adep[nx.first + hl];
-        nu = ud[(i64)(nx.first + hl) * ud_stride];
+    int32_t na[kPre];
+    int2 nu[kPre];
+    const bool pre = active && vi.parent >= 0;
+    if (pre && nx.parent >= 0) nn = vinfo[nx.parent];
+#pragma unroll
+    for (int c = 0; c < kPre; c++) {
+      const int32_t j = hl + 32 * c;
+      na[c] = 0;
+      nu[c] = make_int2(kInf, kInf);
+      if (pre && j < nx.deg) {
+        na[c] = adep[nx.first + j];
+        nu[c] = ud[(i64)(nx.first + j) * ud_stride];
       }
     }
     if (active) {
@@ -389,12 +404,15 @@ __global__ void __launch_bounds__(64) k_query(const VInfo* __restrict__ vinfo,
       const int32_t dx = lab[p];
       if (dx != kInf) {
         nscan += vi.deg;
-        if (hl < vi.deg) {
-          const int32_t v = inf_add(dx, half ? ru.y : ru.x, st);
-          const int32_t pos = D - ra;
-          if (v < lab[pos]) lab[pos] = v;
+#pragma unroll
+        for (int c = 0; c < kPre; c++) {
+          if (hl + 32 * c < vi.deg) {
+            const int32_t v = inf_add(dx, half ? ru[c].y : ru[c].x, st);
+            const int32_t pos = D - ra[c];
+            if (v < lab[pos]) lab[pos] = v;
+          }
         }
-        for (int32_t j = hl + 32; j < vi.deg; j += 32) {  // rows longer than 32 arcs
+        for (int32_t j = hl + 32 * kPre; j < vi.deg; j += 32) {  // rows longer than kPre * 32 arcs
           const int32_t a = vi.first + j;
           const int32_t pos = D - adep[a];
           const int2 u = ud[(i64)a * ud_stride];
@@ -405,8 +423,19 @@ __global__ void __launch_bounds__(64) k_query(const VInfo* __restrict__ vinfo,
     }
     __syncthreads();
     if (active) {
-      if (vi.parent < 0) active = false;
-      else { x = vi.parent; vi = nx; nx = nn; ra = na; ru = nu; p++; }
+      if (vi.parent < 0) {
+        active = false;
+      } else {
+        x = vi.parent;
+        vi = nx;
+        nx = nn;
+        p++;
+#pragma unroll
+        for (int c = 0; c < kPre; c++) {
+          ra[c] = na[c];
+          ru[c] = nu[c];
+        }
+      }
     }
   }
   const int32_t lcad = __shfl(lca_depth, 32, 64);
@@ -535,7 +564,8 @@ extern "C" void nwcch_dev_destroy(nwcch_dev* h) {
   (void)hipSetDevice(h->d.device);
   (void)hipDeviceSynchronize();
   for (hipEvent_t e : h->d.ev_pool) (void)hipEventDestroy(e);
-  if (h->d.stage_ev) (void)hipEventDestroy(h->d.stage_ev);
+  for (auto& st : h->d.stage)
+    if (st.ev) (void)hipEventDestroy(st.ev);
   for (void* p : h->d.owned_host) (void)hipHostFree(p);
   for (void* p : h->d.owned) (void)hipFree(p);
   delete h;
@@ -636,10 +666,16 @@ int launch_queries(nwcch_dev* h, const int32_t* ud, int64_t ud_stride, const int
     const size_t lds = (size_t)2 * maxlen * sizeof(int32_t);
     if (lds > 160 * 1024) throw std::runtime_error("elimination tree too deep for the LDS query kernel");
     ProfScope ps(d, s, kKQuery);
-    hipLaunchKernelGGL(k_query, dim3((unsigned)nq), dim3(64), lds, s, d.vinfo, d.euler, d.adep,
-                       reinterpret_cast<const int2*>(ud), (i64)ud_stride, (const long long*)ud_table, qmetric,
-                       n_metrics, s_rank, t_rank, (i64)nq, order, (int32_t)d.n, maxlen, dist, scanned,
-                       d.status);
+    if (nq <= d.query_deep_max)
+      hipLaunchKernelGGL(k_query<4>, dim3((unsigned)nq), dim3(64), lds, s, d.vinfo, d.euler, d.adep,
+                         reinterpret_cast<const int2*>(ud), (i64)ud_stride, (const long long*)ud_table, qmetric,
+                         n_metrics, s_rank, t_rank, (i64)nq, order, (int32_t)d.n, maxlen, dist, scanned,
+                         d.status);
+    else
+      hipLaunchKernelGGL(k_query<1>, dim3((unsigned)nq), dim3(64), lds, s, d.vinfo, d.euler, d.adep,
+                         reinterpret_cast<const int2*>(ud), (i64)ud_stride, (const long long*)ud_table, qmetric,
+                         n_metrics, s_rank, t_rank, (i64)nq, order, (int32_t)d.n, maxlen, dist, scanned,
+                         d.status);
     HIP_OK(hipGetLastError());
   } catch (const std::exception& e) {
     return fail(e);
@@ -677,6 +713,16 @@ extern "C" int nwcch_query(nwcch_dev* h, const int32_t* ud, const int32_t* s_ran
 extern "C" int nwcch_set_tuning(nwcch_dev* h, const char* key, int32_t value) {
   if (!h || !key) { set_error("null argument"); return NWCCH_ERR_ARG; }
   const std::string k(key);
+  if (k == "upd_blocks") {
+    if (value < 1 || value > 65536) { set_error("upd_blocks out of range [1, 65536]"); return NWCCH_ERR_ARG; }
+    h->d.upd_blocks = value;
+    return NWCCH_OK;
+  }
+  if (k == "query_deep_max") {
+    if (value < 0) { set_error("query_deep_max must be >= 0"); return NWCCH_ERR_ARG; }
+    h->d.query_deep_max = value;
+    return NWCCH_OK;
+  }
   if (k == "flat_xcd") {
     if (value < 0 || value > 1) { set_error("flat_xcd is 0 or 1"); return NWCCH_ERR_ARG; }
     h->d.flat_xcd = value;
@@ -701,6 +747,8 @@ extern "C" int nwcch_get_tuning(const nwcch_dev* h, const char* key, int32_t* va
   else if (k == "batch_tri_per_lane") *value = h->d.batch_tri_per_lane;
   else if (k == "flat_min_tri") *value = h->d.flat_min_tri;
   else if (k == "flat_xcd") *value = h->d.flat_xcd;
+  else if (k == "query_deep_max") *value = h->d.query_deep_max;
+  else if (k == "upd_blocks") *value = h->d.upd_blocks;
   else { set_error("unknown tuning key: " + k); return NWCCH_ERR_ARG; }
   return NWCCH_OK;
 }

@@ -64,6 +64,10 @@ struct DevTopo {
   // tunables (nwcch_set_tuning) and the launch geometry derived from them
   int32_t tri_per_lane = kTriPerLane;        // single-metric level kernel
   int32_t batch_tri_per_lane = 8;            // batched level kernel
+  int32_t upd_blocks = 1024;                 // per-height update launches: at most this many
+                                             // workgroups of 4 wavefronts
+  int32_t query_deep_max = 32768;            // query launches up to this size prefetch 4 x 32
+                                             // arcs of the next row, larger ones 32
   int32_t flat_min_tri = 0;                  // levels whose heaviest arc has >= this many
                                              // triangles run triangle-parallel
   int32_t flat_xcd = 0;                      // triangle-parallel levels: one contiguous
@@ -92,12 +96,19 @@ struct DevTopo {
   int32_t* qcnt = nullptr;           // [levels] bucket fill
   int32_t* chg = nullptr;            // [mp] changed arcs of the last update (unordered)
   long long* upd_res = nullptr;      // [4] evaluated, changed, negative changed, first negative
-  i64* stage_e = nullptr;            // staging of the changes (grown on demand)
-  int32_t* stage_w = nullptr;
-  i64* hstage_e = nullptr;           // pinned host side of the staging
-  int32_t* hstage_w = nullptr;
-  i64 stage_cap = 0;
-  hipEvent_t stage_ev = nullptr;     // the last staging copy
+  // staging of the changes: a ring of slots (device + pinned host buffers,
+  // grown on demand), each reused only after its previous copy has completed
+  struct Stage {
+    i64* e = nullptr;
+    int32_t* w = nullptr;
+    i64* he = nullptr;
+    int32_t* hw = nullptr;
+    i64 cap = 0;
+    hipEvent_t ev = nullptr;
+  };
+  static constexpr int kStages = 8;
+  Stage stage[kStages];
+  int stage_next = 0;
   std::vector<void*> owned_host;     // pinned host allocations
   int32_t epoch = 0;
   // scratch

@@ -304,12 +304,15 @@ extern "C" int nwcch_update(nwcch_dev* h, int32_t* w, int32_t* ud, const int64_t
       if (a >= 0) L0 = std::min(L0, d.arc_lvl_host[a]);
     }
     // the changes travel through pinned host staging, so the caller's arrays
-    // may be released as soon as this call returns; the staging is reused only
-    // after the previous call's copies have completed
-    if (d.stage_ev) HIP_OK(hipEventSynchronize(d.stage_ev));
-    else HIP_OK(hipEventCreateWithFlags(&d.stage_ev, hipEventDisableTiming));
-    if (k > d.stage_cap) {
-      i64 cap = std::max<i64>(k, 2 * d.stage_cap);
+    // may be released as soon as this call returns.  Slots rotate; a slot is
+    // reused only after its previous copies have completed, so a new update
+    // waits on the host only when kStages earlier ones are still queued.
+    DevTopo::Stage& sg = d.stage[d.stage_next];
+    d.stage_next = (d.stage_next + 1) % DevTopo::kStages;
+    if (sg.ev) HIP_OK(hipEventSynchronize(sg.ev));
+    else HIP_OK(hipEventCreateWithFlags(&sg.ev, hipEventDisableTiming));
+    if (k > sg.cap) {
+      i64 cap = std::max<i64>(k, 2 * sg.cap);
       HIP_OK(hipStreamSynchronize(s));
       i64* e2 = nullptr;
       int32_t* w2 = nullptr;
@@ -323,26 +326,26 @@ extern "C" int nwcch_update(nwcch_dev* h, int32_t* w, int32_t* ud, const int64_t
       HIP_OK(hipHostMalloc(&hw, cap * sizeof(int32_t), hipHostMallocDefault));
       d.owned_host.push_back(he);
       d.owned_host.push_back(hw);
-      d.stage_e = e2;
-      d.stage_w = w2;
-      d.hstage_e = (i64*)he;
-      d.hstage_w = (int32_t*)hw;
-      d.stage_cap = cap;
+      sg.e = e2;
+      sg.w = w2;
+      sg.he = (i64*)he;
+      sg.hw = (int32_t*)hw;
+      sg.cap = cap;
     }
     if (k > 0) {
-      std::memcpy(d.hstage_e, edge_idx, k * sizeof(i64));
-      std::memcpy(d.hstage_w, new_w, k * sizeof(int32_t));
-      HIP_OK(hipMemcpyAsync(d.stage_e, d.hstage_e, k * sizeof(i64), hipMemcpyHostToDevice, s));
-      HIP_OK(hipMemcpyAsync(d.stage_w, d.hstage_w, k * sizeof(int32_t), hipMemcpyHostToDevice, s));
-      HIP_OK(hipEventRecord(d.stage_ev, s));
+      std::memcpy(sg.he, edge_idx, k * sizeof(i64));
+      std::memcpy(sg.hw, new_w, k * sizeof(int32_t));
+      HIP_OK(hipMemcpyAsync(sg.e, sg.he, k * sizeof(i64), hipMemcpyHostToDevice, s));
+      HIP_OK(hipMemcpyAsync(sg.w, sg.hw, k * sizeof(int32_t), hipMemcpyHostToDevice, s));
+      HIP_OK(hipEventRecord(sg.ev, s));
     }
     d.epoch++;
     UpdArgs A = make_args(d, w, ud, changed, changed_cap);
     hipLaunchKernelGGL(k_upd_reset, dim3(std::max(1, std::min(64, (d.levels + 255) / 256))), dim3(256), 0, s,
                        A, d.levels);
     if (k > 0)
-      hipLaunchKernelGGL(k_upd_seed, dim3((unsigned)((k + 255) / 256)), dim3(256), 0, s, A, d.stage_e,
-                         d.stage_w, (i64)k, d.m);
+      hipLaunchKernelGGL(k_upd_seed, dim3((unsigned)((k + 255) / 256)), dim3(256), 0, s, A, sg.e, sg.w, (i64)k,
+                         d.m);
     // mode: 0 = auto, 1 = one persistent workgroup, 2 = one launch per height
     const bool persistent = mode == 1 || (mode == 0 && k <= 64);
     if (L0 < d.levels) {
@@ -352,7 +355,7 @@ extern "C" int nwcch_update(nwcch_dev* h, int32_t* w, int32_t* ud, const int64_t
       } else {
         for (int32_t L = L0; L < d.levels; L++) {
           const i64 cap = d.lv[L].end - d.lv[L].begin;
-          const int blocks = (int)std::max<i64>(1, std::min<i64>((cap + 3) / 4, 1024));
+          const int blocks = (int)std::max<i64>(1, std::min<i64>((cap + 3) / 4, d.upd_blocks));
           ProfScope ps(d, s, kKUpdate);
           hipLaunchKernelGGL(k_upd_level, dim3(blocks), dim3(256), 0, s, A, L);
         }
