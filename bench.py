@@ -96,14 +96,16 @@ def event_ms(fn, reps):
     return [a.elapsed_time(b) for a, b in evs]
 
 
-def prepare(cfg):
+def prepare(cfg, progress=None):
     from nwcch import _lib, graphs
     t0 = time.perf_counter()
-    g = graphs.make_config(cfg)
+    g = graphs.make_config(cfg, progress=progress)
     t_gen = time.perf_counter() - t0
+    log(f"config {cfg}: graph n={g.n} m={g.m} generated in {t_gen:.1f}s")
     t0 = time.perf_counter()
     rank = _lib.order(g.n, g.off, g.tgt, g.x, g.y, 64)
     t_order = time.perf_counter() - t0
+    log(f"config {cfg}: nested dissection {t_order:.1f}s")
     t0 = time.perf_counter()
     topo = _lib.Topology.from_graph(g.n, g.off, g.tgt, rank)
     t_topo = time.perf_counter() - t0
@@ -149,11 +151,11 @@ def plain(args, out):
     scan_ms = event_ms(lambda: dt.negcycle_scan(ud, out=scan), args.steps)
     # per-launch durations of the level kernel: HIP events recorded by the
     # library around each launch, on the stream the launch goes to
-    n_lv = int(np.count_nonzero(np.asarray(level_triangles(topo)) > 0))
-    dt.profile_begin(args.steps * (n_lv + 2))
+    dt.profile_begin(args.steps * (topo.levels + 2))
     for _ in range(args.steps):
         dt.customize(w, out=ud)
     kid, kms = dt.profile_end()
+    n_lv = int((kid == 0).sum()) // args.steps   # level launches per customization
     base_ms = float(np.median(kms[kid == 5]))
     launches = 1 + n_lv          # the base pass + one launch per level with triangles
     best = min(t_step, t_graph)
@@ -258,16 +260,6 @@ def plain(args, out):
     del dt
 
 
-def level_triangles(topo):
-    from nwcch import _lib  # noqa: F401
-    sched, toff, _ = topo.export_triangles()
-    ex = topo.export()
-    import numpy as np
-    tail = np.repeat(np.arange(topo.n), np.diff(ex["first"]))
-    lvl = ex["level"][tail[sched]]
-    return np.bincount(lvl, weights=np.diff(toff), minlength=topo.levels)
-
-
 def cpu_baseline(g, rank, topo, reps=5):
     from oracle import oracle as O
     if O.have_ref():
@@ -337,11 +329,11 @@ def batched(args, out):
     t_s = timed(single_all, 2)
     bms = statistics.median(event_ms(lambda: dt.customize_batch(Wd, out=UD, slab=slab), args.batch_steps))
     B = cust_bytes(g.m, topo.mp, topo.triangles, topo.n_mapped, K)
-    n_lv = int(np.count_nonzero(np.asarray(level_triangles(topo)) > 0))
-    dt.profile_begin(args.batch_steps * (n_lv + 4))
+    dt.profile_begin(args.batch_steps * (topo.levels + 4))
     for _ in range(args.batch_steps):
         dt.customize_batch(Wd, out=UD, slab=slab)
     kid, kms = dt.profile_end()
+    n_lv = int((kid == 4).sum()) // args.batch_steps
     bbase_ms = float(np.median(kms[kid == 5]))
     blv_us = 1000.0 * (bms - bbase_ms) / n_lv
     B_blv = 8 * topo.triangles + K * 16 * topo.mp   # operand ids shared, 16 B per arc per metric
@@ -392,7 +384,7 @@ def serving(args, out):
     W, _ = graphs.metric_family(g, K, seed=3, hmax=800)
     dt = DeviceTopology(topo, 0)
     srv = serve.Server(dt, rank, metric_budget=K * 2 * (4 * topo.m + 8 * topo.mp), slot_budget=64 << 20,
-                       max_batch=65536, n_streams=2, snapshot=True)
+                       max_batch=65536, n_streams=args.serve_streams, snapshot=True)
     w0 = {}
     for k in range(K):
         if not srv.add_metric(k, W[k]):
@@ -405,7 +397,7 @@ def serving(args, out):
                                       update_rate=args.serve_update_rate)
         reqs = serve.absolute_updates(reqs, {k: srv.weights_at(k, srv.metrics[k].version, w0[k])
                                              for k in range(K)})
-        st = srv.run(reqs, max_seconds=60 + 10 * args.serve_seconds)
+        st = srv.run(reqs, max_seconds=60 + 10 * args.serve_seconds, max_inflight=args.serve_inflight)
         st["arrival_rate"] = rate
         runs.append(st)
         all_reqs += reqs
@@ -439,7 +431,8 @@ def full_scale(args, out):
     from nwcch import graphs
     from nwcch.device import DeviceTopology
 
-    g, rank, topo, host = prepare(4)
+    os.environ["NWCCH_PROGRESS"] = "1"   # a line per big nested-dissection cell
+    g, rank, topo, host = prepare(4, progress=lambda msg: log(f"config 4: {msg}"))
     dt = DeviceTopology(topo, 0)
     log(f"device topology uploaded ({torch.cuda.memory_allocated() / 1e9:.1f} GB)")
     w = dt.weights(g.w)
@@ -449,10 +442,10 @@ def full_scale(args, out):
     dt.check_status()
     cust_ms = event_ms(lambda: dt.customize(w, out=ud), args.steps)
     scan_ms = event_ms(lambda: dt.negcycle_scan(ud, out=scan), args.steps)
-    n_lv = int(np.count_nonzero(np.asarray(level_triangles(topo)) > 0))
-    dt.profile_begin(n_lv + 2)
+    dt.profile_begin(topo.levels + 2)
     dt.customize(w, out=ud)
     kid, kms = dt.profile_end()
+    n_lv = int((kid == 0).sum())
     base_ms = float(np.median(kms[kid == 5]))
     cmed = statistics.median(cust_ms)
     B_lv = level_bytes(topo.mp, topo.triangles)
@@ -582,6 +575,8 @@ def main():
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--serve-rates", type=float, nargs="+", default=[200_000, 1_000_000, 2_000_000])
     ap.add_argument("--serve-seconds", type=float, default=0.25)
+    ap.add_argument("--serve-inflight", type=int, default=2, help="query launches on the device at once")
+    ap.add_argument("--serve-streams", type=int, default=2, help="streams the query launches rotate over")
     ap.add_argument("--serve-update-rate", type=float, default=40.0)
     args = ap.parse_args()
     if args.gpus != 1:

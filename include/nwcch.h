@@ -204,6 +204,9 @@ int nwcch_status(nwcch_dev* h, uint32_t* out, int clear, void* stream);
  *                   triangles per lane; default 4, values 1..256.
  *   "batch_tri_per_lane"  batched sweep: wavefronts per arc, up to 16, likewise;
  *                   default 8, values 1..256.
+ *   "upd_persistent_max"  nwcch_update in mode 0 runs updates of at most this
+ *                   many changed arcs in one persistent workgroup, larger ones
+ *                   one launch per height; default 2.
  *   "query_deep_max"  query launches of at most this many queries prefetch 128
  *                   arcs of the next chain vertex's row (shorter latency),
  *                   larger launches 32 (more wavefronts per CU); default 32768. */

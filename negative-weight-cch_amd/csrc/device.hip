@@ -713,9 +713,9 @@ extern "C" int nwcch_query(nwcch_dev* h, const int32_t* ud, const int32_t* s_ran
 extern "C" int nwcch_set_tuning(nwcch_dev* h, const char* key, int32_t value) {
   if (!h || !key) { set_error("null argument"); return NWCCH_ERR_ARG; }
   const std::string k(key);
-  if (k == "upd_blocks") {
-    if (value < 1 || value > 65536) { set_error("upd_blocks out of range [1, 65536]"); return NWCCH_ERR_ARG; }
-    h->d.upd_blocks = value;
+  if (k == "upd_persistent_max") {
+    if (value < 0) { set_error("upd_persistent_max must be >= 0"); return NWCCH_ERR_ARG; }
+    h->d.upd_persistent_max = value;
     return NWCCH_OK;
   }
   if (k == "query_deep_max") {
@@ -748,7 +748,7 @@ extern "C" int nwcch_get_tuning(const nwcch_dev* h, const char* key, int32_t* va
   else if (k == "flat_min_tri") *value = h->d.flat_min_tri;
   else if (k == "flat_xcd") *value = h->d.flat_xcd;
   else if (k == "query_deep_max") *value = h->d.query_deep_max;
-  else if (k == "upd_blocks") *value = h->d.upd_blocks;
+  else if (k == "upd_persistent_max") *value = h->d.upd_persistent_max;
   else { set_error("unknown tuning key: " + k); return NWCCH_ERR_ARG; }
   return NWCCH_OK;
 }

@@ -14,6 +14,8 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
+#include <cstdio>
 #include <limits>
 #include <memory>
 #include <cmath>
@@ -427,6 +429,8 @@ void nested_dissection(i64 n, const std::vector<i64>& noff, const std::vector<i3
   std::condition_variable cv;
   int busy = 0;
   std::string err;
+  const char* pe = std::getenv("NWCCH_PROGRESS");
+  const bool progress = pe && pe[0] == '1';
   auto work = [&](Task t, std::vector<Task>& out) {
     std::vector<i32>& cell = t.nodes;
     const i64 s = (i64)cell.size();
@@ -444,8 +448,16 @@ void nested_dissection(i64 n, const std::vector<i64>& noff, const std::vector<i3
       locv[cell[l]] = (i32)l;
       cellof[cell[l]].store(loc.id, std::memory_order_relaxed);
     }
+    const auto t_split = std::chrono::steady_clock::now();
     Split sp = inertial_split(cell, loc, noff, nbr, x, y, s >= 20000);
     if (!sp.ok) sp = geometric_split(cell, loc, noff, nbr, x, y);
+    if (progress && s >= 1000000) {
+      // a line per big cell: hours-long orders of continental graphs stay visible
+      const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_split).count();
+      std::fprintf(stderr, "nested dissection: cell of %lld vertices split (separator %zu) in %.1f s\n",
+                   (long long)s, sp.sep.size(), dt);
+      std::fflush(stderr);
+    }
     const i64 hi = t.lo + s;
     const i64 ns = (i64)sp.sep.size();
     for (i64 i = 0; i < ns; i++) rank[sp.sep[i]] = (i32)(hi - ns + i);

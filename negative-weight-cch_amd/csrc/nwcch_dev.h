@@ -64,8 +64,8 @@ struct DevTopo {
   // tunables (nwcch_set_tuning) and the launch geometry derived from them
   int32_t tri_per_lane = kTriPerLane;        // single-metric level kernel
   int32_t batch_tri_per_lane = 8;            // batched level kernel
-  int32_t upd_blocks = 1024;                 // per-height update launches: at most this many
-                                             // workgroups of 4 wavefronts
+  int32_t upd_persistent_max = 2;            // updates of at most this many changed arcs run
+                                             // in one persistent workgroup (mode auto)
   int32_t query_deep_max = 32768;            // query launches up to this size prefetch 4 x 32
                                              // arcs of the next row, larger ones 32
   int32_t flat_min_tri = 0;                  // levels whose heaviest arc has >= this many

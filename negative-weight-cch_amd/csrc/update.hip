@@ -12,8 +12,9 @@
 // (epoch stamps), so bucket L never holds more than the arcs of height L.
 //
 // Two drivers share one arc evaluation: a single persistent workgroup that walks
-// the non-empty buckets with workgroup barriers (small updates, latency bound),
-// and one launch per height with many workgroups (large updates).
+// the non-empty buckets with workgroup barriers (one or two changed arcs,
+// latency bound), and one launch per height with many workgroups (larger
+// updates, whose frontiers hold thousands of arcs).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -346,8 +347,10 @@ extern "C" int nwcch_update(nwcch_dev* h, int32_t* w, int32_t* ud, const int64_t
     if (k > 0)
       hipLaunchKernelGGL(k_upd_seed, dim3((unsigned)((k + 255) / 256)), dim3(256), 0, s, A, sg.e, sg.w, (i64)k,
                          d.m);
-    // mode: 0 = auto, 1 = one persistent workgroup, 2 = one launch per height
-    const bool persistent = mode == 1 || (mode == 0 && k <= 64);
+    // mode: 0 = auto, 1 = one persistent workgroup, 2 = one launch per height.
+    // One or two changed arcs usually leave a frontier that one workgroup
+    // handles fastest; more spread over thousands of arcs.
+    const bool persistent = mode == 1 || (mode == 0 && k <= d.upd_persistent_max);
     if (L0 < d.levels) {
       if (persistent) {
         ProfScope ps(d, s, kKUpdate);
@@ -355,7 +358,7 @@ extern "C" int nwcch_update(nwcch_dev* h, int32_t* w, int32_t* ud, const int64_t
       } else {
         for (int32_t L = L0; L < d.levels; L++) {
           const i64 cap = d.lv[L].end - d.lv[L].begin;
-          const int blocks = (int)std::max<i64>(1, std::min<i64>((cap + 3) / 4, d.upd_blocks));
+          const int blocks = (int)std::max<i64>(1, std::min<i64>((cap + 3) / 4, 1024));
           ProfScope ps(d, s, kKUpdate);
           hipLaunchKernelGGL(k_upd_level, dim3(blocks), dim3(256), 0, s, A, L);
         }

@@ -181,13 +181,64 @@ def sample_field(f, box, x, y):
     return v
 
 
+def _tile_edges(job):
+    """Delaunay edges of one tile: the triangulation of the tile's points plus a
+    margin, keeping the edges whose midpoint lies in the tile (half-open, so
+    every edge has one owner).  Returns sorted global (i, j) pairs, i < j."""
+    from scipy.spatial import Delaunay
+    idx, P, x0, x1, y0, y1 = job
+    if len(idx) < 3:
+        return np.empty((0, 2), np.int64)
+    tri = Delaunay(P).simplices
+    e = np.concatenate([tri[:, [0, 1]], tri[:, [1, 2]], tri[:, [2, 0]]])
+    mx = 0.5 * (P[e[:, 0], 0] + P[e[:, 1], 0])
+    my = 0.5 * (P[e[:, 0], 1] + P[e[:, 1], 1])
+    own = (mx >= x0) & (mx < x1) & (my >= y0) & (my < y1)
+    g = idx[e[own]].astype(np.int64)
+    g.sort(axis=1)
+    return g
+
+
+def tiled_delaunay_edges(pts, box, tiles, margin, progress=None, workers=16):
+    """Delaunay edges of a large point set, triangulated tile by tile (tiles x
+    tiles squares of the box, each with `margin` metres of neighbouring points)
+    in a process pool.  Away from tile borders the triangulation is the global
+    one; an edge is taken from the tile that holds its midpoint."""
+    import concurrent.futures as cf
+    import multiprocessing as mp
+    w = box / tiles
+    tx = np.minimum((pts[:, 0] / w).astype(np.int64), tiles - 1)
+    ty = np.minimum((pts[:, 1] / w).astype(np.int64), tiles - 1)
+    jobs = []
+    for i in range(tiles):
+        for j in range(tiles):
+            x0, x1 = i * w, (i + 1) * w if i < tiles - 1 else np.inf
+            y0, y1 = j * w, (j + 1) * w if j < tiles - 1 else np.inf
+            lo = (np.abs(tx - i) <= 1) & (np.abs(ty - j) <= 1)
+            cand = np.nonzero(lo)[0]
+            P = pts[cand]
+            keep = ((P[:, 0] >= x0 - margin) & (P[:, 0] < x1 + margin) &
+                    (P[:, 1] >= y0 - margin) & (P[:, 1] < y1 + margin))
+            jobs.append((cand[keep], P[keep], x0 if i else -np.inf, x1, y0 if j else -np.inf, y1))
+    out = []
+    with cf.ProcessPoolExecutor(max_workers=workers, mp_context=mp.get_context("fork")) as ex:
+        for k, e in enumerate(ex.map(_tile_edges, jobs)):
+            out.append(e)
+            if progress and (k + 1) % tiles == 0:
+                progress(f"delaunay tiles {k + 1}/{len(jobs)}")
+    return np.concatenate(out)
+
+
 def road_graph(n_points, seed, box=60000.0, clusters=200, sigma=2000.0, out_degree=2.8,
                oneway=0.05, hmax=800, alpha=ALPHA, slope=SLOPE, neg_share=0.10,
-               name=None) -> RoadGraph:
+               name=None, tiles=0, progress=None) -> RoadGraph:
     """Road-like graph: a seeded clustered Poisson (Thomas) point process, its
     Delaunay triangulation thinned to a mean out-degree of ``out_degree`` (Euclidean
     MST plus random Delaunay edges, so the result stays connected), ``oneway`` of
-    the kept streets one-way, largest SCC kept, integer fractal heights."""
+    the kept streets one-way, largest SCC kept, integer fractal heights.  With
+    ``tiles`` > 0 the triangulation is built tile by tile in parallel (for tens
+    of millions of points); ``progress`` receives a line after each stage."""
+    say = progress or (lambda msg: None)
     from scipy.spatial import Delaunay
     from scipy.sparse import coo_matrix
     from scipy.sparse.csgraph import minimum_spanning_tree
@@ -210,13 +261,19 @@ def road_graph(n_points, seed, box=60000.0, clusters=200, sigma=2000.0, out_degr
     pts = pts[first]
     pts = pts[rng.permutation(pts.shape[0])]
     n0 = pts.shape[0]
-    tri = Delaunay(pts).simplices
-    e = np.concatenate([tri[:, [0, 1]], tri[:, [1, 2]], tri[:, [2, 0]]])
-    e.sort(axis=1)
+    say(f"points: {n0}")
+    if tiles:
+        e = tiled_delaunay_edges(pts, box, tiles, margin=2.0 * sigma, progress=say)
+    else:
+        tri = Delaunay(pts).simplices
+        e = np.concatenate([tri[:, [0, 1]], tri[:, [1, 2]], tri[:, [2, 0]]])
+        e.sort(axis=1)
     ke = np.unique(e[:, 0].astype(np.int64) * n0 + e[:, 1])   # distinct edges, lexicographic
-    e = np.stack([ke // n0, ke % n0], 1).astype(tri.dtype)
+    e = np.stack([ke // n0, ke % n0], 1).astype(np.int32)
+    say(f"delaunay edges: {len(e)}")
     ln = np.hypot(*(pts[e[:, 0]] - pts[e[:, 1]]).T)
     mst = minimum_spanning_tree(coo_matrix((ln + 1e-9, (e[:, 0], e[:, 1])), shape=(n0, n0))).tocoo()
+    say("spanning tree done")
     mst_e = np.stack([np.minimum(mst.row, mst.col), np.maximum(mst.row, mst.col)], 1)
     key_all = e[:, 0].astype(np.int64) * n0 + e[:, 1]
     key_mst = mst_e[:, 0].astype(np.int64) * n0 + mst_e[:, 1]
@@ -319,17 +376,17 @@ CONFIGS = {
     1: dict(kind="road", n_points=264_000, seed=1, hmax=800),
     2: dict(kind="road", n_points=321_000, seed=2, hmax=800, out_degree=2.5),
     3: dict(kind="road", n_points=1_100_000, seed=3, hmax=800, out_degree=2.45),
-    4: dict(kind="road", n_points=23_900_000, seed=4, hmax=3000, out_degree=2.44),
+    4: dict(kind="road", n_points=23_900_000, seed=4, hmax=3000, out_degree=2.44, tiles=8),
 }
 
 
-def make_config(idx: int) -> RoadGraph:
+def make_config(idx: int, progress=None) -> RoadGraph:
     c = dict(CONFIGS[idx])
     kind = c.pop("kind")
     if kind == "grid":
         return grid_graph(**c)
     n_points = c.pop("n_points")
-    return road_graph(n_points, name=f"c{idx}", **c)
+    return road_graph(n_points, name=f"c{idx}", progress=progress, **c)
 
 
 def random_pairs(n, count, seed):

@@ -168,3 +168,31 @@ def test_dropin_matches_reference_python_vectors(name):
     c2 = CCH(gn, order=rank, log=lambda *_: None)
     assert c2.customize(gn) == G[f"{name}/neg_witnesses"].tolist()
     assert c2.customize(gn, stop_on_negative_cycle=True) == G[f"{name}/neg_stop_witness"].tolist()
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_updates_queued_back_to_back(c0, mode):
+    """Twelve updates issued without any host read in between (more than the
+    staging slots) end in the oracle's state after the same twelve updates."""
+    g, rank = c0
+    print(f"case: config 0, 12 queued updates, mode={mode}", flush=True)
+    dt, st = _setup(g, rank)
+    w = dt.weights(g.w)
+    ud = dt.customize(w)
+    up, dn, _ = st.customize(g.w)
+    upd = O.Updater(st, g.w, up, dn)
+    rng = np.random.default_rng(40 + mode)
+    results = []
+    for rnd in range(12):
+        k = [1, 3, 40][rnd % 3]
+        idx = rng.choice(g.m, k, replace=False)
+        new = (upd.w[idx] + rng.integers(1, 300, k)).astype(np.int32)   # raises: no cycles
+        results.append(dt.update(w, ud, idx, new, mode=mode).clone())
+        ev0, _, _ = upd.update(idx, new)
+        results[-1] = (results[-1], ev0)
+    torch.cuda.synchronize()
+    for res, ev0 in results:
+        assert int(res[0].item()) == ev0
+    got = ud.cpu().numpy()
+    assert np.array_equal(got[:, 0], upd.up) and np.array_equal(got[:, 1], upd.dn)
+    assert dt.status() == 0
