@@ -87,6 +87,10 @@ def rlib():
         L.ref_negcycle.restype = i64
         L.ref_query.argtypes = [p, p, p, i64, p, p, p]
         L.ref_levels.argtypes = [p, p]
+        L.ref_load_dir.argtypes = [C.c_char_p, C.c_char_p]
+        L.ref_load_dir.restype = C.c_void_p
+        L.ref_graph_info.argtypes = [p, p]
+        L.ref_graph_export.argtypes = [p, p, p, p, p, C.c_int32, p]
         _r = L
     return _r
 
@@ -276,6 +280,31 @@ class Reference:
         R.ref_info(self.h, _p(info))
         self.mp, self.triangles, self.depth = int(info[0]), int(info[1]), int(info[2])
         self.m = int(off[-1])
+
+    @classmethod
+    def from_dir(cls, d, order_file):
+        """The reference's own reader (Graph::load, cpp/cch.cpp:72-95) on a graph
+        directory, with the order from a raw int32 file (cpp/cch.cpp:766-777)."""
+        R = rlib()
+        self = cls.__new__(cls)
+        self.h = R.ref_load_dir(str(d).encode(), str(order_file).encode())
+        info = np.zeros(4, np.int64)
+        R.ref_info(self.h, _p(info))
+        self.mp, self.triangles, self.depth, self.n = (int(v) for v in info)
+        g = np.zeros(3, np.int64)
+        R.ref_graph_info(self.h, _p(g))
+        self.m, self.n_metrics = int(g[1]), int(g[2])
+        return self
+
+    def graph(self, metric_index=-1):
+        """(off, tgt, x, y, w or None) as the reference read them."""
+        off = np.empty(self.n + 1, np.int64)
+        tgt = np.empty(self.m, np.int32)
+        x = np.empty(self.n, np.float64)
+        y = np.empty(self.n, np.float64)
+        w = np.empty(self.m, np.float64) if metric_index >= 0 else None
+        rlib().ref_graph_export(self.h, _p(off), _p(tgt), _p(x), _p(y), int(metric_index), _p(w))
+        return off, tgt, x, y, w
 
     def export(self):
         rank = np.empty(self.n, np.int32)

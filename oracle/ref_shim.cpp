@@ -52,6 +52,42 @@ void* ref_build(int64_t n, const int64_t* off, const int32_t* tgt, const double*
   return h;
 }
 
+// Load a graph directory with the reference's own reader (Graph::load,
+// cpp/cch.cpp:72-95: meta.txt, off.i64, tgt.i32, x.f64, y.f64, w_<metric>.f64)
+// and build the hierarchy with the order read from `order_file` (raw int32
+// rank[v], cpp/cch.cpp:766-777).  Pins the format writer of the library.
+void* ref_load_dir(const char* dir, const char* order_file) {
+  RefHandle* h = new RefHandle();
+  h->g.load(dir);
+  h->g.build_undirected();
+  h->c.rank = read_bin<i32>(order_file);
+  h->c.symbolic(h->g);
+  h->c.map_arcs(h->g);
+  return h;
+}
+
+// out3 = {n, m, number of metrics}
+void ref_graph_info(void* p, int64_t* out3) {
+  RefHandle* h = (RefHandle*)p;
+  out3[0] = h->g.n;
+  out3[1] = h->g.m;
+  out3[2] = (int64_t)h->g.metrics.size();
+}
+
+// The loaded arrays, and the weights of metric `idx` as the reference reads them.
+void ref_graph_export(void* p, int64_t* off, int32_t* tgt, double* x, double* y, int32_t idx, double* w) {
+  RefHandle* h = (RefHandle*)p;
+  const Graph& g = h->g;
+  std::memcpy(off, g.off.data(), g.off.size() * sizeof(int64_t));
+  std::memcpy(tgt, g.tgt.data(), g.tgt.size() * sizeof(int32_t));
+  std::memcpy(x, g.x.data(), g.x.size() * sizeof(double));
+  std::memcpy(y, g.y.data(), g.y.size() * sizeof(double));
+  if (w && idx >= 0 && idx < (int32_t)g.metrics.size()) {
+    const vector<double> wv = g.weights(g.metrics[idx]);
+    std::memcpy(w, wv.data(), wv.size() * sizeof(double));
+  }
+}
+
 void ref_destroy(void* p) { delete (RefHandle*)p; }
 
 // out4 = {mp, triangles, depth, n}
