@@ -434,7 +434,8 @@ def full_scale(args, out):
     os.environ["NWCCH_PROGRESS"] = "1"   # a line per big nested-dissection cell
     g, rank, topo, host = prepare(4, progress=lambda msg: log(f"config 4: {msg}"))
     dt = DeviceTopology(topo, 0)
-    log(f"device topology uploaded ({torch.cuda.memory_allocated() / 1e9:.1f} GB)")
+    free_b, total_b = torch.cuda.mem_get_info()
+    log(f"device topology uploaded ({(total_b - free_b) / 1e9:.1f} GB of HBM in use)")
     w = dt.weights(g.w)
     ud = torch.empty((topo.mp, 2), dtype=torch.int32, device=dt.dev)
     scan = torch.empty(2, dtype=torch.int64, device=dt.dev)
@@ -535,16 +536,27 @@ def full_scale(args, out):
             t_j = time.perf_counter() - t0
             nchk = args.full_check
             workers = min(8, os.cpu_count() or 1)
-            djs = [O.Dijkstra(g.n, g.off, g.tgt, g.w, p) for _ in range(workers)]
+            import queue
+            free = queue.Queue()   # one searcher (with its own scratch arrays) per running task
+            for _ in range(workers):
+                free.put(O.Dijkstra(g.n, g.off, g.tgt, g.w, p))
 
             def run(i):
-                return djs[i % workers](int(qs_h[i]), int(qt_h[i]))
+                dj = free.get()
+                try:
+                    return dj(int(qs_h[i]), int(qt_h[i]))
+                finally:
+                    free.put(dj)
             t0 = time.perf_counter()
             with cf.ThreadPoolExecutor(workers) as ex:
                 want = np.array(list(ex.map(run, range(nchk), chunksize=1)), np.int64)
             t_d = time.perf_counter() - t0
             want = np.where(want >= O.INF, O.INF, want)
             okq = bool(np.array_equal(d_gpu.astype(np.int64), want))
+            if not okq:
+                bad = np.nonzero(d_gpu.astype(np.int64) != want)[0]
+                log(f"config 4 query mismatches at {bad[:10].tolist()}: gpu {d_gpu[bad[:5]].tolist()} "
+                    f"dijkstra {want[bad[:5]].tolist()}")
             res["query_check"] = {"pairs": nchk, "agree_with_johnson_dijkstra": okq, "johnson_s": t_j,
                                   "dijkstra_s": t_d, "negative_cycle": neg}
             log(f"config 4 queries vs Johnson+Dijkstra on {nchk}: {okq} (Johnson {t_j:.1f}s, Dijkstra {t_d:.1f}s)")

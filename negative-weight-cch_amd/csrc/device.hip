@@ -454,6 +454,172 @@ __global__ void __launch_bounds__(64) k_query(const VInfo* __restrict__ vinfo,
   }
 }
 
+// The same query with the ancestor chains read 32 vertices at a time from
+// anc[x][0..31] (the 1st..32nd ancestors of x, -1 above the root): lane j of a
+// half holds chain position j of the current chunk with its VInfo, so no load
+// waits on the previous vertex's parent, and the rows of the next P chain
+// vertices are in flight while one row is relaxed (P slots of 64 arcs, P
+// dividing 32).  The next chunk's ids and VInfo are loaded during the current
+// chunk.  Results are identical to k_query.
+template <int P>
+__global__ void __launch_bounds__(64) k_query_chunk(const VInfo* __restrict__ vinfo,
+                                                    const int2* __restrict__ euler,
+                                                    const int32_t* __restrict__ adep,
+                                                    const int32_t* __restrict__ anc,
+                                                    const int2* __restrict__ ud_one, i64 ud_stride,
+                                                    const long long* __restrict__ ud_table,
+                                                    const int32_t* __restrict__ qmetric, int32_t n_metrics,
+                                                    const int32_t* __restrict__ qs,
+                                                    const int32_t* __restrict__ qt, i64 nq,
+                                                    const int32_t* __restrict__ order, int32_t n,
+                                                    int32_t maxlen, int32_t* __restrict__ dist,
+                                                    int32_t* __restrict__ scanned,
+                                                    uint32_t* status) {
+  static_assert(32 % P == 0, "P must divide 32");
+  extern __shared__ int32_t lds[];
+  const i64 q = order ? (i64)order[blockIdx.x] : (i64)blockIdx.x;
+  const int lane = threadIdx.x;
+  const int half = lane >> 5;
+  const int hl = lane & 31;
+  const int hb = half << 5;
+  if (q >= nq) return;
+  uint32_t st = 0;
+  const int32_t rs = qs[q], rt = qt[q];
+  const int32_t mq = ud_table ? qmetric[q] : 0;
+  const int2* __restrict__ ud =
+      ud_table ? ((mq >= 0 && mq < n_metrics) ? reinterpret_cast<const int2*>(ud_table[mq]) : nullptr) : ud_one;
+  if (rs < 0 || rs >= n || rt < 0 || rt >= n || !ud) {
+    if (lane == 0) {
+      dist[q] = kInf;
+      if (scanned) scanned[q] = 0;
+      atomicOr(status, (uint32_t)kStatusBadQuery);
+    }
+    return;
+  }
+  int32_t* lab = lds + half * maxlen;  // df on the s half, dr on the t half
+  const int32_t x0 = half ? rt : rs;
+  const VInfo v0 = vinfo[x0];
+  const int32_t D = v0.depth;
+  const int32_t Ds = __shfl(D, 0, 64), Dt = __shfl(D, 32, 64);
+  for (int32_t i = hl; i <= D; i += 32) lab[i] = kInf;
+  __syncthreads();
+  if (hl == 0) lab[0] = 0;
+  __syncthreads();
+  const int32_t tin_s = euler[rs].x;
+
+  // chunk 0: position 0 is the start vertex, positions 1..31 its ancestors
+  int32_t cid = hl == 0 ? x0 : anc[(i64)x0 * 32 + hl - 1];
+  VInfo cv = VInfo{0, 0, -1, 0};
+  if (cid >= 0) cv = hl == 0 ? v0 : vinfo[cid];
+  bool cin = false;
+  if (half && cid >= 0) {
+    const int2 e = euler[cid];
+    cin = e.x <= tin_s && tin_s <= e.y;
+  }
+  int32_t ra0[P], ra1[P];
+  int2 ru0[P], ru1[P];
+  auto load_row = [&](int slot, int32_t f, int32_t d) {
+    ra0[slot] = 0; ra1[slot] = 0;
+    ru0[slot] = make_int2(kInf, kInf); ru1[slot] = make_int2(kInf, kInf);
+    if (hl < d) { ra0[slot] = adep[f + hl]; ru0[slot] = ud[(i64)(f + hl) * ud_stride]; }
+    if (hl + 32 < d) { ra1[slot] = adep[f + hl + 32]; ru1[slot] = ud[(i64)(f + hl + 32) * ud_stride]; }
+  };
+#pragma unroll
+  for (int sl = 0; sl < P; sl++) {
+    const int32_t f = __shfl(cv.first, hb + sl, 64), d = __shfl(cv.deg, hb + sl, 64);
+    const int32_t v = __shfl(cid, hb + sl, 64);
+    load_row(sl, f, v >= 0 ? d : 0);
+  }
+  int32_t lca_depth = -1;
+  int32_t nscan = 0, p = 0;
+  bool active = true;  // this half's chain has vertices left
+  while (__any(active)) {
+    // next chunk: its ids now, its VInfo halfway through this chunk
+    const int32_t last = __shfl(cid, hb + 31, 64);
+    const int32_t nid = last >= 0 ? anc[(i64)last * 32 + hl] : -1;
+    VInfo nv = VInfo{0, 0, -1, 0};
+    bool nin = false;
+    // the t half's lowest common ancestor: the first chain vertex whose
+    // preorder interval holds s
+    {
+      const unsigned long long m = __ballot(cin) >> 32;
+      const int32_t dep = __shfl(cv.depth, 32 + (m ? __ffsll((long long)m) - 1 : 0), 64);
+      if (half && lca_depth < 0 && m) lca_depth = dep;
+    }
+#pragma unroll
+    for (int j = 0; j < 32; j++) {
+      const int slot = j % P;
+      if (j == 8) {
+        if (nid >= 0) {
+          nv = vinfo[nid];
+          if (half) {
+            const int2 e = euler[nid];
+            nin = e.x <= tin_s && tin_s <= e.y;
+          }
+        }
+      }
+      const int32_t v = __shfl(cid, hb + j, 64);
+      const int32_t first = __shfl(cv.first, hb + j, 64);
+      const int32_t deg = __shfl(cv.deg, hb + j, 64);
+      if (active && v < 0) active = false;
+      if (active) {
+        const int32_t dx = lab[p];
+        if (dx != kInf) {
+          nscan += deg;
+          if (hl < deg) {
+            const int32_t w = inf_add(dx, half ? ru0[slot].y : ru0[slot].x, st);
+            const int32_t pos = D - ra0[slot];
+            if (w < lab[pos]) lab[pos] = w;
+          }
+          if (hl + 32 < deg) {
+            const int32_t w = inf_add(dx, half ? ru1[slot].y : ru1[slot].x, st);
+            const int32_t pos = D - ra1[slot];
+            if (w < lab[pos]) lab[pos] = w;
+          }
+          for (int32_t k = hl + 64; k < deg; k += 32) {  // rows longer than 64 arcs
+            const int32_t a = first + k;
+            const int32_t pos = D - adep[a];
+            const int2 u = ud[(i64)a * ud_stride];
+            const int32_t w = inf_add(dx, half ? u.y : u.x, st);
+            if (w < lab[pos]) lab[pos] = w;
+          }
+        }
+        p++;
+      }
+      // refill this slot with the row P positions ahead
+      int32_t f2, d2, v2;
+      if (j + P < 32) {
+        v2 = __shfl(cid, hb + j + P, 64);
+        f2 = __shfl(cv.first, hb + j + P, 64);
+        d2 = __shfl(cv.deg, hb + j + P, 64);
+      } else {
+        v2 = __shfl(nid, hb + j + P - 32, 64);
+        f2 = __shfl(nv.first, hb + j + P - 32, 64);
+        d2 = __shfl(nv.deg, hb + j + P - 32, 64);
+      }
+      load_row(slot, f2, (active && v2 >= 0) ? d2 : 0);
+      __syncthreads();
+    }
+    cid = nid;
+    cv = nv;
+    cin = nin;
+  }
+  const int32_t lcad = __shfl(lca_depth, 32, 64);
+  int32_t best = kInf;
+  for (int32_t d = lane; d <= lcad; d += 64) {
+    const int32_t v = inf_add(lds[Ds - d], lds[maxlen + Dt - d], st);
+    best = min(best, v);
+  }
+  best = wave_min(best);
+  const int32_t total = __shfl(nscan, 0, 64) + __shfl(nscan, 32, 64);
+  st = wave_or(st);
+  if (lane == 0) {
+    dist[q] = best;
+    if (scanned) scanned[q] = total;
+    if (st) atomicOr(status, st);
+  }
+}
+
 }  // namespace nwcch
 
 // ================================================================ C ABI (device)
@@ -543,6 +709,23 @@ extern "C" int nwcch_dev_create(const nwcch_topo* topo, int device, nwcch_dev** 
       nxt[x] = tin + 1;
     }
     d.euler = dev_upload(d, eu.data(), eu.size());
+    // the 1st..32nd ancestors of every vertex (parents have higher ranks, so a
+    // vertex's list extends its parent's, computed first)
+    {
+      std::vector<int32_t> anc((size_t)t.n * 32);
+      for (i64 x = t.n - 1; x >= 0; x--) {
+        int32_t* a = &anc[(size_t)x * 32];
+        const int32_t pa = t.parent[x];
+        if (pa < 0) {
+          std::fill(a, a + 32, -1);
+          continue;
+        }
+        a[0] = pa;
+        const int32_t* ap = &anc[(size_t)pa * 32];
+        for (int i = 1; i < 32; i++) a[i] = ap[i - 1];
+      }
+      d.anc = dev_upload(d, anc.data(), anc.size());
+    }
     std::vector<int32_t> adep(t.mp);
     for (i64 a = 0; a < t.mp; a++) adep[a] = t.vdepth[t.head[a]];
     d.adep = dev_upload(d, adep.data(), adep.size());
@@ -666,7 +849,12 @@ int launch_queries(nwcch_dev* h, const int32_t* ud, int64_t ud_stride, const int
     const size_t lds = (size_t)2 * maxlen * sizeof(int32_t);
     if (lds > 160 * 1024) throw std::runtime_error("elimination tree too deep for the LDS query kernel");
     ProfScope ps(d, s, kKQuery);
-    if (nq <= d.query_deep_max)
+    if (d.query_chunk)
+      hipLaunchKernelGGL(k_query_chunk<4>, dim3((unsigned)nq), dim3(64), lds, s, d.vinfo, d.euler, d.adep, d.anc,
+                         reinterpret_cast<const int2*>(ud), (i64)ud_stride, (const long long*)ud_table, qmetric,
+                         n_metrics, s_rank, t_rank, (i64)nq, order, (int32_t)d.n, maxlen, dist, scanned,
+                         d.status);
+    else if (nq <= d.query_deep_max)
       hipLaunchKernelGGL(k_query<4>, dim3((unsigned)nq), dim3(64), lds, s, d.vinfo, d.euler, d.adep,
                          reinterpret_cast<const int2*>(ud), (i64)ud_stride, (const long long*)ud_table, qmetric,
                          n_metrics, s_rank, t_rank, (i64)nq, order, (int32_t)d.n, maxlen, dist, scanned,
@@ -718,6 +906,11 @@ extern "C" int nwcch_set_tuning(nwcch_dev* h, const char* key, int32_t value) {
     h->d.upd_persistent_max = value;
     return NWCCH_OK;
   }
+  if (k == "query_chunk") {
+    if (value < 0 || value > 1) { set_error("query_chunk is 0 or 1"); return NWCCH_ERR_ARG; }
+    h->d.query_chunk = value;
+    return NWCCH_OK;
+  }
   if (k == "query_deep_max") {
     if (value < 0) { set_error("query_deep_max must be >= 0"); return NWCCH_ERR_ARG; }
     h->d.query_deep_max = value;
@@ -748,6 +941,7 @@ extern "C" int nwcch_get_tuning(const nwcch_dev* h, const char* key, int32_t* va
   else if (k == "flat_min_tri") *value = h->d.flat_min_tri;
   else if (k == "flat_xcd") *value = h->d.flat_xcd;
   else if (k == "query_deep_max") *value = h->d.query_deep_max;
+  else if (k == "query_chunk") *value = h->d.query_chunk;
   else if (k == "upd_persistent_max") *value = h->d.upd_persistent_max;
   else { set_error("unknown tuning key: " + k); return NWCCH_ERR_ARG; }
   return NWCCH_OK;

@@ -66,6 +66,7 @@ struct DevTopo {
   int32_t batch_tri_per_lane = 8;            // batched level kernel
   int32_t upd_persistent_max = 2;            // updates of at most this many changed arcs run
                                              // in one persistent workgroup (mode auto)
+  int32_t query_chunk = 1;                   // queries read ancestor chains 32 at a time
   int32_t query_deep_max = 32768;            // query launches up to this size prefetch 4 x 32
                                              // arcs of the next row, larger ones 32
   int32_t flat_min_tri = 0;                  // levels whose heaviest arc has >= this many
@@ -80,6 +81,7 @@ struct DevTopo {
   VInfo* vinfo = nullptr;            // [n]
   int2* euler = nullptr;             // [n] preorder interval of the ET subtree
   int32_t* adep = nullptr;           // [mp] ET depth of the arc's head
+  int32_t* anc = nullptr;            // [n][32] 1st..32nd ancestors, -1 above the root
   // partial update (lazy)
   bool has_update = false;
   int32_t* arc_pos = nullptr;        // [mp]

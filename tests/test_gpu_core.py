@@ -91,11 +91,15 @@ def test_road_config1_bit_exact():
     d0, s0, _ = st.query(up, dn, s, t, threads=8)
     assert np.array_equal(dist.cpu().numpy(), d0)
     assert np.array_equal(sc.cpu().numpy().astype(np.int64), s0)
-    # the shallow-prefetch query kernel (large launches) gives the same answers
-    dt.set_tuning("query_deep_max", 0)
-    dist1, sc1 = dt.query(ud, rank[s], rank[t])
+    # the chain-walk query kernels (deep and shallow prefetch) give the same
+    # answers as the default chunked-ancestor kernel
+    dt.set_tuning("query_chunk", 0)
+    for deep in (32768, 0):
+        dt.set_tuning("query_deep_max", deep)
+        dist1, sc1 = dt.query(ud, rank[s], rank[t])
+        assert torch.equal(dist1, dist) and torch.equal(sc1, sc), deep
+    dt.set_tuning("query_chunk", 1)
     dt.set_tuning("query_deep_max", 32768)
-    assert torch.equal(dist1, dist) and torch.equal(sc1, sc)
     p, neg = O.johnson(g.n, g.off, g.tgt, g.w)
     assert not neg
     dj = O.Dijkstra(g.n, g.off, g.tgt, g.w, p)
