@@ -207,9 +207,14 @@ int nwcch_status(nwcch_dev* h, uint32_t* out, int clear, void* stream);
  *   "upd_persistent_max"  nwcch_update in mode 0 runs updates of at most this
  *                   many changed arcs in one persistent workgroup, larger ones
  *                   one launch per height; default 2.
- *   "query_deep_max"  query launches of at most this many queries prefetch 128
- *                   arcs of the next chain vertex's row (shorter latency),
- *                   larger launches 32 (more wavefronts per CU); default 32768. */
+ *   "query_deep_max"  query launches of at most this many queries use the
+ *                   latency-oriented kernels (deeper prefetch), larger launches
+ *                   the throughput-oriented one (more wavefronts per CU);
+ *                   default 32768.
+ *   "query_chunk"   1 (default): the latency-oriented launches read ancestor
+ *                   chains 32 vertices at a time and prefetch rows four chain
+ *                   vertices ahead; 0: they walk parent pointers, prefetching
+ *                   128 arcs of the next row. */
 int nwcch_set_tuning(nwcch_dev* h, const char* key, int32_t value);
 int nwcch_get_tuning(const nwcch_dev* h, const char* key, int32_t* value);
 

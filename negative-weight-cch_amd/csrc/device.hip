@@ -849,7 +849,7 @@ int launch_queries(nwcch_dev* h, const int32_t* ud, int64_t ud_stride, const int
     const size_t lds = (size_t)2 * maxlen * sizeof(int32_t);
     if (lds > 160 * 1024) throw std::runtime_error("elimination tree too deep for the LDS query kernel");
     ProfScope ps(d, s, kKQuery);
-    if (d.query_chunk)
+    if (d.query_chunk && nq <= d.query_deep_max)
       hipLaunchKernelGGL(k_query_chunk<4>, dim3((unsigned)nq), dim3(64), lds, s, d.vinfo, d.euler, d.adep, d.anc,
                          reinterpret_cast<const int2*>(ud), (i64)ud_stride, (const long long*)ud_table, qmetric,
                          n_metrics, s_rank, t_rank, (i64)nq, order, (int32_t)d.n, maxlen, dist, scanned,
