@@ -304,6 +304,41 @@ __global__ void k_scan_finish(const unsigned long long* src, long long* dst) {
 
 // ------------------------------------------------------------ ET query
 
+// Relax arcs [k0, deg) of a row (k0 = this lane's first index, stride 32) into
+// the labels, four arcs per lane in flight: the loads of a group are issued
+// before any of them is used (the LDS updates of one row touch distinct
+// positions, so their order is free).
+__device__ __forceinline__ void relax_tail(const int32_t* __restrict__ adep, const int2* __restrict__ ud,
+                                           i64 ud_stride, int32_t first, int32_t k0, int32_t deg, int32_t dx,
+                                           bool down, int32_t D, int32_t* lab, uint32_t& st) {
+  int32_t k = k0;
+  for (; k + 96 < deg; k += 128) {
+    int32_t pos[4];
+    int2 u[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int32_t a = first + k + 32 * i;
+      pos[i] = adep[a];
+      u[i] = ud[(i64)a * ud_stride];
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int32_t w = inf_add(dx, down ? u[i].y : u[i].x, st);
+      const int32_t ps = D - pos[i];
+      if (w < lab[ps]) lab[ps] = w;
+    }
+  }
+  for (; k < deg; k += 32) {
+    const int32_t a = first + k;
+    const int32_t ps = D - adep[a];
+    const int2 u = ud[(i64)a * ud_stride];
+    const int32_t w = inf_add(dx, down ? u.y : u.x, st);
+    if (w < lab[ps]) lab[ps] = w;
+  }
+}
+
+
+
 // One query per wavefront (python/cch.py:675-732, python/cch_nb.py:577-615,
 // cpp/cch.cpp:549-575).  The two searches are independent until they meet, so
 // they run at the same time on the two halves of the wavefront: lanes 0-31 walk
@@ -412,13 +447,8 @@ __global__ void __launch_bounds__(64) k_query(const VInfo* __restrict__ vinfo,
             if (v < lab[pos]) lab[pos] = v;
           }
         }
-        for (int32_t j = hl + 32 * kPre; j < vi.deg; j += 32) {  // rows longer than kPre * 32 arcs
-          const int32_t a = vi.first + j;
-          const int32_t pos = D - adep[a];
-          const int2 u = ud[(i64)a * ud_stride];
-          const int32_t v = inf_add(dx, half ? u.y : u.x, st);
-          if (v < lab[pos]) lab[pos] = v;
-        }
+        if (hl + 32 * kPre < vi.deg)  // rows longer than kPre * 32 arcs
+          relax_tail(adep, ud, ud_stride, vi.first, hl + 32 * kPre, vi.deg, dx, half, D, lab, st);
       }
     }
     __syncthreads();
@@ -576,13 +606,8 @@ __global__ void __launch_bounds__(64) k_query_chunk(const VInfo* __restrict__ vi
             const int32_t pos = D - ra1[slot];
             if (w < lab[pos]) lab[pos] = w;
           }
-          for (int32_t k = hl + 64; k < deg; k += 32) {  // rows longer than 64 arcs
-            const int32_t a = first + k;
-            const int32_t pos = D - adep[a];
-            const int2 u = ud[(i64)a * ud_stride];
-            const int32_t w = inf_add(dx, half ? u.y : u.x, st);
-            if (w < lab[pos]) lab[pos] = w;
-          }
+          if (hl + 64 < deg)  // rows longer than 64 arcs
+            relax_tail(adep, ud, ud_stride, first, hl + 64, deg, dx, half, D, lab, st);
         }
         p++;
       }
