@@ -305,14 +305,16 @@ __global__ void k_scan_finish(const unsigned long long* src, long long* dst) {
 // ------------------------------------------------------------ ET query
 
 // Relax arcs [k0, deg) of a row (k0 = this lane's first index, stride 32) into
-// the labels, four arcs per lane in flight: the loads of a group are issued
-// before any of them is used (the LDS updates of one row touch distinct
-// positions, so their order is free).
+// the labels.  With kGroups, four arcs per lane are in flight: the loads of a
+// group are issued before any of them is used (the LDS updates of one row
+// touch distinct positions, so their order is free); without, the kernel keeps
+// fewer registers for more wavefronts per CU.
+template <bool kGroups>
 __device__ __forceinline__ void relax_tail(const int32_t* __restrict__ adep, const int2* __restrict__ ud,
                                            i64 ud_stride, int32_t first, int32_t k0, int32_t deg, int32_t dx,
                                            bool down, int32_t D, int32_t* lab, uint32_t& st) {
   int32_t k = k0;
-  for (; k + 96 < deg; k += 128) {
+  for (; kGroups && k + 96 < deg; k += 128) {
     int32_t pos[4];
     int2 u[4];
 #pragma unroll
@@ -448,7 +450,7 @@ __global__ void __launch_bounds__(64) k_query(const VInfo* __restrict__ vinfo,
           }
         }
         if (hl + 32 * kPre < vi.deg)  // rows longer than kPre * 32 arcs
-          relax_tail(adep, ud, ud_stride, vi.first, hl + 32 * kPre, vi.deg, dx, half, D, lab, st);
+          relax_tail<(kPre > 1)>(adep, ud, ud_stride, vi.first, hl + 32 * kPre, vi.deg, dx, half, D, lab, st);
       }
     }
     __syncthreads();
@@ -607,7 +609,7 @@ __global__ void __launch_bounds__(64) k_query_chunk(const VInfo* __restrict__ vi
             if (w < lab[pos]) lab[pos] = w;
           }
           if (hl + 64 < deg)  // rows longer than 64 arcs
-            relax_tail(adep, ud, ud_stride, first, hl + 64, deg, dx, half, D, lab, st);
+            relax_tail<true>(adep, ud, ud_stride, first, hl + 64, deg, dx, half, D, lab, st);
         }
         p++;
       }
