@@ -475,8 +475,29 @@ def full_scale(args, out):
     qt = torch.as_tensor(rank[qt_h]).to(dt.dev)
     dist = torch.empty(nq, dtype=torch.int32, device=dt.dev)
     order = dt.chain_order(qs, qt)
+    # the query kernels on this very deep hierarchy: the throughput kernel for
+    # the big launch, then each latency kernel for the big launch and alone
+    forms = {}
+    for name, chunk, deep in (("chain_walk_throughput", 0, 0), ("chunked_latency", 1, 1 << 30),
+                              ("chain_walk_latency", 0, 1 << 30)):
+        dt.set_tuning("query_chunk", chunk)
+        dt.set_tuning("query_deep_max", deep)
+        dt.query(ud, qs, qt, want_scanned=False, dist=dist, order=order)
+        bms_q = statistics.median(event_ms(lambda: dt.query(ud, qs, qt, want_scanned=False, dist=dist, order=order),
+                                           2))
+        one = [event_ms(lambda i=i: dt.query(ud, qs[i:i + 1], qt[i:i + 1], want_scanned=False), 1)[0]
+               for i in range(30)]
+        forms[name] = {"batch_ms": bms_q, "queries_per_s": nq / (bms_q / 1e3),
+                       "single_p50_ms": float(np.percentile(one, 50))}
+        log(f"config 4 queries, {name}: {nq} in {bms_q:.1f} ms; single p50 {np.percentile(one, 50):.2f} ms")
+    best_b = min(forms, key=lambda k: forms[k]["batch_ms"])
+    best_s = min(forms, key=lambda k: forms[k]["single_p50_ms"])
+    dt.set_tuning("query_chunk", 1 if best_b == "chunked_latency" else 0)
+    dt.set_tuning("query_deep_max", 0 if best_b == "chain_walk_throughput" else 1 << 30)
     dt.query(ud, qs, qt, want_scanned=False, dist=dist, order=order)
-    qms = statistics.median(event_ms(lambda: dt.query(ud, qs, qt, want_scanned=False, dist=dist, order=order), 3))
+    qms = forms[best_b]["batch_ms"]
+    dt.set_tuning("query_chunk", 1 if best_s == "chunked_latency" else 0)
+    dt.set_tuning("query_deep_max", 0 if best_s == "chain_walk_throughput" else 1 << 30)
     single = [event_ms(lambda i=i: dt.query(ud, qs[i:i + 1], qt[i:i + 1], want_scanned=False), 1)[0]
               for i in range(50)]
     dt.check_status()
@@ -493,9 +514,9 @@ def full_scale(args, out):
         "scan_ms": statistics.median(scan_ms),
         "batched": {"metrics": K, "ms": bms, "triangles_per_s": K * topo.triangles / (bms / 1e3),
                     "flagged_metrics": flagged},
-        "queries": {"count": nq, "ms": qms, "queries_per_s": nq / (qms / 1e3),
-                    "single_p50_ms": float(np.percentile(single, 50)),
-                    "single_p99_ms": float(np.percentile(single, 99))},
+        "queries": {"count": nq, "ms": qms, "queries_per_s": nq / (qms / 1e3), "batch_kernel": best_b,
+                    "single_kernel": best_s, "single_p50_ms": float(np.percentile(single, 50)),
+                    "single_p99_ms": float(np.percentile(single, 99)), "forms": forms},
     }
     if not args.no_check or not args.no_cpu:
         from oracle import oracle as O
