@@ -161,6 +161,15 @@ int nwcch_customize_batch(nwcch_dev* h, const int32_t* W, int32_t K, int32_t sla
 int nwcch_negcycle_scan_batch(const int32_t* UD, int64_t mp, int32_t K, int32_t slab, int64_t* out,
                               void* stream);
 
+/* CCH potential of a customized metric: p (device int32[n], indexed by rank)
+ * = min(0, min over u of dist(u, v)) for the vertex of each rank, the
+ * virtual-source distances Johnson's algorithm computes, in two level-by-level
+ * sweeps over the hierarchy.  Replaces cch_potential (cpp/cch.cpp:635-655) and
+ * CCH.potential (python/cch.py:734-755, which returns it by original id).
+ * Valid for metrics without negative cycles; a value below -(2^31-1) raises
+ * NWCCH_STATUS_OVERFLOW. */
+int nwcch_potential(nwcch_dev* h, const int32_t* ud, int32_t* p, void* stream);
+
 /* Build (once, on the host) and upload the partial-update index of the topology
  * the handle was created from: for every arc the top arcs of the lower
  * triangles it is a side of, arc heights and frontier buckets.  The lazy

@@ -302,6 +302,41 @@ __global__ void k_scan_finish(const unsigned long long* src, long long* dst) {
   dst[1] = src[1] == ~0ull ? -1ll : (long long)src[1];
 }
 
+// ------------------------------------------------------------ CCH potential
+
+// p(v) = min(0, min_u dist(u, v)), the virtual-source distances of Johnson's
+// algorithm, read off a customized hierarchy in two sweeps (cpp/cch.cpp:635-655,
+// python/cch.py:734-755): upward in rank order, then downward.  Both run level
+// by level, one lane per arc of the level (the arcs whose tail has that ET
+// height).  Upward, a tail's label is final once every lower level has pushed
+// into it, and the level pushes only into higher heads; downward, a tail pulls
+// from heads of higher levels, already final.  Labels are indexed by rank and
+// formed in 64 bits; a value below -(2^31-1) raises the overflow bit.
+__global__ void __launch_bounds__(256) k_pot_init(int32_t* p, i64 n) {
+  const i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+__global__ void __launch_bounds__(256) k_pot_level(const int32_t* __restrict__ sched_arc,
+                                                   const int32_t* __restrict__ tail,
+                                                   const int32_t* __restrict__ head, const int2* __restrict__ ud,
+                                                   int32_t* p, i64 begin, i64 end, int down, uint32_t* status) {
+  const i64 q = begin + (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= end) return;
+  const int32_t a = sched_arc[q];
+  const int32_t x = tail[a], y = head[a];
+  const int2 u = ud[a];
+  uint32_t st = 0;
+  if (!down) {  // p[y] = min(p[y], p[x] + up[a])
+    const int32_t v = inf_add(p[x], u.x, st);
+    if (v < p[y]) atomicMin(&p[y], v);
+  } else {      // p[x] = min(p[x], p[y] + down[a])
+    const int32_t v = inf_add(p[y], u.y, st);
+    if (v < p[x]) atomicMin(&p[x], v);
+  }
+  if (st) atomicOr(status, st);
+}
+
 // ------------------------------------------------------------ ET query
 
 // Relax arcs [k0, deg) of a row (k0 = this lane's first index, stride 32) into
@@ -756,6 +791,8 @@ extern "C" int nwcch_dev_create(const nwcch_topo* topo, int device, nwcch_dev** 
     std::vector<int32_t> adep(t.mp);
     for (i64 a = 0; a < t.mp; a++) adep[a] = t.vdepth[t.head[a]];
     d.adep = dev_upload(d, adep.data(), adep.size());
+    d.head = dev_upload(d, t.head.data(), t.head.size());
+    d.tail = dev_upload(d, t.arc_tail.data(), t.arc_tail.size());
     d.status = dev_alloc<uint32_t>(d, 1);
     HIP_OK(hipMemset(d.status, 0, sizeof(uint32_t)));
     d.scan = dev_alloc<unsigned long long>(d, 4);
@@ -923,6 +960,30 @@ extern "C" int nwcch_query_multi(nwcch_dev* h, const int64_t* ud_table, int32_t 
 extern "C" int nwcch_query(nwcch_dev* h, const int32_t* ud, const int32_t* s_rank, const int32_t* t_rank,
                            int64_t nq, int32_t* dist, int32_t* scanned, void* stream) {
   return nwcch_query_strided(h, ud, 1, s_rank, t_rank, nq, nullptr, dist, scanned, stream);
+}
+
+extern "C" int nwcch_potential(nwcch_dev* h, const int32_t* ud, int32_t* p, void* stream) {
+  if (!h || !ud || !p) { set_error("null argument"); return NWCCH_ERR_ARG; }
+  try {
+    DevTopo& d = h->d;
+    hipStream_t s = (hipStream_t)stream;
+    const int2* u = reinterpret_cast<const int2*>(ud);
+    hipLaunchKernelGGL(k_pot_init, dim3((unsigned)std::max<i64>((d.n + 255) / 256, 1)), dim3(256), 0, s, p, d.n);
+    const int32_t L = (int32_t)d.lv.size();
+    for (int pass = 0; pass < 2; pass++) {
+      for (int32_t i = 0; i < L; i++) {
+        const LevelRange& r = d.lv[pass == 0 ? i : L - 1 - i];
+        if (r.end == r.begin) continue;
+        ProfScope ps(d, s, kKOther);
+        hipLaunchKernelGGL(k_pot_level, dim3((unsigned)((r.end - r.begin + 255) / 256)), dim3(256), 0, s,
+                           d.sched_arc, d.tail, d.head, u, p, r.begin, r.end, pass, d.status);
+      }
+    }
+    HIP_OK(hipGetLastError());
+  } catch (const std::exception& e) {
+    return fail(e);
+  }
+  return NWCCH_OK;
 }
 
 extern "C" int nwcch_set_tuning(nwcch_dev* h, const char* key, int32_t value) {

@@ -82,6 +82,9 @@ struct DevTopo {
   int2* euler = nullptr;             // [n] preorder interval of the ET subtree
   int32_t* adep = nullptr;           // [mp] ET depth of the arc's head
   int32_t* anc = nullptr;            // [n][32] 1st..32nd ancestors, -1 above the root
+  // potential sweeps
+  int32_t* head = nullptr;           // [mp] upper end of each arc
+  int32_t* tail = nullptr;           // [mp] lower end of each arc
   // partial update (lazy)
   bool has_update = false;
   int32_t* arc_pos = nullptr;        // [mp]

@@ -198,6 +198,13 @@ class CCH:
             return True
         return self._last is not None and self._last[2] > 0
 
+    def potential(self):
+        """A feasible potential of the customized metric, by original vertex id:
+        p(v) = min(0, min_u dist(u, v)) (python/cch.py:734-755); ints."""
+        pr = self.dev.potential(self.ud).cpu().numpy()
+        self.dev.check_status()
+        return pr[np.asarray(self.rank)].tolist()
+
     # ---------------------------------------------------------------- queries
     def query(self, s, t):
         """(distance, arcs scanned) between original vertices s and t

@@ -112,6 +112,14 @@ class DeviceTopology:
         check(lib().nwcch_negcycle_scan(ptr(ud), ud.shape[0], ptr(out), _stream(stream)), "nwcch_negcycle_scan")
         return out
 
+    def potential(self, ud: torch.Tensor, out: torch.Tensor | None = None, stream=None) -> torch.Tensor:
+        """CCH potential indexed by rank: device int32 [n] (cpp/cch.cpp:635-655)."""
+        assert ud.dtype == torch.int32 and ud.shape == (self.mp, 2) and ud.is_contiguous()
+        if out is None:
+            out = torch.empty(self.n, dtype=torch.int32, device=self.dev)
+        check(lib().nwcch_potential(self._h, ptr(ud), ptr(out), _stream(stream)), "nwcch_potential")
+        return out
+
     def negcycle_list(self, ud: torch.Tensor, stream=None) -> list[int]:
         """All witness arcs, ascending (python/cch.py:287)."""
         cnt, _ = (int(v) for v in self.negcycle_scan(ud, stream=stream).cpu())

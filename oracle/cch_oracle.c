@@ -28,6 +28,9 @@
  *                     pinned by: Bellman-Ford, Johnson+Dijkstra, the KATs of
  *                     python/test_techniques.py:171-178, the exhaustive 3-vertex
  *                     family (363-385), oracle/_ref Query::run, scan bound
+ *   orc_query_path    python/cch.py:406-507 (query_path, _unpack)
+ *                     pinned by: golden vectors of the reference's own
+ *                     query_path (tests/golden), path validity and length
  *   orc_update_*      python/cch.py:289-402, 666-671
  *                     pinned by: full recomputation (Theorem 4), and the closed
  *                     forms last_changed = {arcs whose full value changed} and
@@ -706,4 +709,106 @@ i64 orc_customize_stop(i64 n, const i64* first, const i32* head, i64 m, const i6
     }
   }
   return -1;
+}
+
+/* ---------------------------------------------------------- path unpacking */
+
+/* Shortest path with its vertices (python/cch.py:406-507, query_path, _unpack,
+ * _lower_triangle_vertices), ranks in and out.  lfirst/lnbr: the lower
+ * neighbours of every vertex, ascending (the x of the lower triangles of an
+ * arc (y, z) in the order of the reference's by_c groups).  tail[a]: lower end
+ * of arc a.  Writes up to cap ranks to path; *len = path length (-1 if
+ * unreachable).  Returns the distance (INF32 if unreachable). */
+i64 orc_query_path(i64 n, const i64* first, const i32* head, const i32* parent, const i32* tail,
+                   const i64* lfirst, const i32* lnbr, const i32* up, const i32* dn, i32 rs, i32 rt,
+                   i32* path, i64 cap, i64* len) {
+  const i64 INF64 = INF32;
+  i64* df = (i64*)malloc((size_t)n * sizeof(i64));
+  i64* dr = (i64*)malloc((size_t)n * sizeof(i64));
+  i64* pf = (i64*)malloc((size_t)n * sizeof(i64));
+  i64* pr = (i64*)malloc((size_t)n * sizeof(i64));
+  char* anc = (char*)calloc((size_t)n, 1);
+  for (i64 v = 0; v < n; v++) { df[v] = dr[v] = INF64; pf[v] = pr[v] = -1; }
+  df[rs] = 0;
+  dr[rt] = 0;
+  for (i64 x = rs; x >= 0; x = parent[x]) {
+    if (df[x] >= INF64) continue;
+    for (i64 a = first[x]; a < first[x + 1]; a++) {
+      if (up[a] == INF32) continue;
+      i64 v = df[x] + up[a];
+      if (v < df[head[a]]) { df[head[a]] = v; pf[head[a]] = a; }
+    }
+  }
+  for (i64 x = rt; x >= 0; x = parent[x]) {
+    if (dr[x] >= INF64) continue;
+    for (i64 a = first[x]; a < first[x + 1]; a++) {
+      if (dn[a] == INF32) continue;
+      i64 v = dr[x] + dn[a];
+      if (v < dr[head[a]]) { dr[head[a]] = v; pr[head[a]] = a; }
+    }
+  }
+  for (i64 x = rt; x >= 0; x = parent[x]) anc[x] = 1;
+  i64 best = INF64, peak = -1;
+  for (i64 x = rs; x >= 0; x = parent[x])
+    if (anc[x] && df[x] < INF64 && dr[x] < INF64 && df[x] + dr[x] < best) { best = df[x] + dr[x]; peak = x; }
+  *len = -1;
+  if (peak >= 0) {
+    /* legs as (from, to) rank pairs: up from rs to the peak, then down to rt */
+    i64 nl = 0, cl = 64;
+    i32* legs = (i32*)malloc((size_t)cl * 2 * sizeof(i32));
+    for (i64 x = peak; x != rs;) {
+      i32 v = tail[pf[x]];
+      if (nl == cl) { cl *= 2; legs = (i32*)realloc(legs, (size_t)cl * 2 * sizeof(i32)); }
+      legs[2 * nl] = v; legs[2 * nl + 1] = (i32)x; nl++;
+      x = v;
+    }
+    for (i64 i = 0, j = nl - 1; i < j; i++, j--) {  /* reverse */
+      i32 a0 = legs[2 * i], a1 = legs[2 * i + 1];
+      legs[2 * i] = legs[2 * j]; legs[2 * i + 1] = legs[2 * j + 1];
+      legs[2 * j] = a0; legs[2 * j + 1] = a1;
+    }
+    for (i64 x = peak; x != rt;) {
+      i32 v = tail[pr[x]];
+      if (nl == cl) { cl *= 2; legs = (i32*)realloc(legs, (size_t)cl * 2 * sizeof(i32)); }
+      legs[2 * nl] = (i32)x; legs[2 * nl + 1] = v; nl++;
+      x = v;
+    }
+    i64 L = 0;
+    if (L < cap) path[L] = nl ? legs[0] : rs;
+    L++;
+    i64 sc = 64, sp = 0;
+    i32* st = (i32*)malloc((size_t)sc * 2 * sizeof(i32));
+    for (i64 l = 0; l < nl; l++) {
+      st[0] = legs[2 * l]; st[1] = legs[2 * l + 1]; sp = 1;
+      while (sp > 0) {
+        sp--;
+        const i32 v = st[2 * sp], w = st[2 * sp + 1];
+        const i32 lo = v < w ? v : w, hi = v < w ? w : v;
+        const i64 a = find_arc(first, head, lo, hi);
+        const i64 cost = v < w ? up[a] : dn[a];
+        i32 mid = -1;
+        /* lower triangles of arc (lo, hi): x < lo adjacent to both, ascending */
+        for (i64 k = lfirst[lo]; k < lfirst[lo + 1] && mid < 0; k++) {
+          const i32 u = lnbr[k];
+          const i64 av = find_arc(first, head, u, v), aw = find_arc(first, head, u, w);
+          if (av < 0 || aw < 0) continue;
+          const i64 c1 = dn[av], c2 = up[aw];
+          if (c1 != INF32 && c2 != INF32 && cost != INF32 && c1 + c2 == cost) mid = u;
+        }
+        if (mid < 0) {
+          if (L < cap) path[L] = w;
+          L++;
+        } else {
+          if (sp + 2 > sc) { sc *= 2; st = (i32*)realloc(st, (size_t)sc * 2 * sizeof(i32)); }
+          st[2 * sp] = mid; st[2 * sp + 1] = w; sp++;   /* popped second */
+          st[2 * sp] = v; st[2 * sp + 1] = mid; sp++;   /* popped first */
+        }
+      }
+    }
+    free(st);
+    free(legs);
+    *len = L;
+  }
+  free(df); free(dr); free(pf); free(pr); free(anc);
+  return peak >= 0 ? best : INF32;
 }

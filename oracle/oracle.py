@@ -58,6 +58,8 @@ def olib():
         L.orc_bellman_ford.restype = C.c_int
         L.orc_has_negative_cycle.argtypes = [i64, p, p, p]
         L.orc_has_negative_cycle.restype = C.c_int
+        L.orc_query_path.argtypes = [i64, p, p, p, p, p, p, p, p, i32, i32, p, i64, p]
+        L.orc_query_path.restype = i64
         L.orc_johnson.argtypes = [i64, p, p, p, p]
         L.orc_johnson.restype = C.c_int
         L.orc_dijkstra_pot.argtypes = [i64, p, p, p, p, i32, i32, p, p, p, p]
@@ -91,6 +93,7 @@ def rlib():
         L.ref_load_dir.restype = C.c_void_p
         L.ref_graph_info.argtypes = [p, p]
         L.ref_graph_export.argtypes = [p, p, p, p, p, C.c_int32, p]
+        L.ref_potential.argtypes = [p, p, p, p]
         _r = L
     return _r
 
@@ -179,6 +182,27 @@ class Structure:
     def query(self, up, dn, s, t, threads=1):
         """Original vertex ids."""
         return self.query_ranks(up, dn, self.rank[np.asarray(s)], self.rank[np.asarray(t)], threads)
+
+    def query_path(self, up, dn, s, t):
+        """(dist, [original vertex ids]) of python/cch.py:406-507; INF, [] if unreachable."""
+        if not hasattr(self, "_lower"):
+            tail = np.repeat(np.arange(self.n, dtype=np.int32), np.diff(self.first)).astype(np.int32)
+            order = np.argsort(self.head, kind="stable")
+            lnbr = np.ascontiguousarray(tail[order], np.int32)
+            lfirst = np.zeros(self.n + 1, np.int64)
+            np.add.at(lfirst, self.head.astype(np.int64) + 1, 1)
+            self._lower = (tail, np.cumsum(lfirst), lnbr)
+            self._inv = np.argsort(self.rank).astype(np.int32)
+        tail, lfirst, lnbr = self._lower
+        cap = 4 * self.n + 16
+        path = np.empty(cap, np.int32)
+        ln = np.zeros(1, np.int64)
+        d = olib().orc_query_path(self.n, _p(self.first), _p(self.head), _p(self.parent), _p(tail), _p(lfirst),
+                                  _p(lnbr), _p(np.ascontiguousarray(up, np.int32)),
+                                  _p(np.ascontiguousarray(dn, np.int32)), int(self.rank[s]), int(self.rank[t]),
+                                  _p(path), cap, _p(ln))
+        L = int(ln[0])
+        return int(d), ([] if L < 0 else self._inv[path[:L]].tolist())
 
 
 class Updater:
@@ -324,6 +348,13 @@ class Reference:
 
     def negcycle(self, up, dn):
         return int(rlib().ref_negcycle(self.h, _p(up), _p(dn)))
+
+    def potential(self, up, dn):
+        """cch_potential (cpp/cch.cpp:635-655), by original vertex id."""
+        out = np.empty(self.n, np.float64)
+        rlib().ref_potential(self.h, _p(np.ascontiguousarray(up, np.float64)),
+                             _p(np.ascontiguousarray(dn, np.float64)), _p(out))
+        return out
 
     def query(self, up, dn, s, t):
         s = np.ascontiguousarray(s, np.int32)

@@ -88,6 +88,16 @@ void ref_graph_export(void* p, int64_t* off, int32_t* tgt, double* x, double* y,
   }
 }
 
+// The reference's CCH potential (cpp/cch.cpp:635-655) of a customization given
+// as float64 up/down values; out[v] by original vertex id.
+void ref_potential(void* p, const double* up, const double* dn, double* out) {
+  RefHandle* h = (RefHandle*)p;
+  const size_t mp = h->c.head.size();
+  vector<double> u(up, up + mp), d(dn, dn + mp);
+  const vector<double> pv = cch_potential(h->c, u, d);
+  std::memcpy(out, pv.data(), pv.size() * sizeof(double));
+}
+
 void ref_destroy(void* p) { delete (RefHandle*)p; }
 
 // out4 = {mp, triangles, depth, n}

@@ -8,7 +8,9 @@ for the outputs that no closed form or other reference test fixes:
   and ``last_changed`` (the changed arcs in pop order);
 * ``CCH.customize(g, stop_on_negative_cycle=True)`` (python/cch.py:264-278):
   the witness arc;
-* ``CCH.customize(g)``: the full witness list (python/cch.py:287).
+* ``CCH.customize(g)``: the full witness list (python/cch.py:287);
+* ``CCH.query_path`` (python/cch.py:406-507): the vertex sequence of the
+  unpacked shortest path, which depends on the order of ties.
 
 Only data is written: ``cch_py_vectors.npz`` with the inputs (graph CSR, order,
 weights, update batches) and the reference's outputs.  The graphs come from
@@ -71,6 +73,19 @@ def main():
         out[f"{name}/off"], out[f"{name}/tgt"] = g.off, g.tgt
         out[f"{name}/w"], out[f"{name}/rank"] = g.w, rank
         out[f"{name}/up0"], out[f"{name}/dn0"] = as_i32(c.up), as_i32(c.down)
+        # unpacked shortest paths on the customized metric
+        qr = np.random.default_rng(23)
+        qs_, qt_ = qr.integers(0, g.n, 30), qr.integers(0, g.n, 30)
+        pd, po, pv = [], [0], []
+        for a, b in zip(qs_.tolist(), qt_.tolist()):
+            dd, path = c.query_path(a, b)
+            pd.append(INF32 if dd == float("inf") else int(dd))
+            pv.extend(path)
+            po.append(len(pv))
+        out[f"{name}/path_s"], out[f"{name}/path_t"] = qs_.astype(np.int32), qt_.astype(np.int32)
+        out[f"{name}/path_dist"] = np.array(pd, np.int64)
+        out[f"{name}/path_off"] = np.array(po, np.int64)
+        out[f"{name}/path_v"] = np.array(pv, np.int32)
         rng = np.random.default_rng(17)
         w = g.w.astype(np.int64).copy()
         for u, k in enumerate([1, 3, 10, 40, 100, 10]):

@@ -57,3 +57,17 @@ def test_witnesses_equal_reference_python(gold, name):
     neg = np.nonzero((up != O.INF) & (dn != O.INF) & (up.astype(np.int64) + dn < 0))[0]
     assert neg.tolist() == gold[f"{name}/neg_witnesses"].tolist()
     assert [st.customize_stop(wn)] == gold[f"{name}/neg_stop_witness"].tolist()
+
+
+@pytest.mark.parametrize("name", cases())
+def test_query_path_equals_reference_python(gold, name):
+    """Unpacked shortest paths (python/cch.py:406-507): the same distance and the
+    same vertex sequence, ties resolved as the reference resolves them."""
+    off, tgt, w, rank = (gold[f"{name}/{k}"] for k in ("off", "tgt", "w", "rank"))
+    st = O.Structure(len(rank), off, tgt, rank)
+    up, dn, _ = st.customize(w)
+    po, pv = gold[f"{name}/path_off"], gold[f"{name}/path_v"]
+    for i, (s, t) in enumerate(zip(gold[f"{name}/path_s"], gold[f"{name}/path_t"])):
+        d, path = st.query_path(up, dn, int(s), int(t))
+        assert d == int(gold[f"{name}/path_dist"][i]), (name, i)
+        assert path == pv[po[i]:po[i + 1]].tolist(), (name, i)

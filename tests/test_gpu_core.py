@@ -187,3 +187,28 @@ def test_cch_nb_dropin(c0):
     assert cch_nb.negative_cycle_arcs(upw.clone(), dnw.clone()) == 0
     dist, _ = cch_nb.query_many(rank[s], rank[t], first, head, upw, dnw)
     assert np.array_equal(dist.cpu().numpy(), d0)
+
+
+@pytest.mark.parametrize("which", ["c0", "c1", "small"])
+def test_potential_equals_johnson(c0, which):
+    """GPU CCH potential (two level sweeps) equals the oracle's Johnson
+    virtual-source distances, bit for bit, and is feasible."""
+    if which == "c0":
+        g, rank = c0
+    elif which == "c1":
+        g = graphs.make_config(1)
+        rank = _lib.order(g.n, g.off, g.tgt, g.x, g.y, 64)
+    else:
+        g = small_graphs()[5]
+        rank = order_of(g)
+    print("case: potential", which, g.n, flush=True)
+    dt, st = _setup(g, rank)
+    ud = dt.customize(g.w)
+    pr = dt.potential(ud).cpu().numpy()
+    assert dt.status() == 0
+    p = pr[rank].astype(np.int64)          # by original vertex id
+    want, neg = O.johnson(g.n, g.off, g.tgt, g.w)
+    assert not neg
+    assert np.array_equal(p, want)
+    tails = np.repeat(np.arange(g.n), np.diff(g.off))
+    assert (g.w.astype(np.int64) + p[tails] - p[g.tgt] >= 0).all()

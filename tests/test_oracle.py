@@ -333,3 +333,24 @@ def test_negative_cycle_count_equals_reference_cpp():
     up, dn, _ = st.customize(w)
     ru, rd = ref.customize(w)
     assert st.negcycle(up, dn)[0] == ref.negcycle(ru, rd) > 0
+
+
+@pytest.mark.skipif(not O.have_ref(), reason="the reference's C++ was not built")
+def test_johnson_potential_equals_reference_cch_potential():
+    """The virtual-source distances the oracle's Johnson computes equal the
+    reference's CCH potential read off its own customization
+    (cpp/cch.cpp:635-655), on the small family and the 64x64 grid."""
+    from conftest import order_of, small_graphs
+    from nwcch import graphs
+    cases = small_graphs()[::3] + [graphs.make_config(0)]
+    for g in cases:
+        rank = order_of(g)
+        ref = O.Reference(g.n, g.off, g.tgt, rank)
+        up, dn = ref.customize(g.w)
+        p_ref = ref.potential(up, dn)
+        p, neg = O.johnson(g.n, g.off, g.tgt, g.w)
+        assert not neg
+        assert np.array_equal(p_ref.astype(np.int64), p), g.name
+        # feasibility: reduced weights are non-negative
+        tails = np.repeat(np.arange(g.n), np.diff(g.off))
+        assert (g.w.astype(np.int64) + p[tails] - p[g.tgt] >= 0).all()
