@@ -161,6 +161,16 @@ int nwcch_customize_batch(nwcch_dev* h, const int32_t* W, int32_t K, int32_t sla
 int nwcch_negcycle_scan_batch(const int32_t* UD, int64_t mp, int32_t K, int32_t slab, int64_t* out,
                               void* stream);
 
+/* Shortest paths with their vertices: for query i, dist[i] as nwcch_query and
+ * the ranks of the unpacked path from s_rank[i] to t_rank[i] in
+ * path[i * max_len ...] (device int32 [nq][max_len]); path_len[i] = the number
+ * of vertices (-1 if unreachable).  A longer path keeps its first max_len
+ * vertices and raises NWCCH_STATUS_QUEUE_FULL.  Shortcuts are split at the
+ * first lower triangle, x ascending, whose sides sum to the shortcut's value,
+ * as CCH.query_path / _unpack do (python/cch.py:406-507). */
+int nwcch_query_path(nwcch_dev* h, const int32_t* ud, const int32_t* s_rank, const int32_t* t_rank, int64_t nq,
+                     int32_t max_len, int32_t* dist, int32_t* path_len, int32_t* path, void* stream);
+
 /* CCH potential of a customized metric: p (device int32[n], indexed by rank)
  * = min(0, min over u of dist(u, v)) for the vertex of each rank, the
  * virtual-source distances Johnson's algorithm computes, in two level-by-level

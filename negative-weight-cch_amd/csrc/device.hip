@@ -682,6 +682,232 @@ __global__ void __launch_bounds__(64) k_query_chunk(const VInfo* __restrict__ vi
   }
 }
 
+// ------------------------------------------------------------ path unpacking
+
+// Arc (lo, hi), lo < hi, found by the wavefront in lo's sorted row (chordality
+// guarantees it exists for the arcs of a path); -1 if absent.
+__device__ __forceinline__ int32_t wave_find_arc(const VInfo* __restrict__ vinfo, const int32_t* __restrict__ head,
+                                                 int32_t lo, int32_t hi, int lane) {
+  const VInfo r = vinfo[lo];
+  for (int32_t k = 0; k < r.deg; k += 64) {
+    const bool hit = k + lane < r.deg && head[r.first + k + lane] == hi;
+    const unsigned long long m = __ballot(hit);
+    if (m) return r.first + k + __ffsll((long long)m) - 1;
+  }
+  return -1;
+}
+
+// One query with its path (python/cch.py:406-507), one wavefront per query.
+// The two searches run as in k_query, with the improving arc of every label and
+// the vertex of every chain position kept in LDS.  The peak is the first common
+// ancestor (from s upward) of least df + dr; the legs are the shortcut arcs down
+// to s and t; each leg is unpacked with an explicit stack, splitting a shortcut
+// at the first lower triangle (x ascending, the reference's order) whose two
+// sides sum to its value.  path[q][0..max_len) receives ranks; path_len[q] is
+// the full length (-1 if unreachable), and a path longer than max_len raises
+// the capacity bit.
+__global__ void __launch_bounds__(64) k_query_path(const VInfo* __restrict__ vinfo,
+                                                   const int2* __restrict__ euler,
+                                                   const int32_t* __restrict__ adep,
+                                                   const int32_t* __restrict__ head,
+                                                   const int32_t* __restrict__ tail,
+                                                   const int32_t* __restrict__ arc_pos,
+                                                   const PosRec* __restrict__ prec,
+                                                   const int2* __restrict__ tri_ab,
+                                                   const int2* __restrict__ ud,
+                                                   const int32_t* __restrict__ qs,
+                                                   const int32_t* __restrict__ qt, i64 nq, int32_t n,
+                                                   int32_t maxlen, int32_t max_path, int32_t* __restrict__ dist,
+                                                   int32_t* __restrict__ path_len, int32_t* __restrict__ path,
+                                                   uint32_t* status) {
+  extern __shared__ int32_t lds[];
+  const i64 q = blockIdx.x;
+  const int lane = threadIdx.x;
+  const int half = lane >> 5;
+  const int hl = lane & 31;
+  if (q >= nq) return;
+  uint32_t st = 0;
+  const int32_t rs = qs[q], rt = qt[q];
+  if (rs < 0 || rs >= n || rt < 0 || rt >= n) {
+    if (lane == 0) {
+      dist[q] = kInf;
+      path_len[q] = -1;
+      atomicOr(status, (uint32_t)kStatusBadQuery);
+    }
+    return;
+  }
+  // LDS: per half labels, improving arcs and chain vertices by position; then
+  // the legs (2 ints each) and the unpacking stack (2 ints each)
+  int32_t* lab = lds + half * 3 * maxlen;
+  int32_t* par = lab + maxlen;
+  int32_t* cid = par + maxlen;
+  int32_t* legs = lds + 6 * maxlen;          // 2 * maxlen legs at most
+  int32_t* stk = legs + 4 * maxlen;          // 2 * maxlen entries
+  int32_t x = half ? rt : rs;
+  VInfo vi = vinfo[x];
+  const int32_t D = vi.depth;
+  const int32_t Ds = __shfl(D, 0, 64), Dt = __shfl(D, 32, 64);
+  for (int32_t i = hl; i <= D; i += 32) { lab[i] = kInf; par[i] = -1; }
+  __syncthreads();
+  if (hl == 0) lab[0] = 0;
+  __syncthreads();
+  const int32_t tin_s = euler[rs].x;
+  int32_t lca_depth = -1;
+  int32_t p = 0;
+  bool active = true;
+  while (__any(active)) {
+    VInfo nx = VInfo{0, 0, -1, 0};
+    if (active && vi.parent >= 0) nx = vinfo[vi.parent];
+    if (active) {
+      if (hl == 0) cid[p] = x;
+      if (half && lca_depth < 0) {
+        const int2 e = euler[x];
+        if (e.x <= tin_s && tin_s <= e.y) lca_depth = vi.depth;
+      }
+      const int32_t dx = lab[p];
+      if (dx != kInf) {
+        for (int32_t j = hl; j < vi.deg; j += 32) {
+          const int32_t a = vi.first + j;
+          const int32_t pos = D - adep[a];
+          const int2 u = ud[a];
+          const int32_t w = inf_add(dx, half ? u.y : u.x, st);
+          if (w < lab[pos]) { lab[pos] = w; par[pos] = a; }
+        }
+      }
+    }
+    __syncthreads();
+    if (active) {
+      if (vi.parent < 0) active = false;
+      else { x = vi.parent; vi = nx; p++; }
+    }
+  }
+  // peak: the first s-chain position (from s upward) of least df + dr among
+  // the common ancestors (depth <= depth of the lowest common ancestor)
+  const int32_t lcad = __shfl(lca_depth, 32, 64);
+  const int32_t* labs = lds;
+  const int32_t* labt = lds + 3 * maxlen;
+  long long best = (long long)kInf;
+  int32_t bestp = 0x7fffffff;
+  for (int32_t ps = Ds - lcad + lane; ps <= Ds; ps += 64) {
+    const int32_t pt = Dt - (Ds - ps);
+    const int32_t a = labs[ps], b = labt[pt];
+    if (a != kInf && b != kInf) {
+      const long long v = (long long)a + b;
+      if (v < best) { best = v; bestp = ps; }
+    }
+  }
+  // wave-wide: least value, then the lowest position holding it
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const long long ob = __shfl_xor(best, o, 64);
+    const int32_t op = __shfl_xor(bestp, o, 64);
+    if (ob < best || (ob == best && op < bestp)) { best = ob; bestp = op; }
+  }
+  if (lcad < 0 || best >= (long long)kInf) {
+    if (lane == 0) { dist[q] = kInf; path_len[q] = -1; }
+    if (st) atomicOr(status, st);
+    return;
+  }
+  if (best <= -(long long)kInf) st |= kStatusOverflow;
+  const int32_t* pars = lds + maxlen;
+  const int32_t* cids = lds + 2 * maxlen;
+  const int32_t* part = lds + 4 * maxlen;
+  // legs, up side: from the peak down to s through the improving arcs, stored
+  // from the far end so that they read s -> peak
+  int32_t nl = 0;
+  if (lane == 0) {
+    int32_t pos = bestp;
+    int32_t cnt = 0;
+    while (pos != 0) { pos = Ds - vinfo[tail[pars[pos]]].depth; cnt++; }
+    pos = bestp;
+    int32_t k = cnt;
+    while (pos != 0) {
+      const int32_t a = pars[pos];
+      const int32_t v = tail[a];
+      k--;
+      legs[2 * k] = v;
+      legs[2 * k + 1] = cids[pos];
+      pos = Ds - vinfo[v].depth;
+    }
+    // down side: from the peak to t
+    nl = cnt;
+    const int32_t peak = cids[bestp];
+    int32_t pt = Dt - (Ds - bestp);
+    int32_t xv = peak;
+    while (pt != 0) {
+      const int32_t a = part[pt];
+      const int32_t v = tail[a];
+      legs[2 * nl] = xv;
+      legs[2 * nl + 1] = v;
+      nl++;
+      xv = v;
+      pt = Dt - vinfo[v].depth;
+    }
+  }
+  nl = __shfl(nl, 0, 64);
+  __syncthreads();
+  int32_t* out = path + q * (i64)max_path;
+  int32_t L = 1;
+  if (lane == 0 && max_path > 0) out[0] = nl ? legs[0] : rs;
+  for (int32_t l = 0; l < nl; l++) {
+    int32_t sp = 1;
+    if (lane == 0) { stk[0] = legs[2 * l]; stk[1] = legs[2 * l + 1]; }
+    __syncthreads();
+    while (sp > 0) {
+      sp--;
+      const int32_t v = stk[2 * sp], w = stk[2 * sp + 1];
+      const int32_t lo = min(v, w), hi = max(v, w);
+      const int32_t a = wave_find_arc(vinfo, head, lo, hi, lane);
+      int32_t mid = -1;
+      if (a >= 0) {
+        const int2 ua = ud[a];
+        const int32_t cost = v < w ? ua.x : ua.y;
+        const PosRec r = prec[arc_pos[a]];
+        for (int32_t k = 0; k < r.tri_count && mid < 0; k += 64) {
+          bool hit = false;
+          int2 ab = make_int2(0, 0);
+          if (k + lane < r.tri_count) {
+            ab = tri_ab[r.tri_begin + k + lane];   // (arc(x, lo), arc(x, hi))
+            const int2 A = ud[ab.x], B = ud[ab.y];
+            // v -> x -> w: down of arc(x, v), then up of arc(x, w)
+            const int32_t c1 = v == lo ? A.y : B.y;
+            const int32_t c2 = v == lo ? B.x : A.x;
+            hit = c1 != kInf && c2 != kInf && cost != kInf && (long long)c1 + c2 == (long long)cost;
+          }
+          const unsigned long long m = __ballot(hit);
+          if (m) {
+            const int first = __ffsll((long long)m) - 1;
+            const int32_t xa = __shfl(ab.x, first, 64);
+            mid = tail[xa];
+          }
+        }
+      }
+      if (mid >= 0 && sp + 2 > 2 * maxlen) {  // cannot happen: nesting is bounded by the ET depth
+        st |= kStatusQueueFull;
+        mid = -1;
+      }
+      if (mid < 0) {
+        if (lane == 0 && L < max_path) out[L] = w;
+        L++;
+      } else {
+        if (lane == 0) {
+          stk[2 * sp] = mid; stk[2 * sp + 1] = w;
+          stk[2 * sp + 2] = v; stk[2 * sp + 3] = mid;
+        }
+        sp += 2;
+      }
+      __syncthreads();
+    }
+  }
+  st = wave_or(st);
+  if (lane == 0) {
+    dist[q] = (int32_t)best;
+    path_len[q] = L;
+    if (L > max_path) st |= kStatusQueueFull;
+    if (st) atomicOr(status, st);
+  }
+}
+
 }  // namespace nwcch
 
 // ================================================================ C ABI (device)
@@ -792,6 +1018,7 @@ extern "C" int nwcch_dev_create(const nwcch_topo* topo, int device, nwcch_dev** 
     for (i64 a = 0; a < t.mp; a++) adep[a] = t.vdepth[t.head[a]];
     d.adep = dev_upload(d, adep.data(), adep.size());
     d.head = dev_upload(d, t.head.data(), t.head.size());
+    d.arc_pos = dev_upload(d, t.arc_pos.data(), t.arc_pos.size());
     d.tail = dev_upload(d, t.arc_tail.data(), t.arc_tail.size());
     d.status = dev_alloc<uint32_t>(d, 1);
     HIP_OK(hipMemset(d.status, 0, sizeof(uint32_t)));
@@ -960,6 +1187,32 @@ extern "C" int nwcch_query_multi(nwcch_dev* h, const int64_t* ud_table, int32_t 
 extern "C" int nwcch_query(nwcch_dev* h, const int32_t* ud, const int32_t* s_rank, const int32_t* t_rank,
                            int64_t nq, int32_t* dist, int32_t* scanned, void* stream) {
   return nwcch_query_strided(h, ud, 1, s_rank, t_rank, nq, nullptr, dist, scanned, stream);
+}
+
+extern "C" int nwcch_query_path(nwcch_dev* h, const int32_t* ud, const int32_t* s_rank, const int32_t* t_rank,
+                                int64_t nq, int32_t max_len, int32_t* dist, int32_t* path_len, int32_t* path,
+                                void* stream) {
+  if (!h || !ud || !s_rank || !t_rank || !dist || !path_len || (max_len > 0 && !path) || nq < 0 || max_len < 0) {
+    set_error("bad argument");
+    return NWCCH_ERR_ARG;
+  }
+  if (nq == 0) return NWCCH_OK;
+  if (nq > 0x7fffffffll) { set_error("at most 2^31-1 queries per launch"); return NWCCH_ERR_CAPACITY; }
+  try {
+    DevTopo& d = h->d;
+    hipStream_t s = (hipStream_t)stream;
+    const int32_t maxlen = d.depth + 1;
+    const size_t lds = (size_t)14 * maxlen * sizeof(int32_t);
+    if (lds > 160 * 1024) throw std::runtime_error("elimination tree too deep for the LDS path kernel");
+    ProfScope ps(d, s, kKQuery);
+    hipLaunchKernelGGL(k_query_path, dim3((unsigned)nq), dim3(64), lds, s, d.vinfo, d.euler, d.adep, d.head, d.tail,
+                       d.arc_pos, d.prec, d.tri_ab, reinterpret_cast<const int2*>(ud), s_rank, t_rank, (i64)nq,
+                       (int32_t)d.n, maxlen, max_len, dist, path_len, path, d.status);
+    HIP_OK(hipGetLastError());
+  } catch (const std::exception& e) {
+    return fail(e);
+  }
+  return NWCCH_OK;
 }
 
 extern "C" int nwcch_potential(nwcch_dev* h, const int32_t* ud, int32_t* p, void* stream) {

@@ -255,7 +255,7 @@ extern "C" int nwcch_update_prepare(nwcch_dev* h, nwcch_topo* topo) {
     if (t.mp != d.mp || t.n != d.n) throw std::runtime_error("topology does not match the device handle");
     build_side_index(t);
     HIP_OK(hipSetDevice(d.device));
-    d.arc_pos = up_upload(d, t.arc_pos.data(), t.arc_pos.size());
+    if (!d.arc_pos) d.arc_pos = up_upload(d, t.arc_pos.data(), t.arc_pos.size());
     std::vector<int32_t> alvl(t.mp);
     for (i64 a = 0; a < t.mp; a++) alvl[a] = t.vlevel[t.arc_tail[a]];
     d.arc_lvl_host = alvl;

@@ -61,6 +61,7 @@ def lib():
             "nwcch_set_tuning": ([vp, C.c_char_p, i32], C.c_int),
             "nwcch_get_tuning": ([vp, C.c_char_p, p], C.c_int),
             "nwcch_potential": ([vp, p, p, vp], C.c_int),
+            "nwcch_query_path": ([vp, p, p, p, i64, i32, p, p, p, vp], C.c_int),
         }
         optional = {}
         for name, (args, res) in sigs.items():
@@ -104,7 +105,8 @@ def exported_symbols():
             "nwcch_query", "nwcch_status", "nwcch_profile_begin", "nwcch_profile_end",
             "nwcch_update_prepare", "nwcch_update", "nwcch_update_changed", "nwcch_query_strided",
             "nwcch_customize_batch", "nwcch_negcycle_scan_batch", "nwcch_batch_slab_width",
-            "nwcch_set_tuning", "nwcch_get_tuning", "nwcch_query_multi", "nwcch_potential"]
+            "nwcch_set_tuning", "nwcch_get_tuning", "nwcch_query_multi", "nwcch_potential",
+            "nwcch_query_path"]
 
 
 # ------------------------------------------------------------ host topology

@@ -198,6 +198,18 @@ class CCH:
             return True
         return self._last is not None and self._last[2] > 0
 
+    def query_path(self, s, t, max_len: int = 1 << 16):
+        """(distance, [original vertex ids]) of the shortest s-t path, shortcuts
+        unpacked as python/cch.py:406-507 does; (inf, []) if unreachable."""
+        d, plen, path = self.dev.query_path(self.ud, [int(self.rank[s])], [int(self.rank[t])], max_len=max_len)
+        self.dev.check_status()
+        L = int(plen[0].item())
+        if L < 0:
+            return float("inf"), []
+        if not hasattr(self, "_inv"):
+            self._inv = np.argsort(np.asarray(self.rank)).astype(np.int64)
+        return int(d[0].item()), self._inv[path[0, :L].cpu().numpy()].tolist()
+
     def potential(self):
         """A feasible potential of the customized metric, by original vertex id:
         p(v) = min(0, min_u dist(u, v)) (python/cch.py:734-755); ints."""

@@ -112,6 +112,20 @@ class DeviceTopology:
         check(lib().nwcch_negcycle_scan(ptr(ud), ud.shape[0], ptr(out), _stream(stream)), "nwcch_negcycle_scan")
         return out
 
+    def query_path(self, ud: torch.Tensor, s_rank, t_rank, max_len: int = 4096, stream=None):
+        """Distances and unpacked paths (ranks) -> (dist int32 [nq], path_len int32
+        [nq], path int32 [nq, max_len]); device tensors, not synchronised."""
+        assert ud.dtype == torch.int32 and ud.shape == (self.mp, 2) and ud.is_contiguous()
+        qs = torch.as_tensor(s_rank).to(self.dev, torch.int32).contiguous()
+        qt = torch.as_tensor(t_rank).to(self.dev, torch.int32).contiguous()
+        nq = qs.shape[0]
+        dist = torch.empty(nq, dtype=torch.int32, device=self.dev)
+        plen = torch.empty(nq, dtype=torch.int32, device=self.dev)
+        path = torch.empty((nq, max_len), dtype=torch.int32, device=self.dev)
+        check(lib().nwcch_query_path(self._h, ptr(ud), ptr(qs), ptr(qt), nq, int(max_len), ptr(dist), ptr(plen),
+                                     ptr(path), _stream(stream)), "nwcch_query_path")
+        return dist, plen, path
+
     def potential(self, ud: torch.Tensor, out: torch.Tensor | None = None, stream=None) -> torch.Tensor:
         """CCH potential indexed by rank: device int32 [n] (cpp/cch.cpp:635-655)."""
         assert ud.dtype == torch.int32 and ud.shape == (self.mp, 2) and ud.is_contiguous()
