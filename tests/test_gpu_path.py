@@ -97,4 +97,4 @@ def test_path_edge_cases():
     assert dist[2] == INF and plen[2] == -1                    # nothing reaches 5
     assert dt.status() & 8                                      # the truncated path
     dist, plen, path = dt.query_path(ud, rank[s], rank[t], max_len=8)
-    assert dt.status() == 0 and inv[path[1, :5]].tolist() == [0, 1, 2, 3, 4]
+    assert dt.status() == 0 and inv[path[1, :5].cpu().numpy()].tolist() == [0, 1, 2, 3, 4]
