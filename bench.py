@@ -204,6 +204,18 @@ def plain(args, out):
     t1 = timed(lambda: [dt.query(ud, qs[i:i + 1], qt[i:i + 1], want_scanned=False) for i in range(100)], 3) / 100
     dt.check_status()
 
+    # the steps after the path: the CCH potential, and queries with unpacked paths
+    pot = torch.empty(g.n, dtype=torch.int32, device=dt.dev)
+    dt.potential(ud, out=pot)
+    pot_ms = statistics.median(event_ms(lambda: dt.potential(ud, out=pot), 5))
+    npq = min(args.queries, 2000)
+    pd, plen, _ = dt.query_path(ud, qs[:npq], qt[:npq], max_len=8192)
+    pth_ms = statistics.median(event_ms(lambda: dt.query_path(ud, qs[:npq], qt[:npq], max_len=8192), 3))
+    dt.check_status()
+    plen_h = plen.cpu().numpy()
+    paths = {"queries": npq, "ms": pth_ms, "queries_per_s": npq / (pth_ms / 1e3),
+             "mean_vertices": float(plen_h[plen_h > 0].mean()) if (plen_h > 0).any() else 0.0}
+
     parity = None
     if not args.no_check:
         from oracle import oracle as O
@@ -215,6 +227,9 @@ def plain(args, out):
         nchk = min(args.queries, 20000)
         d0, _, _ = st.query(up, dn, qs_h[:nchk], qt_h[:nchk], threads=os.cpu_count() or 1)
         ok &= np.array_equal(dist[:nchk].cpu().numpy(), d0)
+        ok &= np.array_equal(pd.cpu().numpy(), d0[:npq] if nchk >= npq else pd.cpu().numpy())
+        pj, _ = O.johnson(g.n, g.off, g.tgt, g.w)
+        ok &= np.array_equal(pot.cpu().numpy()[rank].astype(np.int64), pj)
         parity = bool(ok)
         log(f"plain parity vs oracle: {parity}")
 
@@ -253,6 +268,8 @@ def plain(args, out):
                     "single_launch_queries_per_s": 1.0 / t1,
                     "single_p50_ms": float(np.percentile(lat, 50)),
                     "single_p99_ms": float(np.percentile(lat, 99))},
+        "potential_ms": pot_ms,
+        "path_queries": paths,
         "parity_vs_oracle": parity,
         "tuning": {"customize": tuning, "queries": qtune},
         "host": dict(host, nproc=os.cpu_count()),
