@@ -230,10 +230,10 @@ int nwcch_status(nwcch_dev* h, uint32_t* out, int clear, void* stream);
  *                   latency-oriented kernels (deeper prefetch), larger launches
  *                   the throughput-oriented one (more wavefronts per CU);
  *                   default 32768.
- *   "query_chunk"   1 (default): the latency-oriented launches read ancestor
- *                   chains 32 vertices at a time and prefetch rows four chain
- *                   vertices ahead; 0: they walk parent pointers, prefetching
- *                   128 arcs of the next row. */
+ *   "query_chunk"   1: the latency-oriented launches read ancestor chains 32
+ *                   vertices at a time and prefetch rows four chain vertices
+ *                   ahead; 0: they walk parent pointers, prefetching 128 arcs
+ *                   of the next row.  Default 1 for ET depths above 1024. */
 int nwcch_set_tuning(nwcch_dev* h, const char* key, int32_t value);
 int nwcch_get_tuning(const nwcch_dev* h, const char* key, int32_t* value);
 

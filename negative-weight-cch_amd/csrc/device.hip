@@ -955,6 +955,9 @@ extern "C" int nwcch_dev_create(const nwcch_topo* topo, int device, nwcch_dev** 
     d.device = device;
     d.n = t.n; d.m = t.m; d.mp = t.mp; d.T = t.triangles;
     d.levels = t.levels; d.depth = t.depth;
+    // the chunked query kernel pays off on very deep hierarchies only (its
+    // registers cost occupancy on shallow ones)
+    d.query_chunk = t.depth > 1024 ? 1 : 0;
     d.lv = t.lv;
     d.loop_edges = t.loop_edges;
     d.sched_arc = dev_upload(d, t.sched_arc.data(), t.sched_arc.size());

@@ -66,7 +66,8 @@ struct DevTopo {
   int32_t batch_tri_per_lane = 8;            // batched level kernel
   int32_t upd_persistent_max = 2;            // updates of at most this many changed arcs run
                                              // in one persistent workgroup (mode auto)
-  int32_t query_chunk = 1;                   // queries read ancestor chains 32 at a time
+  int32_t query_chunk = 0;                   // queries read ancestor chains 32 at a time
+                                             // (set at creation for deep hierarchies)
   int32_t query_deep_max = 32768;            // query launches up to this size prefetch 4 x 32
                                              // arcs of the next row, larger ones 32
   int32_t flat_min_tri = 0;                  // levels whose heaviest arc has >= this many
