@@ -297,10 +297,25 @@ def cpu_baseline(g, rank, topo, reps=5):
         ts.append(time.perf_counter() - t0)
     tmed = statistics.median(ts)
     log(f"cpu baseline ({kind}): {1000 * tmed:.2f} ms per customization")
-    return {"value": topo.triangles / tmed, "unit": "triangles/s", "cores": 1, "kind": kind,
-            "sample": f"{what} on the same graph and order, median of {reps} full customizations "
-                      f"(range {1000 * min(ts):.1f}-{1000 * max(ts):.1f} ms)",
-            "ms": 1000 * tmed}
+    res = {"value": topo.triangles / tmed, "unit": "triangles/s", "cores": 1, "kind": kind,
+           "sample": f"{what} on the same graph and order, median of {reps} full customizations "
+                     f"(range {1000 * min(ts):.1f}-{1000 * max(ts):.1f} ms)",
+           "ms": 1000 * tmed}
+    if kind == "reference":
+        # the reference's own multi-threaded form beside it, all host threads
+        thr = os.cpu_count() or 1
+        ref.customize(g.w, threads=thr)
+        tp = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            ref.customize(g.w, threads=thr)
+            tp.append(time.perf_counter() - t0)
+        tpm = statistics.median(tp)
+        log(f"cpu baseline (reference, {thr} threads): {1000 * tpm:.2f} ms per customization")
+        res["parallel"] = {"value": topo.triangles / tpm, "cores": thr, "ms": 1000 * tpm,
+                           "sample": "cpp/cch.cpp CCH::customize_parallel (OpenMP, level buckets), "
+                                     f"median of {reps}"}
+    return res
 
 
 def batched(args, out):

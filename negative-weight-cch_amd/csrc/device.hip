@@ -179,16 +179,18 @@ __global__ void __launch_bounds__(256) k_customize_level(LevelLaunch L, const Po
 // level (a level's triangles are contiguous, grouped by the arc they improve),
 // so a heavy arc is spread over as many lanes as it has triangles and every
 // lane does one dependent gather.  Lanes of one arc are combined by a segmented
-// suffix minimum over the wavefront; the first lane of each segment folds it
-// into the arc with an atomic minimum (the arc holds its base value; min is
-// order independent, so the result is exact and deterministic).  The atomics
-// are performed beyond the XCD's L2 and leave no dirty lines for the end of
-// the launch to write back; with the level's arcs in count order, consecutive
-// segments target scattered arcs rather than one cache line.
+// suffix minimum over the wavefront.  The first lane of each segment then
+// folds it into the arc: with an atomic minimum into the base value the arc
+// holds (min is order independent: exact and deterministic), or, with kStore,
+// an arc whose triangles all lie in this wavefront is written with one store
+// of min(base, segment) (1: plain, 2: non-temporal) and only arcs cut by a
+// wavefront boundary use atomics.  The atomics are performed beyond the XCD's
+// L2.
 //
 // With xcd_slices, workgroup b takes slice (b % 8) * per + b / 8 of the level
 // (grid = 8 * per): workgroups are dealt to the 8 XCDs in turn, so each XCD
 // works through one contiguous eighth of the level.
+template <int kStore>
 __global__ void __launch_bounds__(256) k_customize_flat(i64 q0, i64 nt, const int2* __restrict__ tri_ab,
                                                         const int32_t* __restrict__ tri_c, int2* ud,
                                                         uint32_t* status, int xcd_slices) {
@@ -198,33 +200,76 @@ __global__ void __launch_bounds__(256) k_customize_flat(i64 q0, i64 nt, const in
   const i64 i = (i64)slice * blockDim.x + threadIdx.x;
   const bool valid = i < nt;
   uint32_t st = 0;
-  int32_t c = -1, up = kInf, dn = kInf;
+  int32_t c = -1, up = kInf, dn = kInf, c_out = -1;
+  int2 ab = make_int2(0, 0);
   if (valid) {
     const i64 q = q0 + i;
     c = tri_c[q];
-    const int2 ab = tri_ab[q];
+    ab = tri_ab[q];
+    if (kStore) {  // the neighbouring wavefronts' arcs at the two edges of this one
+      if (lane == 0 && i > 0) c_out = tri_c[q - 1];
+      if (lane == 63 && i + 1 < nt) c_out = tri_c[q + 1];
+    }
+  }
+  const int32_t cprev = __shfl_up(c, 1, 64);
+  const bool head = valid && (lane == 0 || cprev != c);
+  int2 base = make_int2(kInf, kInf);
+  if (valid) {
+    if (kStore && head) base = ud[c];
     const int2 A = ud[ab.x], B = ud[ab.y];
     up = inf_add(A.y, B.x, st);  // y -> x -> z
     dn = inf_add(B.y, A.x, st);  // z -> x -> y
   }
+  int open = kStore ? (valid && c_out == c) : 1;  // the arc continues past this wavefront
   // an arc's lanes are contiguous: if lane+o has the same arc, so has every lane between
 #pragma unroll
   for (int o = 1; o < 64; o <<= 1) {
     const int32_t c2 = __shfl_down(c, o, 64);
     const int32_t u2 = __shfl_down(up, o, 64);
     const int32_t d2 = __shfl_down(dn, o, 64);
+    const int o2 = kStore ? __shfl_down(open, o, 64) : 1;
     if (lane + o < 64 && c2 == c) {
       up = min(up, u2);
       dn = min(dn, d2);
+      if (kStore) open |= o2;
     }
   }
-  const int32_t cprev = __shfl_up(c, 1, 64);
-  if (valid && (lane == 0 || cprev != c)) {
-    int32_t* v = reinterpret_cast<int32_t*>(ud) + 2 * (i64)c;
-    if (up != kInf) atomicMin(v, up);
-    if (dn != kInf) atomicMin(v + 1, dn);
+  if (head) {
+    if (kStore && !open) {
+      const int2 v = make_int2(min(base.x, up), min(base.y, dn));
+      if (kStore == 2) {  // the (up, down) pair as one 64-bit word, little-endian
+        const unsigned long long word =
+            ((unsigned long long)(uint32_t)v.y << 32) | (unsigned long long)(uint32_t)v.x;
+        __builtin_nontemporal_store(word, reinterpret_cast<unsigned long long*>(ud) + c);
+      } else {
+        ud[c] = v;
+      }
+    } else {
+      int32_t* v = reinterpret_cast<int32_t*>(ud) + 2 * (i64)c;
+      if (up != kInf) atomicMin(v, up);
+      if (dn != kInf) atomicMin(v + 1, dn);
+    }
   }
   if (st) atomicOr(status, st);
+}
+
+// The triangle list re-ordered inside every level by arc id (tail rank): arcs
+// of one tail are then adjacent, and nested dissection makes a run of tails
+// one region of the graph, so the gathers of a run share cache lines.  Built
+// on demand from the schedule-order list: each record's triangles are copied
+// to their new place (levels keep their ranges).
+__global__ void __launch_bounds__(256) k_reorder_tri(const PosRec* __restrict__ prec, const i64* __restrict__ nbegin,
+                                                     i64 mp, const int2* __restrict__ tri_ab, int2* tri_ab2,
+                                                     int32_t* tri_c2) {
+  const i64 stride = (i64)gridDim.x * blockDim.x;
+  for (i64 p = (i64)blockIdx.x * blockDim.x + threadIdx.x; p < mp; p += stride) {
+    const PosRec r = prec[p];
+    const i64 nb = nbegin[p];
+    for (int32_t j = 0; j < r.tri_count; j++) {
+      tri_ab2[nb + j] = tri_ab[r.tri_begin + j];
+      tri_c2[nb + j] = r.arc;
+    }
+  }
 }
 
 // tri_c[q] = the arc that triangle q improves (expanded from the schedule records)
@@ -944,6 +989,35 @@ int fail(const std::exception& e) {
 
 }  // namespace
 
+// The arc-id-ordered copy of the triangle list (k_reorder_tri), built once.
+void build_arc_order(DevTopo& d) {
+  std::vector<i64> nbegin(d.mp);
+  std::vector<i64> pos;
+  i64 q = 0;
+  for (const LevelRange& r : d.lv) {
+    pos.resize(r.end - r.begin);
+    for (i64 p = r.begin; p < r.end; p++) pos[p - r.begin] = p;
+    std::sort(pos.begin(), pos.end(), [&](i64 a, i64 b) { return d.sched_host[a] < d.sched_host[b]; });
+    for (i64 p : pos) {
+      nbegin[p] = q;
+      q += d.pos_cnt[p];
+    }
+  }
+  if (q != d.T) throw std::runtime_error("triangle list does not cover the levels");
+  i64* dn = dev_upload(d, nbegin.data(), nbegin.size());
+  d.tri_ab2 = dev_alloc<int2>(d, (size_t)std::max<i64>(d.T, 1));
+  d.tri_c2 = dev_alloc<int32_t>(d, (size_t)std::max<i64>(d.T, 1));
+  if (d.mp > 0) {
+    const i64 blocks = std::min<i64>((d.mp + 255) / 256, 16384);
+    hipLaunchKernelGGL(k_reorder_tri, dim3((unsigned)blocks), dim3(256), 0, 0, d.prec, dn, d.mp, d.tri_ab, d.tri_ab2,
+                       d.tri_c2);
+    HIP_OK(hipGetLastError());
+  }
+  HIP_OK(hipDeviceSynchronize());
+  d.owned.erase(std::find(d.owned.begin(), d.owned.end(), (void*)dn));   // the starts are not kept
+  HIP_OK(hipFree(dn));
+}
+
 extern "C" int nwcch_dev_create(const nwcch_topo* topo, int device, nwcch_dev** out) {
   if (!topo || !out) { set_error("null argument"); return NWCCH_ERR_ARG; }
   *out = nullptr;
@@ -970,6 +1044,7 @@ extern "C" int nwcch_dev_create(const nwcch_topo* topo, int device, nwcch_dev** 
     d.prec = dev_upload(d, pr.data(), pr.size());
     d.pos_cnt.resize(t.mp);
     for (i64 p = 0; p < t.mp; p++) d.pos_cnt[p] = pr[p].tri_count;
+    d.sched_host = t.sched_arc;
     d.lv_tri0.resize(d.lv.size() + 1);
     for (size_t i = 0; i < d.lv.size(); i++) d.lv_tri0[i] = t.tri_off[d.lv[i].begin];
     d.lv_tri0[d.lv.size()] = t.triangles;
@@ -979,6 +1054,7 @@ extern "C" int nwcch_dev_create(const nwcch_topo* topo, int device, nwcch_dev** 
       hipLaunchKernelGGL(k_fill_tri_c, dim3((unsigned)blocks), dim3(256), 0, 0, d.prec, t.mp, d.tri_c);
       HIP_OK(hipGetLastError());
     }
+    build_arc_order(d);   // the triangle-parallel sweep's default order
     std::vector<VInfo> vi(t.n);
     for (i64 x = 0; x < t.n; x++)
       vi[x] = VInfo{(int32_t)t.first[x], (int32_t)(t.first[x + 1] - t.first[x]), t.parent[x], t.vdepth[x]};
@@ -1076,8 +1152,17 @@ extern "C" int nwcch_customize(nwcch_dev* h, const int32_t* w, int32_t* ud, void
         const i64 q0 = d.lv_tri0[i], nt = d.lv_tri0[i + 1] - q0;
         const i64 nb = (nt + 255) / 256;
         const i64 grid = d.flat_xcd ? 8 * ((nb + 7) / 8) : nb;  // XCD slices: 8 equal shares
-        hipLaunchKernelGGL(k_customize_flat, dim3((unsigned)grid), dim3(256), 0, s, q0, nt, d.tri_ab, d.tri_c, u,
-                           d.status, d.flat_xcd);
+        const int2* tab = d.flat_order ? d.tri_ab2 : d.tri_ab;
+        const int32_t* tc = d.flat_order ? d.tri_c2 : d.tri_c;
+        if (d.flat_store == 1)
+          hipLaunchKernelGGL(k_customize_flat<1>, dim3((unsigned)grid), dim3(256), 0, s, q0, nt, tab, tc, u, d.status,
+                             d.flat_xcd);
+        else if (d.flat_store == 2)
+          hipLaunchKernelGGL(k_customize_flat<2>, dim3((unsigned)grid), dim3(256), 0, s, q0, nt, tab, tc, u, d.status,
+                             d.flat_xcd);
+        else
+          hipLaunchKernelGGL(k_customize_flat<0>, dim3((unsigned)grid), dim3(256), 0, s, q0, nt, tab, tc, u, d.status,
+                             d.flat_xcd);
         continue;
       }
       const LevelLaunch& L = launches[i];
@@ -1260,6 +1345,23 @@ extern "C" int nwcch_set_tuning(nwcch_dev* h, const char* key, int32_t value) {
     h->d.query_deep_max = value;
     return NWCCH_OK;
   }
+  if (k == "flat_store") {
+    if (value < 0 || value > 2) { set_error("flat_store is 0, 1 or 2"); return NWCCH_ERR_ARG; }
+    h->d.flat_store = value;
+    return NWCCH_OK;
+  }
+  if (k == "flat_order") {
+    if (value < 0 || value > 1) { set_error("flat_order is 0 or 1"); return NWCCH_ERR_ARG; }
+    if (value == 1 && !h->d.tri_ab2) {
+      try {
+        build_arc_order(h->d);
+      } catch (const std::exception& e) {
+        return fail(e);
+      }
+    }
+    h->d.flat_order = value;
+    return NWCCH_OK;
+  }
   if (k == "flat_xcd") {
     if (value < 0 || value > 1) { set_error("flat_xcd is 0 or 1"); return NWCCH_ERR_ARG; }
     h->d.flat_xcd = value;
@@ -1284,6 +1386,8 @@ extern "C" int nwcch_get_tuning(const nwcch_dev* h, const char* key, int32_t* va
   else if (k == "batch_tri_per_lane") *value = h->d.batch_tri_per_lane;
   else if (k == "flat_min_tri") *value = h->d.flat_min_tri;
   else if (k == "flat_xcd") *value = h->d.flat_xcd;
+  else if (k == "flat_store") *value = h->d.flat_store;
+  else if (k == "flat_order") *value = h->d.flat_order;
   else if (k == "query_deep_max") *value = h->d.query_deep_max;
   else if (k == "query_chunk") *value = h->d.query_chunk;
   else if (k == "upd_persistent_max") *value = h->d.upd_persistent_max;
