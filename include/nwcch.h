@@ -41,6 +41,7 @@ extern "C" {
 #define NWCCH_STATUS_BAD_WEIGHT 2u
 #define NWCCH_STATUS_BAD_QUERY 4u
 #define NWCCH_STATUS_QUEUE_FULL 8u
+#define NWCCH_STATUS_PATH_LIMIT 16u /* a path unpacking passed 4n + 16 vertices: the metric has a negative cycle */
 
 typedef struct nwcch_topo nwcch_topo; /* host topology (metric independent) */
 typedef struct nwcch_dev nwcch_dev;   /* device copy of one topology */
@@ -62,6 +63,19 @@ int nwcch_order(int64_t n, const int64_t* off, const int32_t* tgt, const double*
  * cch_nb.symbolic + edge_arcs (python/cch_nb.py:431-515); cpp/cch.cpp:377-435. */
 int nwcch_topo_create(int64_t n, const int64_t* off, const int32_t* tgt, const int32_t* rank,
                       nwcch_topo** out);
+
+/* nwcch_topo_create with options.  flags & NWCCH_TOPO_DEVICE_INDEX: the
+ * lower-triangle index (python/cch.py:168-186, by_c cch.py:308) is built on GPU
+ * `device` instead of the host.  The host then counts nothing and keeps no
+ * triangle list: GPU `device` counts every arc's lower triangles, the host
+ * orders the schedule from the counts, nwcch_dev_create fills the triangle
+ * lists in device memory, and nwcch_update_prepare fills the side lists
+ * (python/cch.py:309-324) there.  Every array equals the host-built one.
+ * nwcch_topo_export_triangles then serves sched_arc and tri_off only;
+ * nwcch_dev_export_triangles reads the list back.  A GPU is needed. */
+#define NWCCH_TOPO_DEVICE_INDEX 1
+int nwcch_topo_create_ex(int64_t n, const int64_t* off, const int32_t* tgt, const int32_t* rank,
+                         int32_t flags, int device, nwcch_topo** out);
 
 /* The same from an existing upward CSR (first[n+1], head[mp]) and arc map
  * (e_arc[m] with -1 for loops, e_up[m]), i.e. the arrays cch_nb.customize takes
@@ -95,6 +109,17 @@ void nwcch_dev_destroy(nwcch_dev* h);
 
 /* out6 = {n, m, mp, triangles, depth, levels} */
 int nwcch_dev_info(const nwcch_dev* h, int64_t* out6);
+
+/* Host copy of the handle's triangle list in schedule order, tri_ab[2T] as
+ * nwcch_topo_export_triangles gives it (the device-built list of a topology
+ * made with NWCCH_TOPO_DEVICE_INDEX included).  Synchronous. */
+int nwcch_dev_export_triangles(const nwcch_dev* h, int32_t* tri_ab);
+
+/* Host copy of the partial-update side lists after nwcch_update_prepare:
+ * side_off[mp+1], side_top[side_off[mp]] (= 2T) -- for arc a the top arcs of
+ * the lower triangles a is a side of (python/cch.py:309-324).  Either pointer
+ * may be NULL.  Synchronous. */
+int nwcch_dev_export_sides(const nwcch_dev* h, int64_t* side_off, int32_t* side_top);
 
 /* Full customization of one metric: w[m] original arc weights (device) ->
  * ud[mp][2] (device).  Replaces cch_nb.customize (python/cch_nb.py:529-563),
@@ -165,7 +190,9 @@ int nwcch_negcycle_scan_batch(const int32_t* UD, int64_t mp, int32_t K, int32_t 
  * the ranks of the unpacked path from s_rank[i] to t_rank[i] in
  * path[i * max_len ...] (device int32 [nq][max_len]); path_len[i] = the number
  * of vertices (-1 if unreachable).  A longer path keeps its first max_len
- * vertices and raises NWCCH_STATUS_QUEUE_FULL.  Shortcuts are split at the
+ * vertices and raises NWCCH_STATUS_QUEUE_FULL.  On a metric with a negative
+ * cycle the unpacking can grow exponentially: past 4n + 16 vertices it stops,
+ * path_len[i] = -2 and NWCCH_STATUS_PATH_LIMIT is raised.  Shortcuts are split at the
  * first lower triangle, x ascending, whose sides sum to the shortcut's value,
  * as CCH.query_path / _unpack do (python/cch.py:406-507). */
 int nwcch_query_path(nwcch_dev* h, const int32_t* ud, const int32_t* s_rank, const int32_t* t_rank, int64_t nq,
@@ -180,7 +207,8 @@ int nwcch_query_path(nwcch_dev* h, const int32_t* ud, const int32_t* s_rank, con
  * NWCCH_STATUS_OVERFLOW. */
 int nwcch_potential(nwcch_dev* h, const int32_t* ud, int32_t* p, void* stream);
 
-/* Build (once, on the host) and upload the partial-update index of the topology
+/* Build (once, on the host, or on the device for a topology made with
+ * NWCCH_TOPO_DEVICE_INDEX) and upload the partial-update index of the topology
  * the handle was created from: for every arc the top arcs of the lower
  * triangles it is a side of, arc heights and frontier buckets.  The lazy
  * build_update_index of python/cch.py:289-347 (by_c, as_side, tail,

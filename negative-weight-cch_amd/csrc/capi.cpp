@@ -1,4 +1,7 @@
 // Host half of the C ABI: error reporting and the metric-independent topology.
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <stdexcept>
 #include <string>
@@ -31,6 +34,8 @@ int map_error(const std::exception& e) {
   if (s.find("chordal") != std::string::npos || s.find("missing") != std::string::npos)
     return NWCCH_ERR_CHORDAL;
   if (s.find("too many") != std::string::npos) return NWCCH_ERR_CAPACITY;
+  // HIP failures of the device-built index (HIP_OK names the call)
+  if (s.find("hip") != std::string::npos || s.find("device") != std::string::npos) return NWCCH_ERR_RUNTIME;
   return NWCCH_ERR_ARG;
 }
 
@@ -42,6 +47,23 @@ bool valid_csr(int64_t n, const int64_t* off) {
 }
 
 }  // namespace
+
+// With NWCCH_PROGRESS=1, a line on stderr per finished preprocessing phase.
+struct Phases {
+  bool on;
+  std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+  Phases() {
+    const char* e = std::getenv("NWCCH_PROGRESS");
+    on = e && e[0] == '1';
+  }
+  void done(const char* what) {
+    if (!on) return;
+    const auto t1 = std::chrono::steady_clock::now();
+    std::fprintf(stderr, "topology: %s %.2f s\n", what, std::chrono::duration<double>(t1 - t0).count());
+    std::fflush(stderr);
+    t0 = t1;
+  }
+};
 
 extern "C" const char* nwcch_last_error(void) { return g_last_error.c_str(); }
 
@@ -64,20 +86,38 @@ extern "C" int nwcch_order(int64_t n, const int64_t* off, const int32_t* tgt, co
 
 extern "C" int nwcch_topo_create(int64_t n, const int64_t* off, const int32_t* tgt,
                                  const int32_t* rank, nwcch_topo** out) {
+  return nwcch_topo_create_ex(n, off, tgt, rank, 0, 0, out);
+}
+
+extern "C" int nwcch_topo_create_ex(int64_t n, const int64_t* off, const int32_t* tgt, const int32_t* rank,
+                                    int32_t flags, int device, nwcch_topo** out) {
   if (!out) { set_error("null out"); return NWCCH_ERR_ARG; }
   *out = nullptr;
   if (!valid_csr(n, off) || (off[n] > 0 && !tgt) || !rank) {
     set_error("nwcch_topo_create: bad CSR or null argument");
     return NWCCH_ERR_ARG;
   }
+  if (flags & ~NWCCH_TOPO_DEVICE_INDEX) { set_error("nwcch_topo_create_ex: unknown flags"); return NWCCH_ERR_ARG; }
   nwcch_topo* h = new nwcch_topo();
   try {
+    Phases ph;
     std::vector<i64> noff;
     std::vector<i32> nbr;
     undirected_csr(n, off, tgt, noff, nbr);
     symbolic(h->t, n, noff, nbr, rank);
+    ph.done("symbolic contraction");
     map_arcs(h->t, off, tgt, off[n]);
-    build_index(h->t);
+    ph.done("arc map");
+    if (flags & NWCCH_TOPO_DEVICE_INDEX) {
+      build_meta(h->t);
+      std::vector<i32> cnt = count_triangles_device(h->t, device);
+      ph.done("triangle counts on the device");
+      build_schedule(h->t, cnt);
+      ph.done("schedule");
+    } else {
+      build_index(h->t);
+      ph.done("triangle index and schedule");
+    }
   } catch (const std::exception& e) {
     delete h;
     return map_error(e);
@@ -154,6 +194,12 @@ extern "C" int nwcch_topo_export_triangles(const nwcch_topo* h, int32_t* sched_a
   const Topology& t = h->t;
   if (sched_arc) std::memcpy(sched_arc, t.sched_arc.data(), t.sched_arc.size() * sizeof(int32_t));
   if (tri_off) std::memcpy(tri_off, t.tri_off.data(), t.tri_off.size() * sizeof(int64_t));
-  if (tri_ab) std::memcpy(tri_ab, t.tri_ab.data(), t.tri_ab.size() * sizeof(int32_t));
+  if (tri_ab) {
+    if (!t.has_tri) {
+      set_error("the triangle list of this topology lives on the device (nwcch_dev_export_triangles)");
+      return NWCCH_ERR_ARG;
+    }
+    std::memcpy(tri_ab, t.tri_ab.data(), t.tri_ab.size() * sizeof(int32_t));
+  }
   return NWCCH_OK;
 }

@@ -281,6 +281,63 @@ __global__ void __launch_bounds__(256) k_fill_tri_c(const PosRec* __restrict__ p
   }
 }
 
+// ------------------------------------------------ triangle index on the device
+
+// The lower triangles of arc j = (y, z) in pull form (python/cch.py:168-186,
+// by_c grouping cch.py:308): the x < y adjacent to both y and z.  The entries
+// a = arc(x, y) of y's down list come in ascending x; row x holds z after a
+// (rows are sorted), at b = arc(x, z), exactly when (x; y, z) is a triangle.
+// One wavefront per arc, lanes take 64 down entries at a time.  kFill 0 writes
+// the count of every arc; kFill 1 writes the (a, b) pairs at the arc's run in
+// schedule order (run0) and in arc-id order (run1), placed by the ballot prefix
+// so x ascends inside a run, and counts arcs whose run length disagrees with
+// cnt into *bad (writes never pass the run's end).
+template <int kFill>
+__global__ void __launch_bounds__(256) k_tri_pull(i64 mp, const i64* __restrict__ first,
+                                                  const int32_t* __restrict__ head, const int32_t* __restrict__ tail,
+                                                  const i64* __restrict__ down_off, const int32_t* __restrict__ down_arc,
+                                                  int32_t* cnt, const i64* __restrict__ run0,
+                                                  const i64* __restrict__ run1, int2* tri_ab, int32_t* tri_c,
+                                                  int2* tri_ab2, int32_t* tri_c2, unsigned int* bad) {
+  const int lane = threadIdx.x & 63;
+  const unsigned long long below = (1ull << lane) - 1ull;
+  const i64 wstride = (i64)gridDim.x * (blockDim.x >> 6);
+  for (i64 j = (i64)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); j < mp; j += wstride) {
+    const int32_t y = tail[j], z = head[j];
+    const i64 d0 = down_off[y], d1 = down_off[y + 1];
+    const int32_t want = kFill ? cnt[j] : 0;
+    const i64 q0 = kFill ? run0[j] : 0, q1 = kFill ? run1[j] : 0;
+    int32_t c = 0;
+    for (i64 base = d0; base < d1; base += 64) {
+      const i64 idx = base + lane;
+      int32_t a = -1;
+      i64 b = -1;
+      if (idx < d1) {
+        a = down_arc[idx];
+        b = row_find(head, (i64)a + 1, first[tail[a] + 1], z);
+      }
+      const unsigned long long m = __ballot(b >= 0);
+      if (kFill && b >= 0) {
+        const int32_t r = c + __popcll(m & below);
+        if (r < want) {
+          tri_ab[q0 + r] = make_int2(a, (int32_t)b);
+          tri_c[q0 + r] = (int32_t)j;
+          tri_ab2[q1 + r] = make_int2(a, (int32_t)b);
+          tri_c2[q1 + r] = (int32_t)j;
+        }
+      }
+      c += __popcll(m);
+    }
+    if (lane == 0) {
+      if (kFill) {
+        if (c != want) atomicAdd(bad, 1u);
+      } else {
+        cnt[j] = c;
+      }
+    }
+  }
+}
+
 // -------------------------------------------------------- negative-cycle scan
 
 // out[0] = number of arcs with up + dn < 0 (INF never negative), out[1] = the
@@ -892,13 +949,17 @@ __global__ void __launch_bounds__(64) k_query_path(const VInfo* __restrict__ vin
   nl = __shfl(nl, 0, 64);
   __syncthreads();
   int32_t* out = path + q * (i64)max_path;
-  int32_t L = 1;
+  // Without a negative cycle an unpacked path repeats no vertex except around
+  // zero-weight cycles; through a negative cycle the expansion can grow
+  // exponentially with the depth, so it stops after 4n + 16 vertices.
+  const long long lim = 4ll * n + 16;
+  long long L = 1;
   if (lane == 0 && max_path > 0) out[0] = nl ? legs[0] : rs;
-  for (int32_t l = 0; l < nl; l++) {
+  for (int32_t l = 0; l < nl && L <= lim; l++) {
     int32_t sp = 1;
     if (lane == 0) { stk[0] = legs[2 * l]; stk[1] = legs[2 * l + 1]; }
     __syncthreads();
-    while (sp > 0) {
+    while (sp > 0 && L <= lim) {
       sp--;
       const int32_t v = stk[2 * sp], w = stk[2 * sp + 1];
       const int32_t lo = min(v, w), hi = max(v, w);
@@ -947,8 +1008,13 @@ __global__ void __launch_bounds__(64) k_query_path(const VInfo* __restrict__ vin
   st = wave_or(st);
   if (lane == 0) {
     dist[q] = (int32_t)best;
-    path_len[q] = L;
-    if (L > max_path) st |= kStatusQueueFull;
+    if (L > lim) {
+      path_len[q] = -2;
+      st |= kStatusPathLimit;
+    } else {
+      path_len[q] = (int32_t)L;
+      if (L > max_path) st |= kStatusQueueFull;
+    }
     if (st) atomicOr(status, st);
   }
 }
@@ -989,21 +1055,105 @@ int fail(const std::exception& e) {
 
 }  // namespace
 
-// The arc-id-ordered copy of the triangle list (k_reorder_tri), built once.
-void build_arc_order(DevTopo& d) {
-  std::vector<i64> nbegin(d.mp);
-  std::vector<i64> pos;
-  i64 q = 0;
-  for (const LevelRange& r : d.lv) {
-    pos.resize(r.end - r.begin);
-    for (i64 p = r.begin; p < r.end; p++) pos[p - r.begin] = p;
-    std::sort(pos.begin(), pos.end(), [&](i64 a, i64 b) { return d.sched_host[a] < d.sched_host[b]; });
-    for (i64 p : pos) {
-      nbegin[p] = q;
-      q += d.pos_cnt[p];
+void nwcch::down_csr(const Topology& t, std::vector<i64>& off, std::vector<int32_t>& arc) {
+  off.assign(t.n + 1, 0);
+  for (i64 a = 0; a < t.mp; a++) off[t.head[a] + 1]++;
+  for (i64 v = 0; v < t.n; v++) off[v + 1] += off[v];
+  arc.resize(t.mp);
+  std::vector<i64> pos(off.begin(), off.end() - 1);
+  for (i64 a = 0; a < t.mp; a++) arc[pos[t.head[a]]++] = (int32_t)a;
+}
+
+namespace {
+
+// Start of every arc's run in the arc-id-ordered triangle list, by schedule
+// position: inside a level the arcs follow in id order (a stable counting sort
+// of the arcs by level; levels keep their triangle ranges).
+std::vector<i64> arc_order_starts(const DevTopo& d) {
+  std::vector<int32_t> lvl(d.mp), cnt(d.mp);
+  std::vector<i64> next(d.lv.size());
+  for (size_t L = 0; L < d.lv.size(); L++) {
+    next[L] = d.lv[L].begin;
+    for (i64 p = d.lv[L].begin; p < d.lv[L].end; p++) {
+      lvl[d.sched_host[p]] = (int32_t)L;
+      cnt[d.sched_host[p]] = d.pos_cnt[p];
     }
   }
+  std::vector<int32_t> order(d.mp);
+  for (i64 a = 0; a < d.mp; a++) order[next[lvl[a]]++] = (int32_t)a;
+  std::vector<i64> start(d.mp);
+  i64 q = 0;
+  for (i64 p = 0; p < d.mp; p++) {
+    start[order[p]] = q;
+    q += cnt[order[p]];
+  }
   if (q != d.T) throw std::runtime_error("triangle list does not cover the levels");
+  std::vector<i64> nbegin(d.mp);
+  for (i64 p = 0; p < d.mp; p++) nbegin[p] = start[d.sched_host[p]];
+  return nbegin;
+}
+
+// The triangle lists built on the device from the upward CSR and the schedule
+// (k_tri_pull): the schedule-order list with its arc ids and the arc-id-ordered
+// copy, without a host triangle list.  d.head and d.tail are resident.
+void build_index_device(DevTopo& d, const Topology& t) {
+  std::vector<i64> doff;
+  std::vector<int32_t> darc;
+  down_csr(t, doff, darc);
+  const std::vector<i64> nb = arc_order_starts(d);
+  std::vector<i64> run0(t.mp), run1(t.mp);
+  std::vector<int32_t> cnt(t.mp);
+  for (i64 p = 0; p < t.mp; p++) {
+    const int32_t a = t.sched_arc[p];
+    run0[a] = t.tri_off[p];
+    run1[a] = nb[p];
+    cnt[a] = (int32_t)(t.tri_off[p + 1] - t.tri_off[p]);
+  }
+  TempBuf<i64> first(t.first.data(), t.first.size()), dof(doff.data(), doff.size());
+  TempBuf<i64> r0(run0.data(), run0.size()), r1(run1.data(), run1.size());
+  TempBuf<int32_t> dar(darc.data(), darc.size()), cn(cnt.data(), cnt.size());
+  TempBuf<unsigned int> bad(1);
+  HIP_OK(hipMemset(bad.p, 0, sizeof(unsigned int)));
+  const size_t T = (size_t)std::max<i64>(t.triangles, 1);
+  d.tri_ab = dev_alloc<int2>(d, T);
+  d.tri_c = dev_alloc<int32_t>(d, T);
+  d.tri_ab2 = dev_alloc<int2>(d, T);
+  d.tri_c2 = dev_alloc<int32_t>(d, T);
+  if (t.mp > 0) {
+    const i64 blocks = std::min<i64>((t.mp + 3) / 4, (i64)1 << 16);
+    hipLaunchKernelGGL(k_tri_pull<1>, dim3((unsigned)blocks), dim3(256), 0, 0, t.mp, first.p, d.head, d.tail,
+                       dof.p, dar.p, cn.p, r0.p, r1.p, d.tri_ab, d.tri_c, d.tri_ab2, d.tri_c2, bad.p);
+    HIP_OK(hipGetLastError());
+  }
+  unsigned int hb = 0;
+  HIP_OK(hipMemcpy(&hb, bad.p, sizeof(hb), hipMemcpyDeviceToHost));
+  if (hb) throw std::runtime_error("device triangle index disagrees with the triangle counts");
+}
+
+}  // namespace
+
+std::vector<int32_t> nwcch::count_triangles_device(const Topology& t, int device) {
+  HIP_OK(hipSetDevice(device));
+  std::vector<int32_t> out(t.mp, 0);
+  if (t.mp == 0) return out;
+  std::vector<i64> doff;
+  std::vector<int32_t> darc;
+  down_csr(t, doff, darc);
+  TempBuf<i64> first(t.first.data(), t.first.size()), dof(doff.data(), doff.size());
+  TempBuf<int32_t> head(t.head.data(), t.head.size()), tail(t.arc_tail.data(), t.arc_tail.size());
+  TempBuf<int32_t> dar(darc.data(), darc.size()), cnt(t.mp);
+  const i64 blocks = std::min<i64>((t.mp + 3) / 4, (i64)1 << 16);
+  hipLaunchKernelGGL(k_tri_pull<0>, dim3((unsigned)blocks), dim3(256), 0, 0, t.mp, first.p, head.p, tail.p, dof.p,
+                     dar.p, cnt.p, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipMemcpy(out.data(), cnt.p, out.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+  return out;
+}
+
+// The arc-id-ordered copy of the triangle list (k_reorder_tri) from the
+// schedule-order list, built once.
+void build_arc_order(DevTopo& d) {
+  const std::vector<i64> nbegin = arc_order_starts(d);
   i64* dn = dev_upload(d, nbegin.data(), nbegin.size());
   d.tri_ab2 = dev_alloc<int2>(d, (size_t)std::max<i64>(d.T, 1));
   d.tri_c2 = dev_alloc<int32_t>(d, (size_t)std::max<i64>(d.T, 1));
@@ -1037,7 +1187,6 @@ extern "C" int nwcch_dev_create(const nwcch_topo* topo, int device, nwcch_dev** 
     d.sched_arc = dev_upload(d, t.sched_arc.data(), t.sched_arc.size());
     d.ae_off = dev_upload(d, t.ae_off.data(), t.ae_off.size());
     d.ae_idx = dev_upload(d, t.ae_idx.data(), t.ae_idx.size());
-    d.tri_ab = dev_upload(d, reinterpret_cast<const int2*>(t.tri_ab.data()), (size_t)t.triangles);
     std::vector<PosRec> pr(t.mp);
     for (i64 p = 0; p < t.mp; p++)
       pr[p] = PosRec{t.tri_off[p], t.sched_arc[p], (int32_t)(t.tri_off[p + 1] - t.tri_off[p])};
@@ -1048,13 +1197,21 @@ extern "C" int nwcch_dev_create(const nwcch_topo* topo, int device, nwcch_dev** 
     d.lv_tri0.resize(d.lv.size() + 1);
     for (size_t i = 0; i < d.lv.size(); i++) d.lv_tri0[i] = t.tri_off[d.lv[i].begin];
     d.lv_tri0[d.lv.size()] = t.triangles;
-    d.tri_c = dev_alloc<int32_t>(d, (size_t)std::max<i64>(t.triangles, 1));
-    if (t.mp > 0) {
-      const i64 blocks = std::min<i64>((t.mp + 255) / 256, 16384);
-      hipLaunchKernelGGL(k_fill_tri_c, dim3((unsigned)blocks), dim3(256), 0, 0, d.prec, t.mp, d.tri_c);
-      HIP_OK(hipGetLastError());
+    d.head = dev_upload(d, t.head.data(), t.head.size());
+    d.tail = dev_upload(d, t.arc_tail.data(), t.arc_tail.size());
+    if (t.has_tri) {
+      // the host's list in schedule order, its arc ids, and the arc-id-ordered copy
+      d.tri_ab = dev_upload(d, reinterpret_cast<const int2*>(t.tri_ab.data()), (size_t)t.triangles);
+      d.tri_c = dev_alloc<int32_t>(d, (size_t)std::max<i64>(t.triangles, 1));
+      if (t.mp > 0) {
+        const i64 blocks = std::min<i64>((t.mp + 255) / 256, 16384);
+        hipLaunchKernelGGL(k_fill_tri_c, dim3((unsigned)blocks), dim3(256), 0, 0, d.prec, t.mp, d.tri_c);
+        HIP_OK(hipGetLastError());
+      }
+      build_arc_order(d);   // the triangle-parallel sweep's default order
+    } else {
+      build_index_device(d, t);
     }
-    build_arc_order(d);   // the triangle-parallel sweep's default order
     std::vector<VInfo> vi(t.n);
     for (i64 x = 0; x < t.n; x++)
       vi[x] = VInfo{(int32_t)t.first[x], (int32_t)(t.first[x + 1] - t.first[x]), t.parent[x], t.vdepth[x]};
@@ -1096,9 +1253,7 @@ extern "C" int nwcch_dev_create(const nwcch_topo* topo, int device, nwcch_dev** 
     std::vector<int32_t> adep(t.mp);
     for (i64 a = 0; a < t.mp; a++) adep[a] = t.vdepth[t.head[a]];
     d.adep = dev_upload(d, adep.data(), adep.size());
-    d.head = dev_upload(d, t.head.data(), t.head.size());
     d.arc_pos = dev_upload(d, t.arc_pos.data(), t.arc_pos.size());
-    d.tail = dev_upload(d, t.arc_tail.data(), t.arc_tail.size());
     d.status = dev_alloc<uint32_t>(d, 1);
     HIP_OK(hipMemset(d.status, 0, sizeof(uint32_t)));
     d.scan = dev_alloc<unsigned long long>(d, 4);
@@ -1128,6 +1283,35 @@ extern "C" int nwcch_dev_info(const nwcch_dev* h, int64_t* out6) {
   if (!h || !out6) { set_error("null argument"); return NWCCH_ERR_ARG; }
   out6[0] = h->d.n; out6[1] = h->d.m; out6[2] = h->d.mp; out6[3] = h->d.T;
   out6[4] = h->d.depth; out6[5] = h->d.levels;
+  return NWCCH_OK;
+}
+
+extern "C" int nwcch_dev_export_triangles(const nwcch_dev* h, int32_t* tri_ab) {
+  if (!h || !tri_ab) { set_error("null argument"); return NWCCH_ERR_ARG; }
+  try {
+    HIP_OK(hipSetDevice(h->d.device));
+    if (h->d.T > 0)
+      HIP_OK(hipMemcpy(tri_ab, h->d.tri_ab, (size_t)h->d.T * sizeof(int2), hipMemcpyDeviceToHost));
+  } catch (const std::exception& e) {
+    return fail(e);
+  }
+  return NWCCH_OK;
+}
+
+extern "C" int nwcch_dev_export_sides(const nwcch_dev* h, int64_t* side_off, int32_t* side_top) {
+  if (!h) { set_error("null argument"); return NWCCH_ERR_ARG; }
+  if (!h->d.has_update) { set_error("nwcch_update_prepare was not called"); return NWCCH_ERR_ARG; }
+  try {
+    const DevTopo& d = h->d;
+    HIP_OK(hipSetDevice(d.device));
+    std::vector<i64> so(d.mp + 1);
+    HIP_OK(hipMemcpy(so.data(), d.side_off, so.size() * sizeof(i64), hipMemcpyDeviceToHost));
+    if (side_off) std::memcpy(side_off, so.data(), so.size() * sizeof(i64));
+    if (side_top && so[d.mp] > 0)
+      HIP_OK(hipMemcpy(side_top, d.side_top, (size_t)so[d.mp] * sizeof(int32_t), hipMemcpyDeviceToHost));
+  } catch (const std::exception& e) {
+    return fail(e);
+  }
   return NWCCH_OK;
 }
 

@@ -573,7 +573,7 @@ void map_arcs(Topology& t, const i64* off, const i32* tgt, i64 m) {
 
 // ---------------------------------------------- triangle index + schedule
 
-void build_index(Topology& t) {
+void build_meta(Topology& t) {
   const i64 n = t.n, mp = t.mp;
   if ((i64)t.first.size() != n + 1 || (i64)t.first[n] != mp)
     throw std::runtime_error("first/head sizes disagree");
@@ -600,13 +600,38 @@ void build_index(Topology& t) {
     maxlvl = std::max(maxlvl, t.vlevel[x]);
   }
   t.levels = maxlvl + 1;
-
-  // lower-triangle counts per arc (pull form: triangles whose top arc is c)
-  std::vector<i64> cnt(mp + 1, 0);
+  // T = sum over rows of d(d-1)/2 (every pair of upper neighbours is a triangle)
   t.triangles = 0;
   for (i64 x = 0; x < n; x++) {
+    const i64 d = t.first[x + 1] - t.first[x];
+    t.triangles += d * (d - 1) / 2;
+  }
+  t.arc_tail.resize(mp);
+  for (i64 x = 0; x < n; x++)
+    for (i64 a = t.first[x]; a < t.first[x + 1]; a++) t.arc_tail[a] = (i32)x;
+  // original arcs grouped by shortcut
+  t.ae_off.assign(mp + 1, 0);
+  t.ae_idx.clear();
+  if (!t.e_arc.empty()) {
+    if (t.m >= ((i64)1 << 30)) throw std::runtime_error("too many original arcs for int32 edge ids");
+    for (i64 k = 0; k < t.m; k++)
+      if (t.e_arc[k] >= 0) t.ae_off[t.e_arc[k] + 1]++;
+    for (i64 a = 0; a < mp; a++) t.ae_off[a + 1] += t.ae_off[a];
+    t.ae_idx.assign(t.ae_off[mp], 0);
+    std::vector<i32> pos(t.ae_off.begin(), t.ae_off.end() - 1);
+    for (i64 k = 0; k < t.m; k++) {
+      i64 a = t.e_arc[k];
+      if (a >= 0) t.ae_idx[pos[a]++] = (i32)((k << 1) | (t.e_up[k] ? 1 : 0));
+    }
+  }
+}
+
+// Lower-triangle count of every arc (pull form: triangles whose top arc is c),
+// by a merge pointer along row head[i] for each arc i of row x.
+std::vector<i32> count_triangles_host(const Topology& t) {
+  std::vector<i32> cnt(t.mp, 0);
+  for (i64 x = 0; x < t.n; x++) {
     const i64 s = t.first[x], e = t.first[x + 1];
-    t.triangles += (e - s) * (e - s - 1) / 2;
     for (i64 i = s; i < e; i++) {
       const i64 y = t.head[i];
       i64 j = t.first[y];
@@ -619,46 +644,64 @@ void build_index(Topology& t) {
       }
     }
   }
+  return cnt;
+}
+
+// Stable counting sort of ids by key (key < nkeys).
+static std::vector<i32> counting_sort(const std::vector<i32>& ids, const std::vector<i64>& key_of_id, i64 nkeys) {
+  std::vector<i64> start(nkeys + 1, 0);
+  for (i32 a : ids) start[key_of_id[a] + 1]++;
+  for (i64 k = 0; k < nkeys; k++) start[k + 1] += start[k];
+  std::vector<i32> out(ids.size());
+  for (i32 a : ids) out[start[key_of_id[a]]++] = a;
+  return out;
+}
+
+void build_schedule(Topology& t, const std::vector<i32>& cnt) {
+  const i64 mp = t.mp;
+  if ((i64)cnt.size() != mp) throw std::runtime_error("triangle counts do not cover the arcs");
+  i64 tot = 0, maxc = 0;
+  for (i32 c : cnt) { tot += c; maxc = std::max<i64>(maxc, c); }
+  if (tot != t.triangles) throw std::runtime_error("upward CSR is not chordal");
   // schedule: arcs grouped by the ET height of their tail, within a level by
-  // lower-triangle count, descending (big arcs first), ties by arc id
-  std::vector<i32> tail(mp);
-  for (i64 x = 0; x < n; x++)
-    for (i64 a = t.first[x]; a < t.first[x + 1]; a++) tail[a] = (i32)x;
-  t.sched_arc.resize(mp);
-  std::iota(t.sched_arc.begin(), t.sched_arc.end(), 0);
-  std::stable_sort(t.sched_arc.begin(), t.sched_arc.end(), [&](i32 a, i32 b) {
-    i32 la = t.vlevel[tail[a]], lb = t.vlevel[tail[b]];
-    if (la != lb) return la < lb;
-    return cnt[a] > cnt[b];
-  });
+  // lower-triangle count, descending (big arcs first), ties by arc id -- two
+  // stable counting sorts, by count descending and then by level
+  std::vector<i32> ids(mp);
+  std::iota(ids.begin(), ids.end(), 0);
+  std::vector<i64> key(mp);
+  for (i64 a = 0; a < mp; a++) key[a] = maxc - cnt[a];
+  ids = counting_sort(ids, key, maxc + 1);
+  for (i64 a = 0; a < mp; a++) key[a] = t.vlevel[t.arc_tail[a]];
+  t.sched_arc = counting_sort(ids, key, t.levels);
   t.arc_pos.resize(mp);
   for (i64 p = 0; p < mp; p++) t.arc_pos[t.sched_arc[p]] = (i32)p;
   t.tri_off.assign(mp + 1, 0);
   for (i64 p = 0; p < mp; p++) t.tri_off[p + 1] = t.tri_off[p] + cnt[t.sched_arc[p]];
   t.lv.assign(t.levels, LevelRange{});
-  {
-    i64 p = 0;
-    for (i32 L = 0; L < t.levels; L++) {
-      LevelRange r{};
-      r.begin = p;
-      // team classes are contiguous: counts descend within the level
-      int k = 0;
-      while (p < mp && t.vlevel[tail[t.sched_arc[p]]] == L) {
-        const int kc = team_class(cnt[t.sched_arc[p]]);
-        while (k <= kc) r.cls[k++] = p;
-        p++;
-      }
-      while (k <= kTeamClasses) r.cls[k++] = p;
-      r.end = p;
-      r.triangles = t.tri_off[r.end] - t.tri_off[r.begin];
-      t.lv[L] = r;
+  i64 p = 0;
+  for (i32 L = 0; L < t.levels; L++) {
+    LevelRange r{};
+    r.begin = p;
+    // team classes are contiguous: counts descend within the level
+    int k = 0;
+    while (p < mp && t.vlevel[t.arc_tail[t.sched_arc[p]]] == L) {
+      const int kc = team_class(cnt[t.sched_arc[p]]);
+      while (k <= kc) r.cls[k++] = p;
+      p++;
     }
+    while (k <= kTeamClasses) r.cls[k++] = p;
+    r.end = p;
+    r.triangles = t.tri_off[r.end] - t.tri_off[r.begin];
+    t.lv[L] = r;
   }
-  // scatter (a, b) pairs into schedule order, x ascending within an arc
+}
+
+// (a, b) pairs scattered into schedule order, x ascending within an arc.
+void fill_triangles_host(Topology& t) {
   t.tri_ab.assign(2 * t.triangles, 0);
-  std::vector<i64> fill(mp);
-  for (i64 a = 0; a < mp; a++) fill[a] = t.tri_off[t.arc_pos[a]];
-  for (i64 x = 0; x < n; x++) {
+  std::vector<i64> fill(t.mp);
+  for (i64 a = 0; a < t.mp; a++) fill[a] = t.tri_off[t.arc_pos[a]];
+  for (i64 x = 0; x < t.n; x++) {
     const i64 s = t.first[x], e = t.first[x + 1];
     for (i64 i = s; i < e; i++) {
       const i64 y = t.head[i];
@@ -672,21 +715,13 @@ void build_index(Topology& t) {
       }
     }
   }
-  // original arcs grouped by shortcut
-  t.ae_off.assign(mp + 1, 0);
-  if (!t.e_arc.empty()) {
-    if (t.m >= ((i64)1 << 30)) throw std::runtime_error("too many original arcs for int32 edge ids");
-    for (i64 k = 0; k < t.m; k++)
-      if (t.e_arc[k] >= 0) t.ae_off[t.e_arc[k] + 1]++;
-    for (i64 a = 0; a < mp; a++) t.ae_off[a + 1] += t.ae_off[a];
-    t.ae_idx.assign(t.ae_off[mp], 0);
-    std::vector<i32> pos(t.ae_off.begin(), t.ae_off.end() - 1);
-    for (i64 k = 0; k < t.m; k++) {
-      i64 a = t.e_arc[k];
-      if (a >= 0) t.ae_idx[pos[a]++] = (i32)((k << 1) | (t.e_up[k] ? 1 : 0));
-    }
-  }
-  t.arc_tail = std::move(tail);
+  t.has_tri = true;
+}
+
+void build_index(Topology& t) {
+  build_meta(t);
+  build_schedule(t, count_triangles_host(t));
+  fill_triangles_host(t);
 }
 
 void build_side_index(Topology& t) {

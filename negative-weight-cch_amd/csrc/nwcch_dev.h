@@ -2,7 +2,9 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -20,6 +22,7 @@ enum : uint32_t {
   kStatusBadWeight = 2u,   // an input weight equal to INT32_MIN
   kStatusBadQuery = 4u,    // a query vertex out of range
   kStatusQueueFull = 8u,   // partial-update frontier exceeded its capacity
+  kStatusPathLimit = 16u,  // a path unpacking passed 4n + 16 vertices (negative cycle)
 };
 
 // Packed per-vertex record for the query walk.
@@ -160,6 +163,42 @@ struct ProfScope {
 void set_error(const std::string& s);
 const Topology* topo_of(const struct ::nwcch_topo* t);
 Topology* topo_of_mut(struct ::nwcch_topo* t);
+
+// Binary search of `key` in head[lo, hi) (a sorted row); -1 if absent.
+__device__ __forceinline__ i64 row_find(const int32_t* __restrict__ head, i64 lo, const i64 end, int32_t key) {
+  i64 hi = end;
+  while (lo < hi) {
+    const i64 mid = (lo + hi) >> 1;
+    if (head[mid] < key) lo = mid + 1;
+    else hi = mid;
+  }
+  return (lo < end && head[lo] == key) ? lo : -1;
+}
+
+// Down lists of the upward CSR: for every vertex y the arcs (x, y) with x < y,
+// in ascending arc id (so ascending x).
+void down_csr(const Topology& t, std::vector<i64>& off, std::vector<int32_t>& arc);
+
+// A device allocation freed when it goes out of scope (index-building buffers).
+template <class T>
+struct TempBuf {
+  T* p = nullptr;
+  explicit TempBuf(size_t count) {
+    if (hipMalloc((void**)&p, std::max<size_t>(count, 1) * sizeof(T)) != hipSuccess) {
+      p = nullptr;
+      throw std::runtime_error("hipMalloc of an index-building buffer failed");
+    }
+  }
+  TempBuf(const T* host, size_t count) : TempBuf(count) {
+    if (count && hipMemcpy(p, host, count * sizeof(T), hipMemcpyHostToDevice) != hipSuccess)
+      throw std::runtime_error("upload of an index-building buffer failed");
+  }
+  ~TempBuf() {
+    if (p) (void)hipFree(p);
+  }
+  TempBuf(const TempBuf&) = delete;
+  TempBuf& operator=(const TempBuf&) = delete;
+};
 
 }  // namespace nwcch
 

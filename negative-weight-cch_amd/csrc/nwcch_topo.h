@@ -60,6 +60,8 @@ struct Topology {
   std::vector<i32> arc_pos;     // [mp] arc id -> schedule position
   std::vector<i64> tri_off;     // [mp+1] by schedule position
   std::vector<i32> tri_ab;      // [2*T] interleaved (a, b) = (arc(x,y), arc(x,z))
+  bool has_tri = false;         // tri_ab is filled (false: the device builds it)
+  int32_t index_device = -1;    // >= 0: the GPU that counted the triangles
   std::vector<LevelRange> lv;   // [levels]
 
   // original arcs grouped by shortcut (by arc id): entry = (edge << 1) | e_up
@@ -83,7 +85,15 @@ void nested_dissection(i64 n, const std::vector<i64>& noff, const std::vector<i3
 void symbolic(Topology& t, i64 n, const std::vector<i64>& noff, const std::vector<i32>& nbr,
               const i32* rank);
 void map_arcs(Topology& t, const i64* off, const i32* tgt, i64 m);
-void build_index(Topology& t);
+void build_index(Topology& t);   // build_meta + build_schedule + fill_triangles_host
+void build_meta(Topology& t);    // ET, levels, T, arc tails, original arcs by shortcut
+std::vector<i32> count_triangles_host(const Topology& t);
+void build_schedule(Topology& t, const std::vector<i32>& cnt);
+void fill_triangles_host(Topology& t);
 void build_side_index(Topology& t);
+
+// Device builders (device.hip): the lower-triangle count of every arc computed
+// on GPU `device` from the upward CSR alone.
+std::vector<i32> count_triangles_device(const Topology& t, int device);
 
 }  // namespace nwcch

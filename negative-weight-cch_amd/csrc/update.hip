@@ -204,6 +204,35 @@ __global__ void __launch_bounds__(1024) k_upd_persistent(UpdArgs A, int32_t L0, 
   if (st) atomicOr(A.status, st);
 }
 
+// The partial-update side lists (python/cch.py:309-324): arc c = (x, y) is a
+// lower side of one triangle with every other arc k of row x, whose top arc is
+// arc(min(y, head[k]), max(y, head[k])).  Entry order is k ascending, the order
+// a sweep over x and pairs i < k produces.  One wavefront per arc; a missing top
+// arc (a CSR that is not chordal) counts into *bad and writes c itself.
+__global__ void __launch_bounds__(256) k_side_fill(i64 mp, const i64* __restrict__ first,
+                                                   const int32_t* __restrict__ head, const int32_t* __restrict__ tail,
+                                                   const i64* __restrict__ side_off, int32_t* side_top,
+                                                   unsigned int* bad) {
+  const int lane = threadIdx.x & 63;
+  const i64 wstride = (i64)gridDim.x * (blockDim.x >> 6);
+  for (i64 c = (i64)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); c < mp; c += wstride) {
+    const int32_t x = tail[c], y = head[c];
+    const i64 s = first[x], e = first[x + 1];
+    const i64 o = side_off[c];
+    for (i64 k = s + lane; k < e; k += 64) {
+      if (k == c) continue;
+      const int32_t h = head[k];
+      const int32_t lo = min(y, h), hi = max(y, h);
+      i64 j = row_find(head, first[lo], first[lo + 1], hi);
+      if (j < 0) {
+        atomicAdd(bad, 1u);
+        j = c;
+      }
+      side_top[o + (k - s) - (k > c ? 1 : 0)] = (int32_t)j;
+    }
+  }
+}
+
 }  // namespace
 }  // namespace nwcch
 
@@ -253,15 +282,38 @@ extern "C" int nwcch_update_prepare(nwcch_dev* h, nwcch_topo* topo) {
     if (d.has_update) return NWCCH_OK;
     Topology& t = *topo_of_mut(topo);
     if (t.mp != d.mp || t.n != d.n) throw std::runtime_error("topology does not match the device handle");
-    build_side_index(t);
     HIP_OK(hipSetDevice(d.device));
+    if (t.has_tri) {
+      build_side_index(t);
+      d.side_off = up_upload(d, t.side_off.data(), t.side_off.size());
+      d.side_top = up_upload(d, t.side_top.data(), t.side_top.size());
+    } else {
+      // side lists built on the device: arc a of row x has deg(x) - 1 entries
+      std::vector<i64> so(t.mp + 1, 0);
+      for (i64 a = 0; a < t.mp; a++) {
+        const int32_t x = t.arc_tail[a];
+        so[a + 1] = so[a] + (t.first[x + 1] - t.first[x] - 1);
+      }
+      d.side_off = up_upload(d, so.data(), so.size());
+      d.side_top = up_alloc<int32_t>(d, (size_t)so[t.mp]);
+      TempBuf<i64> first(t.first.data(), t.first.size());
+      TempBuf<unsigned int> bad(1);
+      HIP_OK(hipMemset(bad.p, 0, sizeof(unsigned int)));
+      if (t.mp > 0) {
+        const i64 blocks = std::min<i64>((t.mp + 3) / 4, (i64)1 << 16);
+        hipLaunchKernelGGL(k_side_fill, dim3((unsigned)blocks), dim3(256), 0, 0, t.mp, first.p, d.head, d.tail,
+                           d.side_off, d.side_top, bad.p);
+        HIP_OK(hipGetLastError());
+      }
+      unsigned int hb = 0;
+      HIP_OK(hipMemcpy(&hb, bad.p, sizeof(hb), hipMemcpyDeviceToHost));
+      if (hb) throw std::runtime_error("upward CSR is not chordal (a side list lacks its top arc)");
+    }
     if (!d.arc_pos) d.arc_pos = up_upload(d, t.arc_pos.data(), t.arc_pos.size());
     std::vector<int32_t> alvl(t.mp);
     for (i64 a = 0; a < t.mp; a++) alvl[a] = t.vlevel[t.arc_tail[a]];
     d.arc_lvl_host = alvl;
     d.arc_lvl = up_upload(d, alvl.data(), alvl.size());
-    d.side_off = up_upload(d, t.side_off.data(), t.side_off.size());
-    d.side_top = up_upload(d, t.side_top.data(), t.side_top.size());
     std::vector<int32_t> cap(t.levels + 1, 0);
     for (int32_t L = 0; L < t.levels; L++) cap[L + 1] = cap[L] + (int32_t)(t.lv[L].end - t.lv[L].begin);
     d.lvl_cap_off = up_upload(d, cap.data(), cap.size());

@@ -14,8 +14,10 @@ LIB_PATH = Path(__file__).resolve().parent / "libnwcch.so"
 INF = 2147483647
 
 NWCCH_OK = 0
+NWCCH_TOPO_DEVICE_INDEX = 1
 STATUS_BITS = {1: "int32 overflow", 2: "INT32_MIN weight", 4: "query vertex out of range",
-               8: "update frontier full"}
+               8: "update frontier full or path longer than max_len",
+               16: "path unpacking passed 4n + 16 vertices (negative cycle)"}
 
 _lib = None
 
@@ -35,6 +37,7 @@ def lib():
             "nwcch_last_error": ([], C.c_char_p),
             "nwcch_order": ([i64, p, p, p, p, i32, p], C.c_int),
             "nwcch_topo_create": ([i64, p, p, p, C.POINTER(vp)], C.c_int),
+            "nwcch_topo_create_ex": ([i64, p, p, p, i32, C.c_int, C.POINTER(vp)], C.c_int),
             "nwcch_topo_from_csr": ([i64, p, p, i64, p, p, C.POINTER(vp)], C.c_int),
             "nwcch_topo_destroy": ([vp], None),
             "nwcch_topo_info": ([vp, p], C.c_int),
@@ -43,6 +46,8 @@ def lib():
             "nwcch_dev_create": ([vp, C.c_int, C.POINTER(vp)], C.c_int),
             "nwcch_dev_destroy": ([vp], None),
             "nwcch_dev_info": ([vp, p], C.c_int),
+            "nwcch_dev_export_triangles": ([vp, p], C.c_int),
+            "nwcch_dev_export_sides": ([vp, p, p], C.c_int),
             "nwcch_customize": ([vp, p, p, vp], C.c_int),
             "nwcch_negcycle_scan": ([p, i64, p, vp], C.c_int),
             "nwcch_negcycle_list": ([p, i64, p, i64, p, vp], C.c_int),
@@ -106,7 +111,8 @@ def exported_symbols():
             "nwcch_update_prepare", "nwcch_update", "nwcch_update_changed", "nwcch_query_strided",
             "nwcch_customize_batch", "nwcch_negcycle_scan_batch", "nwcch_batch_slab_width",
             "nwcch_set_tuning", "nwcch_get_tuning", "nwcch_query_multi", "nwcch_potential",
-            "nwcch_query_path"]
+            "nwcch_query_path", "nwcch_topo_create_ex", "nwcch_dev_export_triangles",
+            "nwcch_dev_export_sides"]
 
 
 # ------------------------------------------------------------ host topology
@@ -122,16 +128,23 @@ class Topology:
          self.n_loops, self.n_mapped) = (int(v) for v in info)
 
     @classmethod
-    def from_graph(cls, n, off, tgt, rank):
+    def from_graph(cls, n, off, tgt, rank, device_index=None):
+        """device_index = a GPU ordinal: that GPU builds the lower-triangle index
+        (NWCCH_TOPO_DEVICE_INDEX); the host then keeps no triangle list."""
         off = np.ascontiguousarray(off, np.int64)
         tgt = np.ascontiguousarray(tgt, np.int32)
         rank = np.ascontiguousarray(rank, np.int32)
         if off.shape != (n + 1,) or rank.shape != (n,):
             raise ValueError("off must be [n+1] and rank [n]")
         h = C.c_void_p()
-        check(lib().nwcch_topo_create(n, ptr(off), ptr(tgt), ptr(rank), C.byref(h)), "nwcch_topo_create")
+        if device_index is None:
+            check(lib().nwcch_topo_create(n, ptr(off), ptr(tgt), ptr(rank), C.byref(h)), "nwcch_topo_create")
+        else:
+            check(lib().nwcch_topo_create_ex(n, ptr(off), ptr(tgt), ptr(rank), NWCCH_TOPO_DEVICE_INDEX,
+                                             int(device_index), C.byref(h)), "nwcch_topo_create_ex")
         t = cls(h)
         t.rank = rank.copy()
+        t.device_index = device_index
         return t
 
     @classmethod
@@ -161,13 +174,15 @@ class Topology:
         return dict(first=first, head=head, parent=parent, e_arc=e_arc, e_up=e_up,
                     level=level, depth=depth)
 
-    def export_triangles(self):
+    def export_triangles(self, with_list=True):
+        """(sched_arc, tri_off, tri_ab[T, 2]); tri_ab is None without with_list
+        (a topology whose list lives on the device has only the first two)."""
         sched = np.empty(self.mp, np.int32)
         toff = np.empty(self.mp + 1, np.int64)
-        tab = np.empty(2 * self.triangles, np.int32)
+        tab = np.empty(2 * self.triangles, np.int32) if with_list else None
         check(lib().nwcch_topo_export_triangles(self._h, ptr(sched), ptr(toff), ptr(tab)),
               "nwcch_topo_export_triangles")
-        return sched, toff, tab.reshape(-1, 2)
+        return sched, toff, (tab.reshape(-1, 2) if with_list else None)
 
     def close(self):
         if getattr(self, "_h", None):

@@ -258,6 +258,24 @@ class DeviceTopology:
               "nwcch_negcycle_scan_batch")
         return out
 
+    # ------------------------------------------------------------- read-back
+    def export_triangles(self) -> np.ndarray:
+        """The device's triangle list in schedule order, [T, 2] int32 on the host."""
+        tab = np.empty(2 * self.triangles, np.int32)
+        with torch.cuda.device(self.device):
+            check(lib().nwcch_dev_export_triangles(self._h, ptr(tab)), "nwcch_dev_export_triangles")
+        return tab.reshape(-1, 2)
+
+    def export_sides(self):
+        """(side_off [mp+1] int64, side_top [2T] int32) of the update index."""
+        self.prepare_update()
+        so = np.empty(self.mp + 1, np.int64)
+        with torch.cuda.device(self.device):
+            check(lib().nwcch_dev_export_sides(self._h, ptr(so), None), "nwcch_dev_export_sides")
+            st = np.empty(int(so[-1]), np.int32)
+            check(lib().nwcch_dev_export_sides(self._h, None, ptr(st)), "nwcch_dev_export_sides")
+        return so, st
+
     # ---------------------------------------------------------- partial update
     def prepare_update(self):
         if not getattr(self, "_upd_ready", False):

@@ -718,7 +718,9 @@ i64 orc_customize_stop(i64 n, const i64* first, const i32* head, i64 m, const i6
  * neighbours of every vertex, ascending (the x of the lower triangles of an
  * arc (y, z) in the order of the reference's by_c groups).  tail[a]: lower end
  * of arc a.  Writes up to cap ranks to path; *len = path length (-1 if
- * unreachable).  Returns the distance (INF32 if unreachable). */
+ * unreachable, -2 if the unpacking passed 4n + 16 vertices, which only a
+ * metric with a negative cycle makes it do: it stops there).  Returns the
+ * distance (INF32 if unreachable). */
 i64 orc_query_path(i64 n, const i64* first, const i32* head, const i32* parent, const i32* tail,
                    const i64* lfirst, const i32* lnbr, const i32* up, const i32* dn, i32 rs, i32 rt,
                    i32* path, i64 cap, i64* len) {
@@ -777,10 +779,11 @@ i64 orc_query_path(i64 n, const i64* first, const i32* head, const i32* parent, 
     if (L < cap) path[L] = nl ? legs[0] : rs;
     L++;
     i64 sc = 64, sp = 0;
+    const i64 lim = 4 * n + 16;
     i32* st = (i32*)malloc((size_t)sc * 2 * sizeof(i32));
-    for (i64 l = 0; l < nl; l++) {
+    for (i64 l = 0; l < nl && L <= lim; l++) {
       st[0] = legs[2 * l]; st[1] = legs[2 * l + 1]; sp = 1;
-      while (sp > 0) {
+      while (sp > 0 && L <= lim) {
         sp--;
         const i32 v = st[2 * sp], w = st[2 * sp + 1];
         const i32 lo = v < w ? v : w, hi = v < w ? w : v;
@@ -807,7 +810,7 @@ i64 orc_query_path(i64 n, const i64* first, const i32* head, const i32* parent, 
     }
     free(st);
     free(legs);
-    *len = L;
+    *len = L > lim ? -2 : L;
   }
   free(df); free(dr); free(pf); free(pr); free(anc);
   return peak >= 0 ? best : INF32;

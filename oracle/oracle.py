@@ -184,7 +184,8 @@ class Structure:
         return self.query_ranks(up, dn, self.rank[np.asarray(s)], self.rank[np.asarray(t)], threads)
 
     def query_path(self, up, dn, s, t):
-        """(dist, [original vertex ids]) of python/cch.py:406-507; INF, [] if unreachable."""
+        """(dist, [original vertex ids]) of python/cch.py:406-507; INF, [] if unreachable;
+        (dist, None) if the unpacking passed 4n + 16 vertices (a negative cycle)."""
         if not hasattr(self, "_lower"):
             tail = np.repeat(np.arange(self.n, dtype=np.int32), np.diff(self.first)).astype(np.int32)
             order = np.argsort(self.head, kind="stable")
@@ -202,6 +203,8 @@ class Structure:
                                   _p(np.ascontiguousarray(dn, np.int32)), int(self.rank[s]), int(self.rank[t]),
                                   _p(path), cap, _p(ln))
         L = int(ln[0])
+        if L == -2:
+            return int(d), None        # unpacking passed 4n + 16 vertices: a negative cycle
         return int(d), ([] if L < 0 else self._inv[path[:L]].tolist())
 
 

@@ -98,3 +98,31 @@ def test_path_edge_cases():
     assert dt.status() & 8                                      # the truncated path
     dist, plen, path = dt.query_path(ud, rank[s], rank[t], max_len=8)
     assert dt.status() == 0 and inv[path[1, :5].cpu().numpy()].tolist() == [0, 1, 2, 3, 4]
+
+
+def test_path_limit_on_negative_cycle():
+    """A metric with negative cycles: every unpacking ends, the ones the oracle
+    stops at 4n + 16 vertices report length -2 and NWCCH_STATUS_PATH_LIMIT, the
+    others equal the oracle's paths."""
+    from test_oracle import negative_cycle_metric
+    g, rank, w = negative_cycle_metric()
+    print("case: path limit, negative-cycle metric", g.n, flush=True)
+    topo = _lib.Topology.from_graph(g.n, g.off, g.tgt, rank)
+    dt = DeviceTopology(topo)
+    ud = dt.customize(w)
+    dt.status(clear=True)
+    st = O.Structure(g.n, g.off, g.tgt, rank)
+    up, dn, _ = st.customize(w)
+    s, t = graphs.random_pairs(g.n, 32, seed=3)
+    dist, plen, path = (x.cpu().numpy() for x in dt.query_path(ud, rank[s], rank[t], max_len=4 * g.n + 16))
+    inv = np.argsort(rank)
+    stopped = 0
+    for i in range(len(s)):
+        dw, pw = st.query_path(up, dn, int(s[i]), int(t[i]))
+        assert int(dist[i]) == dw, i
+        if pw is None:
+            assert plen[i] == -2, i
+            stopped += 1
+        else:
+            assert inv[path[i, :plen[i]]].tolist() == pw if plen[i] >= 0 else pw == [], i
+    assert stopped > 0 and dt.status() & 16

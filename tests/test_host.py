@@ -76,6 +76,13 @@ def test_triangle_index_is_the_lower_triangle_set(c0):
     tail = np.repeat(np.arange(g.n), np.diff(first))
     lc = level[tail[c]]
     assert (level[tail[tab[:, 0]]] < lc).all() and (level[tail[tab[:, 1]]] < lc).all()
+    # schedule order: level ascending, then triangle count descending, then arc id
+    cnt = np.diff(toff)
+    keys = np.stack([level[tail[sched]], -cnt, sched], 1)
+    assert np.array_equal(np.lexsort(keys.T[::-1]), np.arange(t.mp))
+    # inside an arc's run the lower vertex x (hence the arc a = (x, y)) ascends
+    same = c[1:] == c[:-1]
+    assert (tab[1:, 0][same] > tab[:-1, 0][same]).all()
 
 
 def test_order_is_deterministic_and_valid():
@@ -92,6 +99,21 @@ def test_bad_order_is_rejected():
     rank[3] = 4
     with pytest.raises(_lib.NwcchError, match="permutation"):
         _lib.Topology.from_graph(g.n, g.off, g.tgt, rank)
+
+
+def test_device_index_flags_are_checked():
+    """Unknown flags are rejected before any GPU is touched; without a GPU the
+    device-built index fails loudly instead of falling back to the host."""
+    g = graphs.ev_grid(6, 0)
+    rank = np.arange(g.n, dtype=np.int32)
+    h = ctypes.c_void_p()
+    rc = _lib.lib().nwcch_topo_create_ex(g.n, _lib.ptr(g.off), _lib.ptr(g.tgt), _lib.ptr(rank), 6, 0,
+                                         ctypes.byref(h))
+    assert rc == -1 and b"unknown flags" in _lib.lib().nwcch_last_error()
+    import torch
+    if not torch.cuda.is_available():
+        with pytest.raises(_lib.NwcchError):
+            _lib.Topology.from_graph(g.n, g.off, g.tgt, rank, device_index=0)
 
 
 def test_non_chordal_csr_is_rejected():

@@ -354,3 +354,33 @@ def test_johnson_potential_equals_reference_cch_potential():
         # feasibility: reduced weights are non-negative
         tails = np.repeat(np.arange(g.n), np.diff(g.off))
         assert (g.w.astype(np.int64) + p[tails] - p[g.tgt] >= 0).all()
+
+
+def negative_cycle_metric():
+    """A shifted random graph of 200 vertices whose 10 re-weighted arcs close
+    negative cycles (305 arcs with up + down < 0), with its order."""
+    g = small_graphs()[11]
+    rng = np.random.default_rng(5)
+    idx = rng.choice(g.m, size=10, replace=False)
+    w = g.w.copy()
+    w[idx] = rng.integers(1, 100, size=10).astype(np.int32)
+    return g, order_of(g), w
+
+
+def test_path_unpacking_stops_on_negative_cycle():
+    """Through a negative cycle the shortcut expansion of python/cch.py:406-507
+    grows exponentially with the depth (unbounded, this metric's first query
+    ran past 30 s); the unpacking stops after 4n + 16 vertices and says so."""
+    g, rank, w = negative_cycle_metric()
+    st = O.Structure(g.n, g.off, g.tgt, rank)
+    up, dn, _ = st.customize(w)
+    assert (((up != O.INF) & (dn != O.INF) & (up.astype(np.int64) + dn < 0)).sum()) > 0
+    s, t = graphs.random_pairs(g.n, 32, seed=3)
+    stopped = 0
+    for a, b in zip(s, t):
+        d, path = st.query_path(up, dn, int(a), int(b))
+        if path is None:
+            stopped += 1
+        else:
+            assert len(path) <= 4 * g.n + 16 and (not path or (path[0] == a and path[-1] == b))
+    assert stopped > 0
