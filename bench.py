@@ -96,7 +96,7 @@ def event_ms(fn, reps):
     return [a.elapsed_time(b) for a, b in evs]
 
 
-def prepare(cfg, progress=None):
+def prepare(cfg, progress=None, device_index=None):
     from nwcch import _lib, graphs
     t0 = time.perf_counter()
     g = graphs.make_config(cfg, progress=progress)
@@ -107,7 +107,7 @@ def prepare(cfg, progress=None):
     t_order = time.perf_counter() - t0
     log(f"config {cfg}: nested dissection {t_order:.1f}s")
     t0 = time.perf_counter()
-    topo = _lib.Topology.from_graph(g.n, g.off, g.tgt, rank)
+    topo = _lib.Topology.from_graph(g.n, g.off, g.tgt, rank, device_index=device_index)
     t_topo = time.perf_counter() - t0
     log(f"config {cfg}: n={g.n} m={g.m} neg={g.negative_fraction():.3f} | m+={topo.mp} T={topo.triangles} "
         f"depth={topo.depth} levels={topo.levels} | gen {t_gen:.1f}s order {t_order:.1f}s topo {t_topo:.1f}s")
@@ -236,6 +236,7 @@ def plain(args, out):
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(g, rank, topo)
+    prep = preprocessing(g, rank, topo, host, dt)
 
     out.update({
         "value": topo.triangles / best,
@@ -273,8 +274,45 @@ def plain(args, out):
         "parity_vs_oracle": parity,
         "tuning": {"customize": tuning, "queries": qtune},
         "host": dict(host, nproc=os.cpu_count()),
+        "preprocessing": prep,
     })
     del dt
+
+
+def preprocessing(g, rank, topo, host, dt):
+    """The metric-independent index built on the host and on the GPU
+    (NWCCH_TOPO_DEVICE_INDEX): wall-clock of each step, and whether the
+    device-built triangle and side lists equal the host's."""
+    import numpy as np
+    import torch
+
+    from nwcch import _lib
+    from nwcch.device import DeviceTopology
+
+    def wall(fn):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = fn()
+        torch.cuda.synchronize()
+        return r, time.perf_counter() - t0
+
+    _, t_upd_h = wall(dt.prepare_update)
+    td, t_topo_d = wall(lambda: _lib.Topology.from_graph(g.n, g.off, g.tgt, rank, device_index=0))
+    dd, t_dev_d = wall(lambda: DeviceTopology(td, 0))
+    _, t_upd_d = wall(dd.prepare_update)
+    same = bool(np.array_equal(dd.export_triangles(), topo.export_triangles()[2]))
+    so_h, st_h = dt.export_sides()
+    so_d, st_d = dd.export_sides()
+    same &= bool(np.array_equal(so_h, so_d) and np.array_equal(st_h, st_d))
+    del dd, td
+    res = {"host_index": {"topology_s": host["topology_s"], "update_index_s": t_upd_h},
+           "device_index": {"topology_s": t_topo_d, "dev_create_s": t_dev_d, "update_index_s": t_upd_d},
+           "identical": same,
+           "what": "topology = symbolic contraction + arc map + triangle index + schedule (device index: "
+                   "triangle counts on the GPU, lists filled by dev_create); update_index = side lists"}
+    log(f"preprocessing: host topology {host['topology_s']:.3f} s + side lists {t_upd_h:.3f} s; device-index "
+        f"topology {t_topo_d:.3f} s + dev_create {t_dev_d:.3f} s + side lists {t_upd_d:.3f} s; identical {same}")
+    return res
 
 
 def cpu_baseline(g, rank, topo, reps=5):
@@ -463,9 +501,14 @@ def full_scale(args, out):
     from nwcch import graphs
     from nwcch.device import DeviceTopology
 
-    os.environ["NWCCH_PROGRESS"] = "1"   # a line per big nested-dissection cell
-    g, rank, topo, host = prepare(4, progress=lambda msg: log(f"config 4: {msg}"))
+    os.environ["NWCCH_PROGRESS"] = "1"   # a line per big nested-dissection cell and preprocessing phase
+    # the triangle index is counted and filled on the GPU
+    g, rank, topo, host = prepare(4, progress=lambda msg: log(f"config 4: {msg}"), device_index=0)
+    t0 = time.perf_counter()
     dt = DeviceTopology(topo, 0)
+    torch.cuda.synchronize()
+    host["dev_create_s"] = time.perf_counter() - t0
+    log(f"config 4: device topology created in {host['dev_create_s']:.1f} s (triangle lists filled on the GPU)")
     free_b, total_b = torch.cuda.mem_get_info()
     log(f"device topology uploaded ({(total_b - free_b) / 1e9:.1f} GB of HBM in use)")
     w = dt.weights(g.w)
@@ -613,6 +656,32 @@ def full_scale(args, out):
             res["query_check"] = {"pairs": nchk, "agree_with_johnson_dijkstra": okq, "johnson_s": t_j,
                                   "dijkstra_s": t_d, "negative_cycle": neg}
             log(f"config 4 queries vs Johnson+Dijkstra on {nchk}: {okq} (Johnson {t_j:.1f}s, Dijkstra {t_d:.1f}s)")
+    # the metric-independent index: the side lists on the GPU, then the same
+    # topology built with the host's triangle index for the time beside it and
+    # an entry-for-entry comparison of the triangle lists
+    t0 = time.perf_counter()
+    dt.prepare_update()
+    torch.cuda.synchronize()
+    t_side = time.perf_counter() - t0
+    prep = {"device_index": {"topology_s": host["topology_s"], "dev_create_s": host["dev_create_s"],
+                             "update_index_s": t_side}}
+    log(f"config 4: update side lists on the GPU in {t_side:.1f} s")
+    if not args.no_check:
+        from nwcch import _lib
+        t0 = time.perf_counter()
+        th = _lib.Topology.from_graph(g.n, g.off, g.tgt, rank)
+        t_host = time.perf_counter() - t0
+        sh, oh, tab = th.export_triangles()
+        sd, od, _ = topo.export_triangles(with_list=False)
+        same = bool(np.array_equal(sh, sd) and np.array_equal(oh, od))
+        del sh, oh, sd, od
+        th.close()
+        same &= bool(np.array_equal(dt.export_triangles(), tab))
+        del tab
+        prep["host_index"] = {"topology_s": t_host}
+        prep["identical"] = same
+        log(f"config 4: host-built index {t_host:.1f} s; device-built lists identical: {same}")
+    res["preprocessing"] = prep
     out["full_scale"] = res
 
 
