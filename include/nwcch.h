@@ -244,9 +244,6 @@ int nwcch_status(nwcch_dev* h, uint32_t* out, int clear, void* stream);
  *   "flat_min_tri"  single-metric sweep: a level whose heaviest arc has at least
  *                   this many lower triangles runs triangle-parallel (one lane
  *                   per triangle); 0 = every level (default), 2^30 = none.
- *   "flat_order"    triangle-parallel levels: 1 = triangles of a level in arc-id
- *                   order (arcs of one tail adjacent, default), 0 = in schedule
- *                   order (triangle count descending).
  *   "flat_store"    triangle-parallel levels: 0 = every arc folded with atomic
  *                   minima (default); 1 = arcs lying in one wavefront stored
  *                   with a plain store, 2 = with a non-temporal store.

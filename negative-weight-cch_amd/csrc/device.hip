@@ -5,12 +5,12 @@
 //   ud[mp]  int2 {up, dn}: the customized shortcut weights of one metric, pair per
 //           arc, so one 8-byte gather fetches both directions of an arc.
 //   tri_ab[T] int2 {a, b}: for each lower triangle (x; y, z) of arc c=(y,z) the arcs
-//           a=(x,y), b=(x,z); grouped by c in schedule order (level = ET height
-//           of the tail of c, then lower-triangle count descending), so a level
-//           is one contiguous slice streamed front to back.  tri_c[T]: the c of
-//           each triangle.
-//   prec[mp] PosRec {tri_begin, arc, tri_count}: the schedule, each record
-//           pointing at its slice.
+//           a=(x,y), b=(x,z), x ascending; grouped by c, the levels (ET height of
+//           the tail of c) in order and inside a level the arcs in id order, so
+//           a level is one contiguous slice streamed front to back and the arcs
+//           of one tail are adjacent.  tri_c[T]: the c of each triangle.
+//   prec[mp] PosRec {tri_begin, arc, tri_count}: the schedule (inside a level,
+//           lower-triangle count descending), each record pointing at its run.
 //   ae_off/ae_idx: the original arcs of each shortcut, entry (edge << 1) | up.
 //   vinfo[n] {first, deg, parent, depth}, adep[mp] = ET depth of head[a].
 //
@@ -253,31 +253,23 @@ __global__ void __launch_bounds__(256) k_customize_flat(i64 q0, i64 nt, const in
   if (st) atomicOr(status, st);
 }
 
-// The triangle list re-ordered inside every level by arc id (tail rank): arcs
-// of one tail are then adjacent, and nested dissection makes a run of tails
-// one region of the graph, so the gathers of a run share cache lines.  Built
-// on demand from the schedule-order list: each record's triangles are copied
-// to their new place (levels keep their ranges).
-__global__ void __launch_bounds__(256) k_reorder_tri(const PosRec* __restrict__ prec, const i64* __restrict__ nbegin,
-                                                     i64 mp, const int2* __restrict__ tri_ab, int2* tri_ab2,
-                                                     int32_t* tri_c2) {
+// The host's schedule-order triangle list copied into the device layout: inside
+// every level the arcs follow in arc id (tail rank) order, so the arcs of one
+// tail are adjacent, and nested dissection makes a run of tails one region of
+// the graph, whose gathers share cache lines.  src_rec[p] is record p's run in
+// the source list, start[p] its run in the device list (levels keep their
+// ranges); tri_c receives the arc of every triangle.
+__global__ void __launch_bounds__(256) k_reorder_tri(const PosRec* __restrict__ src_rec,
+                                                     const i64* __restrict__ start, i64 mp,
+                                                     const int2* __restrict__ src, int2* tri_ab, int32_t* tri_c) {
   const i64 stride = (i64)gridDim.x * blockDim.x;
   for (i64 p = (i64)blockIdx.x * blockDim.x + threadIdx.x; p < mp; p += stride) {
-    const PosRec r = prec[p];
-    const i64 nb = nbegin[p];
+    const PosRec r = src_rec[p];
+    const i64 nb = start[p];
     for (int32_t j = 0; j < r.tri_count; j++) {
-      tri_ab2[nb + j] = tri_ab[r.tri_begin + j];
-      tri_c2[nb + j] = r.arc;
+      tri_ab[nb + j] = src[r.tri_begin + j];
+      tri_c[nb + j] = r.arc;
     }
-  }
-}
-
-// tri_c[q] = the arc that triangle q improves (expanded from the schedule records)
-__global__ void __launch_bounds__(256) k_fill_tri_c(const PosRec* __restrict__ prec, i64 mp, int32_t* tri_c) {
-  const i64 stride = (i64)gridDim.x * blockDim.x;
-  for (i64 p = (i64)blockIdx.x * blockDim.x + threadIdx.x; p < mp; p += stride) {
-    const PosRec r = prec[p];
-    for (int32_t j = 0; j < r.tri_count; j++) tri_c[r.tri_begin + j] = r.arc;
   }
 }
 
@@ -288,17 +280,16 @@ __global__ void __launch_bounds__(256) k_fill_tri_c(const PosRec* __restrict__ p
 // a = arc(x, y) of y's down list come in ascending x; row x holds z after a
 // (rows are sorted), at b = arc(x, z), exactly when (x; y, z) is a triangle.
 // One wavefront per arc, lanes take 64 down entries at a time.  kFill 0 writes
-// the count of every arc; kFill 1 writes the (a, b) pairs at the arc's run in
-// schedule order (run0) and in arc-id order (run1), placed by the ballot prefix
-// so x ascends inside a run, and counts arcs whose run length disagrees with
-// cnt into *bad (writes never pass the run's end).
+// the count of every arc; kFill 1 writes the (a, b) pairs and the arc id at the
+// arc's run (run[j]), placed by the ballot prefix so x ascends inside a run,
+// and counts arcs whose run length disagrees with cnt into *bad (writes never
+// pass the run's end).
 template <int kFill>
 __global__ void __launch_bounds__(256) k_tri_pull(i64 mp, const i64* __restrict__ first,
                                                   const int32_t* __restrict__ head, const int32_t* __restrict__ tail,
                                                   const i64* __restrict__ down_off, const int32_t* __restrict__ down_arc,
-                                                  int32_t* cnt, const i64* __restrict__ run0,
-                                                  const i64* __restrict__ run1, int2* tri_ab, int32_t* tri_c,
-                                                  int2* tri_ab2, int32_t* tri_c2, unsigned int* bad) {
+                                                  int32_t* cnt, const i64* __restrict__ run, int2* tri_ab,
+                                                  int32_t* tri_c, unsigned int* bad) {
   const int lane = threadIdx.x & 63;
   const unsigned long long below = (1ull << lane) - 1ull;
   const i64 wstride = (i64)gridDim.x * (blockDim.x >> 6);
@@ -306,7 +297,7 @@ __global__ void __launch_bounds__(256) k_tri_pull(i64 mp, const i64* __restrict_
     const int32_t y = tail[j], z = head[j];
     const i64 d0 = down_off[y], d1 = down_off[y + 1];
     const int32_t want = kFill ? cnt[j] : 0;
-    const i64 q0 = kFill ? run0[j] : 0, q1 = kFill ? run1[j] : 0;
+    const i64 q0 = kFill ? run[j] : 0;
     int32_t c = 0;
     for (i64 base = d0; base < d1; base += 64) {
       const i64 idx = base + lane;
@@ -322,8 +313,6 @@ __global__ void __launch_bounds__(256) k_tri_pull(i64 mp, const i64* __restrict_
         if (r < want) {
           tri_ab[q0 + r] = make_int2(a, (int32_t)b);
           tri_c[q0 + r] = (int32_t)j;
-          tri_ab2[q1 + r] = make_int2(a, (int32_t)b);
-          tri_c2[q1 + r] = (int32_t)j;
         }
       }
       c += __popcll(m);
@@ -1066,7 +1055,7 @@ void nwcch::down_csr(const Topology& t, std::vector<i64>& off, std::vector<int32
 
 namespace {
 
-// Start of every arc's run in the arc-id-ordered triangle list, by schedule
+// Start of every arc's run in the device triangle list, by schedule
 // position: inside a level the arcs follow in id order (a stable counting sort
 // of the arcs by level; levels keep their triangle ranges).
 std::vector<i64> arc_order_starts(const DevTopo& d) {
@@ -1093,41 +1082,55 @@ std::vector<i64> arc_order_starts(const DevTopo& d) {
   return nbegin;
 }
 
-// The triangle lists built on the device from the upward CSR and the schedule
-// (k_tri_pull): the schedule-order list with its arc ids and the arc-id-ordered
-// copy, without a host triangle list.  d.head and d.tail are resident.
-void build_index_device(DevTopo& d, const Topology& t) {
+// The triangle list built on the device from the upward CSR and the schedule
+// (k_tri_pull), without a host triangle list.  start[p] = the run of schedule
+// position p; d.head and d.tail are resident.
+void build_index_device(DevTopo& d, const Topology& t, const std::vector<i64>& start) {
   std::vector<i64> doff;
   std::vector<int32_t> darc;
   down_csr(t, doff, darc);
-  const std::vector<i64> nb = arc_order_starts(d);
-  std::vector<i64> run0(t.mp), run1(t.mp);
+  std::vector<i64> run(t.mp);
   std::vector<int32_t> cnt(t.mp);
   for (i64 p = 0; p < t.mp; p++) {
     const int32_t a = t.sched_arc[p];
-    run0[a] = t.tri_off[p];
-    run1[a] = nb[p];
+    run[a] = start[p];
     cnt[a] = (int32_t)(t.tri_off[p + 1] - t.tri_off[p]);
   }
-  TempBuf<i64> first(t.first.data(), t.first.size()), dof(doff.data(), doff.size());
-  TempBuf<i64> r0(run0.data(), run0.size()), r1(run1.data(), run1.size());
+  TempBuf<i64> first(t.first.data(), t.first.size()), dof(doff.data(), doff.size()), rn(run.data(), run.size());
   TempBuf<int32_t> dar(darc.data(), darc.size()), cn(cnt.data(), cnt.size());
   TempBuf<unsigned int> bad(1);
   HIP_OK(hipMemset(bad.p, 0, sizeof(unsigned int)));
   const size_t T = (size_t)std::max<i64>(t.triangles, 1);
   d.tri_ab = dev_alloc<int2>(d, T);
   d.tri_c = dev_alloc<int32_t>(d, T);
-  d.tri_ab2 = dev_alloc<int2>(d, T);
-  d.tri_c2 = dev_alloc<int32_t>(d, T);
   if (t.mp > 0) {
     const i64 blocks = std::min<i64>((t.mp + 3) / 4, (i64)1 << 16);
     hipLaunchKernelGGL(k_tri_pull<1>, dim3((unsigned)blocks), dim3(256), 0, 0, t.mp, first.p, d.head, d.tail,
-                       dof.p, dar.p, cn.p, r0.p, r1.p, d.tri_ab, d.tri_c, d.tri_ab2, d.tri_c2, bad.p);
+                       dof.p, dar.p, cn.p, rn.p, d.tri_ab, d.tri_c, bad.p);
     HIP_OK(hipGetLastError());
   }
   unsigned int hb = 0;
   HIP_OK(hipMemcpy(&hb, bad.p, sizeof(hb), hipMemcpyDeviceToHost));
   if (hb) throw std::runtime_error("device triangle index disagrees with the triangle counts");
+}
+
+// The host's schedule-order list uploaded and copied into the device order
+// (k_reorder_tri); the upload is freed afterwards.
+void upload_host_list(DevTopo& d, const Topology& t, const std::vector<PosRec>& src_rec,
+                      const std::vector<i64>& start) {
+  TempBuf<int2> src(reinterpret_cast<const int2*>(t.tri_ab.data()), (size_t)t.triangles);
+  TempBuf<PosRec> rec(src_rec.data(), src_rec.size());
+  TempBuf<i64> st(start.data(), start.size());
+  const size_t T = (size_t)std::max<i64>(t.triangles, 1);
+  d.tri_ab = dev_alloc<int2>(d, T);
+  d.tri_c = dev_alloc<int32_t>(d, T);
+  if (t.mp > 0) {
+    const i64 blocks = std::min<i64>((t.mp + 255) / 256, 16384);
+    hipLaunchKernelGGL(k_reorder_tri, dim3((unsigned)blocks), dim3(256), 0, 0, rec.p, st.p, t.mp, src.p, d.tri_ab,
+                       d.tri_c);
+    HIP_OK(hipGetLastError());
+  }
+  HIP_OK(hipDeviceSynchronize());
 }
 
 }  // namespace
@@ -1144,28 +1147,10 @@ std::vector<int32_t> nwcch::count_triangles_device(const Topology& t, int device
   TempBuf<int32_t> dar(darc.data(), darc.size()), cnt(t.mp);
   const i64 blocks = std::min<i64>((t.mp + 3) / 4, (i64)1 << 16);
   hipLaunchKernelGGL(k_tri_pull<0>, dim3((unsigned)blocks), dim3(256), 0, 0, t.mp, first.p, head.p, tail.p, dof.p,
-                     dar.p, cnt.p, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
+                     dar.p, cnt.p, nullptr, nullptr, nullptr, nullptr);
   HIP_OK(hipGetLastError());
   HIP_OK(hipMemcpy(out.data(), cnt.p, out.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
   return out;
-}
-
-// The arc-id-ordered copy of the triangle list (k_reorder_tri) from the
-// schedule-order list, built once.
-void build_arc_order(DevTopo& d) {
-  const std::vector<i64> nbegin = arc_order_starts(d);
-  i64* dn = dev_upload(d, nbegin.data(), nbegin.size());
-  d.tri_ab2 = dev_alloc<int2>(d, (size_t)std::max<i64>(d.T, 1));
-  d.tri_c2 = dev_alloc<int32_t>(d, (size_t)std::max<i64>(d.T, 1));
-  if (d.mp > 0) {
-    const i64 blocks = std::min<i64>((d.mp + 255) / 256, 16384);
-    hipLaunchKernelGGL(k_reorder_tri, dim3((unsigned)blocks), dim3(256), 0, 0, d.prec, dn, d.mp, d.tri_ab, d.tri_ab2,
-                       d.tri_c2);
-    HIP_OK(hipGetLastError());
-  }
-  HIP_OK(hipDeviceSynchronize());
-  d.owned.erase(std::find(d.owned.begin(), d.owned.end(), (void*)dn));   // the starts are not kept
-  HIP_OK(hipFree(dn));
 }
 
 extern "C" int nwcch_dev_create(const nwcch_topo* topo, int device, nwcch_dev** out) {
@@ -1187,10 +1172,11 @@ extern "C" int nwcch_dev_create(const nwcch_topo* topo, int device, nwcch_dev** 
     d.sched_arc = dev_upload(d, t.sched_arc.data(), t.sched_arc.size());
     d.ae_off = dev_upload(d, t.ae_off.data(), t.ae_off.size());
     d.ae_idx = dev_upload(d, t.ae_idx.data(), t.ae_idx.size());
+    // schedule records; their runs are first those of the host's schedule-order
+    // list, then those of the device list (arc-id order inside levels)
     std::vector<PosRec> pr(t.mp);
     for (i64 p = 0; p < t.mp; p++)
       pr[p] = PosRec{t.tri_off[p], t.sched_arc[p], (int32_t)(t.tri_off[p + 1] - t.tri_off[p])};
-    d.prec = dev_upload(d, pr.data(), pr.size());
     d.pos_cnt.resize(t.mp);
     for (i64 p = 0; p < t.mp; p++) d.pos_cnt[p] = pr[p].tri_count;
     d.sched_host = t.sched_arc;
@@ -1199,19 +1185,11 @@ extern "C" int nwcch_dev_create(const nwcch_topo* topo, int device, nwcch_dev** 
     d.lv_tri0[d.lv.size()] = t.triangles;
     d.head = dev_upload(d, t.head.data(), t.head.size());
     d.tail = dev_upload(d, t.arc_tail.data(), t.arc_tail.size());
-    if (t.has_tri) {
-      // the host's list in schedule order, its arc ids, and the arc-id-ordered copy
-      d.tri_ab = dev_upload(d, reinterpret_cast<const int2*>(t.tri_ab.data()), (size_t)t.triangles);
-      d.tri_c = dev_alloc<int32_t>(d, (size_t)std::max<i64>(t.triangles, 1));
-      if (t.mp > 0) {
-        const i64 blocks = std::min<i64>((t.mp + 255) / 256, 16384);
-        hipLaunchKernelGGL(k_fill_tri_c, dim3((unsigned)blocks), dim3(256), 0, 0, d.prec, t.mp, d.tri_c);
-        HIP_OK(hipGetLastError());
-      }
-      build_arc_order(d);   // the triangle-parallel sweep's default order
-    } else {
-      build_index_device(d, t);
-    }
+    const std::vector<i64> start = arc_order_starts(d);
+    if (t.has_tri) upload_host_list(d, t, pr, start);
+    else build_index_device(d, t, start);
+    for (i64 p = 0; p < t.mp; p++) pr[p].tri_begin = start[p];
+    d.prec = dev_upload(d, pr.data(), pr.size());
     std::vector<VInfo> vi(t.n);
     for (i64 x = 0; x < t.n; x++)
       vi[x] = VInfo{(int32_t)t.first[x], (int32_t)(t.first[x + 1] - t.first[x]), t.parent[x], t.vdepth[x]};
@@ -1289,9 +1267,19 @@ extern "C" int nwcch_dev_info(const nwcch_dev* h, int64_t* out6) {
 extern "C" int nwcch_dev_export_triangles(const nwcch_dev* h, int32_t* tri_ab) {
   if (!h || !tri_ab) { set_error("null argument"); return NWCCH_ERR_ARG; }
   try {
-    HIP_OK(hipSetDevice(h->d.device));
-    if (h->d.T > 0)
-      HIP_OK(hipMemcpy(tri_ab, h->d.tri_ab, (size_t)h->d.T * sizeof(int2), hipMemcpyDeviceToHost));
+    const DevTopo& d = h->d;
+    HIP_OK(hipSetDevice(d.device));
+    if (d.T > 0) {
+      std::vector<int2> dev((size_t)d.T);
+      HIP_OK(hipMemcpy(dev.data(), d.tri_ab, (size_t)d.T * sizeof(int2), hipMemcpyDeviceToHost));
+      // the device list keeps arc-id order inside levels; hand it out in schedule order
+      const std::vector<i64> start = arc_order_starts(d);
+      i64 q = 0;
+      for (i64 p = 0; p < d.mp; p++) {
+        std::memcpy(tri_ab + 2 * q, dev.data() + start[p], (size_t)d.pos_cnt[p] * sizeof(int2));
+        q += d.pos_cnt[p];
+      }
+    }
   } catch (const std::exception& e) {
     return fail(e);
   }
@@ -1336,8 +1324,8 @@ extern "C" int nwcch_customize(nwcch_dev* h, const int32_t* w, int32_t* ud, void
         const i64 q0 = d.lv_tri0[i], nt = d.lv_tri0[i + 1] - q0;
         const i64 nb = (nt + 255) / 256;
         const i64 grid = d.flat_xcd ? 8 * ((nb + 7) / 8) : nb;  // XCD slices: 8 equal shares
-        const int2* tab = d.flat_order ? d.tri_ab2 : d.tri_ab;
-        const int32_t* tc = d.flat_order ? d.tri_c2 : d.tri_c;
+        const int2* tab = d.tri_ab;
+        const int32_t* tc = d.tri_c;
         if (d.flat_store == 1)
           hipLaunchKernelGGL(k_customize_flat<1>, dim3((unsigned)grid), dim3(256), 0, s, q0, nt, tab, tc, u, d.status,
                              d.flat_xcd);
@@ -1534,18 +1522,6 @@ extern "C" int nwcch_set_tuning(nwcch_dev* h, const char* key, int32_t value) {
     h->d.flat_store = value;
     return NWCCH_OK;
   }
-  if (k == "flat_order") {
-    if (value < 0 || value > 1) { set_error("flat_order is 0 or 1"); return NWCCH_ERR_ARG; }
-    if (value == 1 && !h->d.tri_ab2) {
-      try {
-        build_arc_order(h->d);
-      } catch (const std::exception& e) {
-        return fail(e);
-      }
-    }
-    h->d.flat_order = value;
-    return NWCCH_OK;
-  }
   if (k == "flat_xcd") {
     if (value < 0 || value > 1) { set_error("flat_xcd is 0 or 1"); return NWCCH_ERR_ARG; }
     h->d.flat_xcd = value;
@@ -1571,7 +1547,6 @@ extern "C" int nwcch_get_tuning(const nwcch_dev* h, const char* key, int32_t* va
   else if (k == "flat_min_tri") *value = h->d.flat_min_tri;
   else if (k == "flat_xcd") *value = h->d.flat_xcd;
   else if (k == "flat_store") *value = h->d.flat_store;
-  else if (k == "flat_order") *value = h->d.flat_order;
   else if (k == "query_deep_max") *value = h->d.query_deep_max;
   else if (k == "query_chunk") *value = h->d.query_chunk;
   else if (k == "upd_persistent_max") *value = h->d.upd_persistent_max;

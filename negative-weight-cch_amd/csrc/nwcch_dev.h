@@ -57,14 +57,12 @@ struct DevTopo {
 
   // customization
   int32_t* sched_arc = nullptr;      // [mp]
-  int2* tri_ab = nullptr;            // [T] in schedule order
+  int2* tri_ab = nullptr;            // [T] levels in order, arcs in id order inside a level
   int32_t* ae_off = nullptr;         // [mp+1] by arc id
   int32_t* ae_idx = nullptr;         // [#mapped original arcs]
   PosRec* prec = nullptr;            // [mp] by schedule position
   std::vector<int32_t> pos_cnt;      // [mp] host: triangles per schedule position
   int32_t* tri_c = nullptr;          // [T] arc improved by each triangle (triangle-parallel levels)
-  int2* tri_ab2 = nullptr;           // [T] the list inside each level in arc-id order (on demand)
-  int32_t* tri_c2 = nullptr;         // [T] its arcs
   std::vector<int32_t> sched_host;   // [mp] host: arc of each schedule position
   std::vector<i64> lv_tri0;          // [levels+1] host: first triangle of each level
   // tunables (nwcch_set_tuning) and the launch geometry derived from them
@@ -80,7 +78,6 @@ struct DevTopo {
                                              // triangles run triangle-parallel
   int32_t flat_store = 0;                    // triangle-parallel levels: 0 atomics only, 1/2
                                              // store arcs that lie in one wavefront (2: non-temporal)
-  int32_t flat_order = 1;                    // 1: arc-id order inside levels (tri_ab2), 0: count order
   int32_t flat_xcd = 0;                      // triangle-parallel levels: one contiguous
                                              // slice of the level per XCD (else interleaved)
   std::vector<LevelLaunch> launches;         // per level, for tri_per_lane

@@ -65,23 +65,20 @@ def tune_customize(dt: DeviceTopology, w, candidates=(1, 2, 4, 8, 16),
     dt.set_tuning("flat_min_tri", fbest)
     layouts = {}
     if fbest != FLAT_OFF:
-        # triangle order inside levels x store mode x XCD slicing
+        # store mode x XCD slicing of the triangle-parallel levels
         best_l = None
-        for order in (1, 0):
-            for store in (0, 1, 2):
-                for xcd in (0, 1):
-                    dt.set_tuning("flat_order", order)
-                    dt.set_tuning("flat_store", store)
-                    t = run("flat_xcd", xcd)
-                    layouts[f"order{order}/store{store}/xcd{xcd}"] = t
-                    if best_l is None or t < layouts[best_l[0]] * (1.0 - margin):
-                        best_l = (f"order{order}/store{store}/xcd{xcd}", order, store, xcd)
-        _, order, store, xcd = best_l
-        dt.set_tuning("flat_order", order)
+        for store in (0, 1, 2):
+            for xcd in (0, 1):
+                dt.set_tuning("flat_store", store)
+                t = run("flat_xcd", xcd)
+                layouts[f"store{store}/xcd{xcd}"] = t
+                if best_l is None or t < layouts[best_l[0]] * (1.0 - margin):
+                    best_l = (f"store{store}/xcd{xcd}", store, xcd)
+        _, store, xcd = best_l
         dt.set_tuning("flat_store", store)
         dt.set_tuning("flat_xcd", xcd)
     dt.check_status()
-    return {"tri_per_lane": best, "flat_min_tri": fbest, "flat_order": dt.get_tuning("flat_order"),
+    return {"tri_per_lane": best, "flat_min_tri": fbest,
             "flat_store": dt.get_tuning("flat_store"), "flat_xcd": dt.get_tuning("flat_xcd"), "ms": times,
             "flat_ms": {("off" if f == FLAT_OFF else str(f)): v for f, v in ftimes.items()},
             "layout_ms": layouts}

@@ -85,17 +85,14 @@ def test_road_config1_bit_exact():
         print("case: config 1 road graph, flat_min_tri", flat, flush=True)
         dt.set_tuning("flat_min_tri", flat)
         _check_customize(dt, st, g.w)
-    # triangle order inside levels x store mode x XCD slicing
+    # store mode x XCD slicing of the triangle-parallel levels
     dt.set_tuning("flat_min_tri", 0)
-    for order in (0, 1):
-        for store in (0, 1, 2):
-            for xcd in (0, 1):
-                print(f"case: config 1, order {order} store {store} xcd {xcd}", flush=True)
-                dt.set_tuning("flat_order", order)
-                dt.set_tuning("flat_store", store)
-                dt.set_tuning("flat_xcd", xcd)
-                _check_customize(dt, st, g.w)
-    dt.set_tuning("flat_order", 1)
+    for store in (0, 1, 2):
+        for xcd in (0, 1):
+            print(f"case: config 1, store {store} xcd {xcd}", flush=True)
+            dt.set_tuning("flat_store", store)
+            dt.set_tuning("flat_xcd", xcd)
+            _check_customize(dt, st, g.w)
     dt.set_tuning("flat_store", 0)
     dt.set_tuning("flat_xcd", 0)
     dt.set_tuning("flat_min_tri", FLAT_OFF)

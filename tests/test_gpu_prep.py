@@ -37,12 +37,12 @@ def _check_same(g, rank, what, updates=True):
     dh, dd = DeviceTopology(th), DeviceTopology(td)
     assert np.array_equal(dd.export_triangles(), tab)
     assert np.array_equal(dh.export_triangles(), tab)
-    # both triangle orders of the level sweep, from the device-built lists
+    # both level kernels read the device-built list
     ud_h = dh.customize(g.w)
-    for order in (1, 0):
-        dd.set_tuning("flat_order", order)
-        assert torch.equal(dd.customize(g.w), ud_h), order
-    dd.set_tuning("flat_order", 1)
+    assert torch.equal(dd.customize(g.w), ud_h)
+    dd.set_tuning("flat_min_tri", 1 << 30)
+    assert torch.equal(dd.customize(g.w), ud_h)
+    dd.set_tuning("flat_min_tri", 0)
     assert dd.status() == 0 and dh.status() == 0
     if not updates or th.mp == 0:
         return
