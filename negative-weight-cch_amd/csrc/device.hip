@@ -58,6 +58,15 @@ __device__ __forceinline__ uint32_t wave_or(uint32_t v) {
   return v;
 }
 
+// LDS writes of a wavefront made visible to its other lanes, without a
+// workgroup barrier: a workgroup of the query kernels holds several
+// independent queries, one per wavefront, whose loops end at different times.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Team classes of one level for R triangles per lane: class k (TS = 64 >> k
 // lanes) holds the arcs with (TS/2)*R < count <= TS*R (class 0 unbounded above,
 // class 6 down to 0).  A level's positions are sorted by count, descending, so
@@ -476,8 +485,8 @@ __device__ __forceinline__ void relax_tail(const int32_t* __restrict__ adep, con
 // and no reset pass over n is needed.  The t half finds the lowest common
 // ancestor with Euler-tour intervals; afterwards every common ancestor (depth
 // 0..depth(lca)) is a meeting candidate, min(df + dr) over them.
-template <int kPre>
-__global__ void __launch_bounds__(64) k_query(const VInfo* __restrict__ vinfo,
+template <int kPre, int W>
+__global__ void __launch_bounds__(64 * W) k_query(const VInfo* __restrict__ vinfo,
                                               const int2* __restrict__ euler,
                                               const int32_t* __restrict__ adep,
                                               const int2* __restrict__ ud_one, i64 ud_stride,
@@ -489,10 +498,14 @@ __global__ void __launch_bounds__(64) k_query(const VInfo* __restrict__ vinfo,
                                               int32_t maxlen, int32_t* __restrict__ dist,
                                               int32_t* __restrict__ scanned,
                                               uint32_t* status) {
-  extern __shared__ int32_t lds[];
+  extern __shared__ int32_t lds_all[];
+  // W wavefronts per workgroup, one query each, each with its own LDS labels;
   // query packing: wavefront i serves query order[i] (e.g. longest chains first)
-  const i64 q = order ? (i64)order[blockIdx.x] : (i64)blockIdx.x;
-  const int lane = threadIdx.x;
+  const i64 slot = (i64)blockIdx.x * W + (threadIdx.x >> 6);
+  if (slot >= nq) return;
+  const i64 q = order ? (i64)order[slot] : slot;
+  int32_t* lds = lds_all + (threadIdx.x >> 6) * 2 * maxlen;
+  const int lane = threadIdx.x & 63;
   const int half = lane >> 5;
   const int hl = lane & 31;
   if (q >= nq) return;
@@ -516,9 +529,9 @@ __global__ void __launch_bounds__(64) k_query(const VInfo* __restrict__ vinfo,
   const int32_t D = vi.depth;
   const int32_t Ds = __shfl(D, 0, 64), Dt = __shfl(D, 32, 64);
   for (int32_t i = hl; i <= D; i += 32) lab[i] = kInf;
-  __syncthreads();
+  wave_sync();
   if (hl == 0) lab[0] = 0;
-  __syncthreads();
+  wave_sync();
   const int32_t tin_s = euler[rs].x;
   int32_t lca_depth = -1;
   int32_t nscan = 0, p = 0;
@@ -579,7 +592,7 @@ __global__ void __launch_bounds__(64) k_query(const VInfo* __restrict__ vinfo,
           relax_tail<(kPre > 1)>(adep, ud, ud_stride, vi.first, hl + 32 * kPre, vi.deg, dx, half, D, lab, st);
       }
     }
-    __syncthreads();
+    wave_sync();
     if (active) {
       if (vi.parent < 0) {
         active = false;
@@ -619,8 +632,8 @@ __global__ void __launch_bounds__(64) k_query(const VInfo* __restrict__ vinfo,
 // vertices are in flight while one row is relaxed (P slots of 64 arcs, P
 // dividing 32).  The next chunk's ids and VInfo are loaded during the current
 // chunk.  Results are identical to k_query.
-template <int P>
-__global__ void __launch_bounds__(64) k_query_chunk(const VInfo* __restrict__ vinfo,
+template <int P, int W>
+__global__ void __launch_bounds__(64 * W) k_query_chunk(const VInfo* __restrict__ vinfo,
                                                     const int2* __restrict__ euler,
                                                     const int32_t* __restrict__ adep,
                                                     const int32_t* __restrict__ anc,
@@ -634,9 +647,12 @@ __global__ void __launch_bounds__(64) k_query_chunk(const VInfo* __restrict__ vi
                                                     int32_t* __restrict__ scanned,
                                                     uint32_t* status) {
   static_assert(32 % P == 0, "P must divide 32");
-  extern __shared__ int32_t lds[];
-  const i64 q = order ? (i64)order[blockIdx.x] : (i64)blockIdx.x;
-  const int lane = threadIdx.x;
+  extern __shared__ int32_t lds_all[];
+  const i64 slot = (i64)blockIdx.x * W + (threadIdx.x >> 6);
+  if (slot >= nq) return;
+  const i64 q = order ? (i64)order[slot] : slot;
+  int32_t* lds = lds_all + (threadIdx.x >> 6) * 2 * maxlen;
+  const int lane = threadIdx.x & 63;
   const int half = lane >> 5;
   const int hl = lane & 31;
   const int hb = half << 5;
@@ -660,9 +676,9 @@ __global__ void __launch_bounds__(64) k_query_chunk(const VInfo* __restrict__ vi
   const int32_t D = v0.depth;
   const int32_t Ds = __shfl(D, 0, 64), Dt = __shfl(D, 32, 64);
   for (int32_t i = hl; i <= D; i += 32) lab[i] = kInf;
-  __syncthreads();
+  wave_sync();
   if (hl == 0) lab[0] = 0;
-  __syncthreads();
+  wave_sync();
   const int32_t tin_s = euler[rs].x;
 
   // chunk 0: position 0 is the start vertex, positions 1..31 its ancestors
@@ -751,7 +767,7 @@ __global__ void __launch_bounds__(64) k_query_chunk(const VInfo* __restrict__ vi
         d2 = __shfl(nv.deg, hb + j + P - 32, 64);
       }
       load_row(slot, f2, (active && v2 >= 0) ? d2 : 0);
-      __syncthreads();
+      wave_sync();
     }
     cid = nid;
     cv = nv;
@@ -1400,21 +1416,31 @@ int launch_queries(nwcch_dev* h, const int32_t* ud, int64_t ud_stride, const int
     const size_t lds = (size_t)2 * maxlen * sizeof(int32_t);
     if (lds > 160 * 1024) throw std::runtime_error("elimination tree too deep for the LDS query kernel");
     ProfScope ps(d, s, kKQuery);
-    if (d.query_chunk && nq <= d.query_deep_max)
-      hipLaunchKernelGGL(k_query_chunk<4>, dim3((unsigned)nq), dim3(64), lds, s, d.vinfo, d.euler, d.adep, d.anc,
-                         reinterpret_cast<const int2*>(ud), (i64)ud_stride, (const long long*)ud_table, qmetric,
-                         n_metrics, s_rank, t_rank, (i64)nq, order, (int32_t)d.n, maxlen, dist, scanned,
-                         d.status);
-    else if (nq <= d.query_deep_max)
-      hipLaunchKernelGGL(k_query<4>, dim3((unsigned)nq), dim3(64), lds, s, d.vinfo, d.euler, d.adep,
-                         reinterpret_cast<const int2*>(ud), (i64)ud_stride, (const long long*)ud_table, qmetric,
-                         n_metrics, s_rank, t_rank, (i64)nq, order, (int32_t)d.n, maxlen, dist, scanned,
-                         d.status);
-    else
-      hipLaunchKernelGGL(k_query<1>, dim3((unsigned)nq), dim3(64), lds, s, d.vinfo, d.euler, d.adep,
-                         reinterpret_cast<const int2*>(ud), (i64)ud_stride, (const long long*)ud_table, qmetric,
-                         n_metrics, s_rank, t_rank, (i64)nq, order, (int32_t)d.n, maxlen, dist, scanned,
-                         d.status);
+    // wavefronts (queries) per workgroup: as many as asked for whose labels
+    // fit 64 KB of LDS per workgroup
+    int W = d.query_wpb;
+    while (W > 1 && (size_t)W * lds > 64 * 1024) W >>= 1;
+    const int kind = (d.query_chunk && nq <= d.query_deep_max) ? 2 : (nq <= d.query_deep_max ? 1 : 0);
+    const dim3 grid((unsigned)((nq + W - 1) / W)), block(64 * W);
+    const int2* u = reinterpret_cast<const int2*>(ud);
+    const long long* tab = (const long long*)ud_table;
+#define NWCCH_QUERY_LAUNCH(KERNEL, ...)                                                                      \
+  hipLaunchKernelGGL(KERNEL, grid, block, W * lds, s, __VA_ARGS__, u, (i64)ud_stride, tab, qmetric, n_metrics, \
+                     s_rank, t_rank, (i64)nq, order, (int32_t)d.n, maxlen, dist, scanned, d.status)
+    if (kind == 2) {
+      if (W == 4) NWCCH_QUERY_LAUNCH((k_query_chunk<4, 4>), d.vinfo, d.euler, d.adep, d.anc);
+      else if (W == 2) NWCCH_QUERY_LAUNCH((k_query_chunk<4, 2>), d.vinfo, d.euler, d.adep, d.anc);
+      else NWCCH_QUERY_LAUNCH((k_query_chunk<4, 1>), d.vinfo, d.euler, d.adep, d.anc);
+    } else if (kind == 1) {
+      if (W == 4) NWCCH_QUERY_LAUNCH((k_query<4, 4>), d.vinfo, d.euler, d.adep);
+      else if (W == 2) NWCCH_QUERY_LAUNCH((k_query<4, 2>), d.vinfo, d.euler, d.adep);
+      else NWCCH_QUERY_LAUNCH((k_query<4, 1>), d.vinfo, d.euler, d.adep);
+    } else {
+      if (W == 4) NWCCH_QUERY_LAUNCH((k_query<1, 4>), d.vinfo, d.euler, d.adep);
+      else if (W == 2) NWCCH_QUERY_LAUNCH((k_query<1, 2>), d.vinfo, d.euler, d.adep);
+      else NWCCH_QUERY_LAUNCH((k_query<1, 1>), d.vinfo, d.euler, d.adep);
+    }
+#undef NWCCH_QUERY_LAUNCH
     HIP_OK(hipGetLastError());
   } catch (const std::exception& e) {
     return fail(e);
@@ -1522,6 +1548,11 @@ extern "C" int nwcch_set_tuning(nwcch_dev* h, const char* key, int32_t value) {
     h->d.flat_store = value;
     return NWCCH_OK;
   }
+  if (k == "query_wpb") {
+    if (value != 1 && value != 2 && value != 4) { set_error("query_wpb is 1, 2 or 4"); return NWCCH_ERR_ARG; }
+    h->d.query_wpb = value;
+    return NWCCH_OK;
+  }
   if (k == "flat_xcd") {
     if (value < 0 || value > 1) { set_error("flat_xcd is 0 or 1"); return NWCCH_ERR_ARG; }
     h->d.flat_xcd = value;
@@ -1546,6 +1577,7 @@ extern "C" int nwcch_get_tuning(const nwcch_dev* h, const char* key, int32_t* va
   else if (k == "batch_tri_per_lane") *value = h->d.batch_tri_per_lane;
   else if (k == "flat_min_tri") *value = h->d.flat_min_tri;
   else if (k == "flat_xcd") *value = h->d.flat_xcd;
+  else if (k == "query_wpb") *value = h->d.query_wpb;
   else if (k == "flat_store") *value = h->d.flat_store;
   else if (k == "query_deep_max") *value = h->d.query_deep_max;
   else if (k == "query_chunk") *value = h->d.query_chunk;

@@ -71,6 +71,7 @@ struct DevTopo {
   int32_t upd_persistent_max = 2;            // updates of at most this many changed arcs run
                                              // in one persistent workgroup (mode auto)
   int32_t query_chunk = 0;                   // queries read ancestor chains 32 at a time
+  int32_t query_wpb = 1;                     // queries (wavefronts) per workgroup: 1, 2 or 4
                                              // (set at creation for deep hierarchies)
   int32_t query_deep_max = 32768;            // query launches up to this size prefetch 4 x 32
                                              // arcs of the next row, larger ones 32

@@ -101,15 +101,20 @@ def test_road_config1_bit_exact():
     d0, s0, _ = st.query(up, dn, s, t, threads=8)
     assert np.array_equal(dist.cpu().numpy(), d0)
     assert np.array_equal(sc.cpu().numpy().astype(np.int64), s0)
-    # the chain-walk query kernels (deep and shallow prefetch) give the same
-    # answers as the default chunked-ancestor kernel
-    dt.set_tuning("query_chunk", 0)
-    for deep in (32768, 0):
-        dt.set_tuning("query_deep_max", deep)
-        dist1, sc1 = dt.query(ud, rank[s], rank[t])
-        assert torch.equal(dist1, dist) and torch.equal(sc1, sc), deep
-    dt.set_tuning("query_chunk", 1)
+    # every query kernel (chain walk with deep or shallow prefetch, chunked
+    # ancestors) with 1, 2 or 4 queries per workgroup gives the same answers
+    chunk0 = dt.get_tuning("query_chunk")
+    for chunk in (0, 1):
+        for deep in (32768, 0):
+            for wpb in (1, 2, 4):
+                dt.set_tuning("query_chunk", chunk)
+                dt.set_tuning("query_deep_max", deep)
+                dt.set_tuning("query_wpb", wpb)
+                dist1, sc1 = dt.query(ud, rank[s], rank[t])
+                assert torch.equal(dist1, dist) and torch.equal(sc1, sc), (chunk, deep, wpb)
+    dt.set_tuning("query_chunk", chunk0)
     dt.set_tuning("query_deep_max", 32768)
+    dt.set_tuning("query_wpb", 1)
     p, neg = O.johnson(g.n, g.off, g.tgt, g.w)
     assert not neg
     dj = O.Dijkstra(g.n, g.off, g.tgt, g.w, p)
