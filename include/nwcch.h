@@ -254,6 +254,15 @@ int nwcch_status(nwcch_dev* h, uint32_t* out, int clear, void* stream);
  *                   triangles per lane; default 4, values 1..256.
  *   "batch_tri_per_lane"  batched sweep: wavefronts per arc, up to 16, likewise;
  *                   default 8, values 1..256.
+ *   "batch_order"   batched sweep: 1 = the arcs of each wave class taken in id
+ *                   order (a position table built per shape; the first call
+ *                   after a shape change synchronises the device), 0 = in
+ *                   schedule order (default).
+ *   "batch_xcd"     batched sweep: 1 = each XCD takes one contiguous slice of a
+ *                   wave class, 0 = workgroups interleaved (default).
+ *   "query_wpb"     queries (wavefronts) per workgroup of the query kernels:
+ *                   1 (default), 2 or 4, fewer where their labels would pass
+ *                   64 KB of LDS.
  *   "upd_persistent_max"  nwcch_update in mode 0 runs updates of at most this
  *                   many changed arcs in one persistent workgroup, larger ones
  *                   one launch per height; default 2.

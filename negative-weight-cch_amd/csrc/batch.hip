@@ -104,16 +104,22 @@ constexpr int kSlabWidth = 64;  // default metrics per slab
 struct BatchLaunch {
   i64 cls[kWaveClasses + 1];
   int32_t blk[kWaveClasses + 1];
+  int32_t xcd;  // 1: a class's blocks (a multiple of 8) renumbered so that XCD x takes the x-th slice
 };
 
 // One level of the batched sweep.  A wavefront holds KL metrics x (64 / KL)
 // triangle sub-lanes; WPA wavefronts of one block split an arc's triangles,
 // each lane keeping four (a, b) pairs -- eight gathers -- in flight per step;
 // sub-lanes meet through cross-lane minima and wavefronts through LDS.  Same
-// no-atomics argument as the single-metric level kernel.
+// no-atomics argument as the single-metric level kernel.  bpos (optional)
+// renumbers the schedule positions of each wave class (arc-id order), and with
+// L.xcd the workgroups of a class that land on one XCD (workgroups go to the 8
+// XCDs round robin) take one contiguous slice of it, so arcs of one tail,
+// which gather the same lower arcs, share that XCD's L2.
 template <int KL>
 __global__ void __launch_bounds__(1024) k_customize_batch_level(BatchLaunch L, int32_t slices,
                                                                 const PosRec* __restrict__ prec,
+                                                                const int32_t* __restrict__ bpos,
                                                                 const int2* __restrict__ tri_ab, int32_t K,
                                                                 i64 mp, int2* UD, uint32_t* status) {
   constexpr int S = 64 / KL;  // triangle sub-lanes per metric
@@ -124,7 +130,11 @@ __global__ void __launch_bounds__(1024) k_customize_batch_level(BatchLaunch L, i
   const int wlog = 4 - j;
   const int WPA = 1 << wlog;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int32_t local = b - L.blk[j];
+  int32_t local = b - L.blk[j];
+  if (L.xcd) {
+    const int32_t per = (L.blk[j + 1] - L.blk[j]) >> 3;
+    local = (local & 7) * per + (local >> 3);
+  }
   const int32_t slice = local % slices;
   const i64 grp = local / slices;
   const i64 p = L.cls[j] + grp * (16 >> wlog) + (wave >> wlog);
@@ -137,7 +147,7 @@ __global__ void __launch_bounds__(1024) k_customize_batch_level(BatchLaunch L, i
   int32_t up = kInf, dn = kInf;
   int32_t c = 0;
   if (live) {
-    const PosRec r = prec[p];
+    const PosRec r = prec[bpos ? (i64)bpos[p] : p];
     c = r.arc;
     if (chunk == 0 && ts == 0) {
       const int2 base = slab[(i64)c * KL];
@@ -225,6 +235,7 @@ __global__ void k_batch_scan_finish(long long* out, int32_t K) {
 // triangles.
 BatchLaunch batch_launch(const DevTopo& d, const LevelRange& r, int S, int32_t slices) {
   BatchLaunch L;
+  L.xcd = d.batch_xcd;
   const i64 R = d.batch_tri_per_lane;
   // positions are sorted by count descending: class j holds counts in
   // (limit(j+1), limit(j)] where limit(j) = (16 >> j) * S * R
@@ -249,10 +260,40 @@ BatchLaunch batch_launch(const DevTopo& d, const LevelRange& r, int S, int32_t s
     if (j < kWaveClasses) {
       const i64 arcs = L.cls[j + 1] - L.cls[j];
       const i64 per = 16 >> (4 - j);
-      b += (int32_t)(((arcs + per - 1) / per) * slices);
+      i64 nb = ((arcs + per - 1) / per) * slices;
+      if (L.xcd) nb = (nb + 7) / 8 * 8;  // padding blocks find no arc and only join the reduction
+      b += (int32_t)nb;
     }
   }
   return L;
+}
+
+// The schedule positions of every wave class in arc-id order (for S sub-lanes
+// and the current triangles per lane), uploaded once per shape.
+void ensure_bpos(DevTopo& d, int S) {
+  const long long key = ((long long)S << 32) | (unsigned)d.batch_tri_per_lane;
+  if (d.bpos && d.bpos_key == key) return;
+  std::vector<int32_t> pos(d.mp);
+  for (const LevelRange& r : d.lv) {
+    const BatchLaunch L = batch_launch(d, r, S, 1);
+    for (int j = 0; j < kWaveClasses; j++) {
+      for (i64 p = L.cls[j]; p < L.cls[j + 1]; p++) pos[p] = (int32_t)p;
+      std::sort(pos.begin() + L.cls[j], pos.begin() + L.cls[j + 1],
+                [&](int32_t a, int32_t b) { return d.sched_host[a] < d.sched_host[b]; });
+    }
+  }
+  if (d.bpos) {  // another shape: the previous array may still be read by queued launches
+    HIP_OK(hipDeviceSynchronize());
+    d.owned.erase(std::find(d.owned.begin(), d.owned.end(), (void*)d.bpos));
+    HIP_OK(hipFree(d.bpos));
+    d.bpos = nullptr;
+  }
+  int32_t* p = nullptr;
+  HIP_OK(hipMalloc((void**)&p, std::max<size_t>(pos.size(), 1) * sizeof(int32_t)));
+  d.owned.push_back(p);
+  if (!pos.empty()) HIP_OK(hipMemcpy(p, pos.data(), pos.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+  d.bpos = p;
+  d.bpos_key = key;
 }
 
 template <int KL>
@@ -260,7 +301,7 @@ void launch_level(const DevTopo& d, const LevelRange& r, int32_t slices, int32_t
   const BatchLaunch L = batch_launch(d, r, 64 / KL, slices);
   if (L.blk[kWaveClasses] == 0) return;
   hipLaunchKernelGGL(k_customize_batch_level<KL>, dim3(L.blk[kWaveClasses]), dim3(1024), 0, s, L, slices,
-                     d.prec, d.tri_ab, K, d.mp, UD, d.status);
+                     d.prec, d.batch_order ? d.bpos : nullptr, d.tri_ab, K, d.mp, UD, d.status);
 }
 
 }  // namespace
@@ -316,6 +357,7 @@ extern "C" int nwcch_customize_batch(nwcch_dev* h, const int32_t* W, int32_t K, 
       hipLaunchKernelGGL(k_base_batch, dim3((unsigned)blocks), dim3(256), 0, s, d.mp, K, sw_log2, slices, d.ae_off,
                          d.ae_idx, d.wt, ud, d.status);
     }
+    if (d.batch_order) ensure_bpos(d, 64 / KL);
     for (const LevelRange& r : d.lv) {
       if (r.end == r.begin || r.triangles == 0) continue;
       ProfScope ps(d, s, kKBatch);

@@ -1548,6 +1548,11 @@ extern "C" int nwcch_set_tuning(nwcch_dev* h, const char* key, int32_t value) {
     h->d.flat_store = value;
     return NWCCH_OK;
   }
+  if (k == "batch_order" || k == "batch_xcd") {
+    if (value < 0 || value > 1) { set_error(k + " is 0 or 1"); return NWCCH_ERR_ARG; }
+    (k == "batch_order" ? h->d.batch_order : h->d.batch_xcd) = value;
+    return NWCCH_OK;
+  }
   if (k == "query_wpb") {
     if (value != 1 && value != 2 && value != 4) { set_error("query_wpb is 1, 2 or 4"); return NWCCH_ERR_ARG; }
     h->d.query_wpb = value;
@@ -1575,6 +1580,8 @@ extern "C" int nwcch_get_tuning(const nwcch_dev* h, const char* key, int32_t* va
   const std::string k(key);
   if (k == "tri_per_lane") *value = h->d.tri_per_lane;
   else if (k == "batch_tri_per_lane") *value = h->d.batch_tri_per_lane;
+  else if (k == "batch_order") *value = h->d.batch_order;
+  else if (k == "batch_xcd") *value = h->d.batch_xcd;
   else if (k == "flat_min_tri") *value = h->d.flat_min_tri;
   else if (k == "flat_xcd") *value = h->d.flat_xcd;
   else if (k == "query_wpb") *value = h->d.query_wpb;

@@ -68,6 +68,8 @@ struct DevTopo {
   // tunables (nwcch_set_tuning) and the launch geometry derived from them
   int32_t tri_per_lane = kTriPerLane;        // single-metric level kernel
   int32_t batch_tri_per_lane = 8;            // batched level kernel
+  int32_t batch_order = 0;                   // 1: arcs of a wave class in id order (bpos)
+  int32_t batch_xcd = 0;                     // 1: each XCD takes a contiguous slice of a class
   int32_t upd_persistent_max = 2;            // updates of at most this many changed arcs run
                                              // in one persistent workgroup (mode auto)
   int32_t query_chunk = 0;                   // queries read ancestor chains 32 at a time
@@ -84,6 +86,8 @@ struct DevTopo {
   std::vector<LevelLaunch> launches;         // per level, for tri_per_lane
   int32_t launches_for = -1;
   int32_t* wt = nullptr;             // batched weights transposed to [m][K] (scratch)
+  int32_t* bpos = nullptr;           // batched sweep: schedule positions, id order inside each wave class
+  long long bpos_key = -1;           // (sub-lanes, triangles per lane) bpos was built for
   i64 wt_cap = 0;
   // query
   VInfo* vinfo = nullptr;            // [n]

@@ -102,7 +102,26 @@ def tune_batch(dt: DeviceTopology, W, slabs=(64, 16, 8), tris=(4, 8, 16), reps: 
                 raise AssertionError(f"slab={sw} batch_tri_per_lane={r} changed the customization")
     (sw, r) = min(times, key=times.get)
     dt.set_tuning("batch_tri_per_lane", r)
-    return {"slab": sw, "batch_tri_per_lane": r, "ms": {f"{a}/{b}": v for (a, b), v in times.items()}}
+    # arc order inside wave classes x XCD slices, at the chosen shape; a
+    # variant must beat the current choice by 2 %
+    out = torch.empty(dt.batch_shape(K, sw), dtype=torch.int32, device=dt.dev)
+    layouts, best = {}, None
+    for order in (0, 1):
+        for xcd in (0, 1):
+            dt.set_tuning("batch_order", order)
+            dt.set_tuning("batch_xcd", xcd)
+            t = _ms(lambda: dt.customize_batch(W, out=out, slab=sw), reps)
+            check = [DeviceTopology.metric_view(out, k) for k in (0, K - 1)]
+            if not all(torch.equal(a, b) for a, b in zip(ref, check)):
+                raise AssertionError(f"batch_order={order} batch_xcd={xcd} changed the customization")
+            layouts[f"order{order}/xcd{xcd}"] = t
+            if best is None or t < layouts[best[0]] * 0.98:
+                best = (f"order{order}/xcd{xcd}", order, xcd)
+    _, order, xcd = best
+    dt.set_tuning("batch_order", order)
+    dt.set_tuning("batch_xcd", xcd)
+    return {"slab": sw, "batch_tri_per_lane": r, "batch_order": order, "batch_xcd": xcd,
+            "ms": {f"{a}/{b}": v for (a, b), v in times.items()}, "layout_ms": layouts}
 
 
 def tune_queries(dt: DeviceTopology, ud, s_rank, t_rank, reps: int = 3) -> dict:

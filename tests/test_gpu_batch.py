@@ -48,3 +48,28 @@ def test_batch_equals_single(c0, K, slab):
             d0, _, _ = st.query(up, dn, s, t)
             assert np.array_equal(d.cpu().numpy(), d0)
     assert dt.status() == 0
+
+
+@pytest.mark.parametrize("order,xcd", [(1, 0), (0, 1), (1, 1)])
+def test_batch_layouts_equal_default(order, xcd):
+    """Arc-id order inside wave classes and XCD slices of the workgroups change
+    which workgroup takes which arc, never a value: config 1 with 24 metrics
+    (slab widths 8 and 64) equals the default layout bit for bit."""
+    import torch
+    g = graphs.make_config(1)
+    rank = _lib.order(g.n, g.off, g.tgt, g.x, g.y, 64)
+    print(f"case: batch layout order={order} xcd={xcd} on config 1", flush=True)
+    topo = _lib.Topology.from_graph(g.n, g.off, g.tgt, rank)
+    dt = DeviceTopology(topo)
+    W, _ = graphs.metric_family(g, 24, seed=3, hmax=800)
+    for slab in (8, 64):
+        ref = dt.customize_batch(W, slab=slab)
+        dt.set_tuning("batch_order", order)
+        dt.set_tuning("batch_xcd", xcd)
+        for tpl in (4, 16):   # two wave-class shapes: the id order is rebuilt for each
+            dt.set_tuning("batch_tri_per_lane", tpl)
+            assert torch.equal(dt.customize_batch(W, slab=slab), ref), (slab, tpl)
+        dt.set_tuning("batch_tri_per_lane", 8)
+        dt.set_tuning("batch_order", 0)
+        dt.set_tuning("batch_xcd", 0)
+    assert dt.status() == 0
