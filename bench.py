@@ -197,12 +197,21 @@ def plain(args, out):
     qfn = lambda: dt.query(ud, qs, qt, want_scanned=False, dist=dist, order=order)  # noqa: E731
     timed(qfn, 2)
     qms = statistics.median(event_ms(qfn, 5))
+    # algorithmic bytes of the batch: 8 per scanned arc (head + value), 12 per
+    # chain vertex (parent + first), 12 per query (s, t, answer)
+    _, qsc = dt.query(ud, qs, qt, order=order)
+    vdep = topo.export()["depth"].astype(np.int64)
+    chain = vdep[rank[qs_h]] + vdep[rank[qt_h]] + 2
+    qbytes = 8 * int(qsc.to(torch.int64).sum().item()) + 12 * int(chain.sum()) + 12 * args.queries
+    q_gbs = qbytes / (qms / 1000.0) / 1e9
     single = []
     for i in range(args.single):
         single.append(event_ms(lambda i=i: dt.query(ud, qs[i:i + 1], qt[i:i + 1], want_scanned=False), 1)[0])
     lat = np.array(single)
     t1 = timed(lambda: [dt.query(ud, qs[i:i + 1], qt[i:i + 1], want_scanned=False) for i in range(100)], 3) / 100
     dt.check_status()
+
+    updates = update_timing(args, g, rank, dt, w, ud)
 
     # the steps after the path: the CCH potential, and queries with unpacked paths
     pot = torch.empty(g.n, dtype=torch.int32, device=dt.dev)
@@ -235,7 +244,8 @@ def plain(args, out):
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(g, rank, topo)
+        ncq = min(args.queries, 20000)
+        cpu = cpu_baseline(g, rank, topo, queries=(qs_h[:ncq], qt_h[:ncq], dist[:ncq].cpu().numpy()))
     prep = preprocessing(g, rank, topo, host, dt)
 
     out.update({
@@ -265,10 +275,14 @@ def plain(args, out):
         "scan": {"ms": statistics.median(scan_ms),
                  "GB_s": 8 * topo.mp / (statistics.median(scan_ms) / 1000.0) / 1e9},
         "queries": {"batch": args.queries, "batch_ms": qms, "queries_per_s": args.queries / (qms / 1000.0),
+                    "roofline": {"bound": "hbm", "achieved": q_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                 "frac": q_gbs / HBM_PEAK_GBS, "algorithmic_bytes": qbytes,
+                                 "bytes_per_query": qbytes / args.queries},
                     "batch_latency_ms": qms,
                     "single_launch_queries_per_s": 1.0 / t1,
                     "single_p50_ms": float(np.percentile(lat, 50)),
                     "single_p99_ms": float(np.percentile(lat, 99))},
+        "updates": updates,
         "potential_ms": pot_ms,
         "path_queries": paths,
         "parity_vs_oracle": parity,
@@ -277,6 +291,60 @@ def plain(args, out):
         "preprocessing": prep,
     })
     del dt
+
+
+def update_timing(args, g, rank, dt, w, ud, reps=5):
+    """Partial re-customization (python/cch.py:349-402) of k = 1, 10, 1000
+    random original arcs changed by +-U(0, 500), each on a fresh copy of the
+    customized metric: GPU time per update (HIP events, host validation and
+    staging included), arcs evaluated, and beside it the oracle's C
+    restatement of update_edges on the same changes (the reference has no
+    compiled update)."""
+    import numpy as np
+    import torch
+
+    from oracle import oracle as O
+    rng = np.random.default_rng(17)
+    upd = None
+    if not args.no_cpu:
+        st = O.Structure(g.n, g.off, g.tgt, rank)
+        base_up, base_dn, _ = st.customize(g.w)
+        upd = O.Updater(st, g.w, base_up, base_dn)   # restored after every change set
+    res = {}
+    for k in (1, 10, 1000):
+        gpu_ms, cpu_ms, evals, same = [], [], [], True
+        for _ in range(reps):
+            idx = rng.choice(g.m, size=k, replace=False).astype(np.int64)
+            neww = np.clip(g.w[idx].astype(np.int64) + rng.integers(-500, 501, size=k), -(1 << 20), 1 << 20)
+            neww = neww.astype(np.int32)
+            w2, ud2 = w.clone(), ud.clone()
+            torch.cuda.synchronize()
+            a = torch.cuda.Event(enable_timing=True)
+            b = torch.cuda.Event(enable_timing=True)
+            a.record()
+            r = dt.update(w2, ud2, idx, neww)
+            b.record()
+            torch.cuda.synchronize()
+            gpu_ms.append(a.elapsed_time(b))
+            ev = int(r[0].item())
+            evals.append(ev)
+            if upd is not None:
+                t0 = time.perf_counter()
+                ev_c, _, _ = upd.update(idx, neww)
+                cpu_ms.append(1000 * (time.perf_counter() - t0))
+                got = ud2.cpu().numpy()
+                same &= ev_c == ev and np.array_equal(got[:, 0], upd.up) and np.array_equal(got[:, 1], upd.dn)
+                upd.update(idx, g.w[idx])
+        dt.status(clear=True)
+        res[str(k)] = {"gpu_ms_median": statistics.median(gpu_ms), "gpu_ms_max": max(gpu_ms),
+                       "evaluated_median": int(statistics.median(evals)),
+                       "cpu_port_ms_median": statistics.median(cpu_ms) if cpu_ms else None,
+                       "equal_to_oracle": same if upd is not None else None}
+        log(f"update k={k}: GPU {statistics.median(gpu_ms):.3f} ms, evaluated {int(statistics.median(evals))}"
+            + (f", oracle (1 core) {statistics.median(cpu_ms):.2f} ms, equal {same}" if cpu_ms else ""))
+    res["what"] = ("k random original arcs changed by +-U(0, 500); GPU: nwcch_update on a fresh copy, HIP events; "
+                   "CPU: the oracle's C restatement of update_edges, one thread (cores 1, kind port)")
+    return res
 
 
 def preprocessing(g, rank, topo, host, dt):
@@ -315,7 +383,22 @@ def preprocessing(g, rank, topo, host, dt):
     return res
 
 
-def cpu_baseline(g, rank, topo, reps=5):
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(g, rank, topo, reps=5, queries=None):
+    """The reference's C++ customization (serial and OpenMP) timed on the host;
+    with queries = (s, t, gpu_dist), also its ET query (Query::run, one thread)
+    on that sample, whose distances must equal the GPU's."""
+    import numpy as np
+
     from oracle import oracle as O
     if O.have_ref():
         ref = O.Reference(g.n, g.off, g.tgt, rank)
@@ -353,6 +436,24 @@ def cpu_baseline(g, rank, topo, reps=5):
         res["parallel"] = {"value": topo.triangles / tpm, "cores": thr, "ms": 1000 * tpm,
                            "sample": "cpp/cch.cpp CCH::customize_parallel (OpenMP, level buckets), "
                                      f"median of {reps}"}
+        if queries is not None:
+            qs, qt, gd = queries
+            up, dn = ref.customize(g.w)
+            ref.query(up, dn, qs[:100], qt[:100])   # warm-up, not timed
+            tq = []
+            for _ in range(3):
+                t0 = time.perf_counter()
+                d = ref.query(up, dn, qs, qt)
+                tq.append(time.perf_counter() - t0)
+            tqm = statistics.median(tq)
+            d = np.where(np.isinf(d), O.INF, d).astype(np.int64)
+            same = bool(np.array_equal(d, gd.astype(np.int64)))
+            log(f"cpu baseline (reference queries): {len(qs)} in {1000 * tqm:.1f} ms; equal to the GPU's: {same}")
+            res["queries"] = {"value": len(qs) / tqm, "unit": "queries/s", "cores": 1, "kind": "reference",
+                              "sample": f"cpp/cch.cpp Query::run, {len(qs)} random s-t pairs of the GPU batch, "
+                                        "median of 3",
+                              "ms": 1000 * tqm, "equal_to_gpu": same}
+    res["cpu_model"] = cpu_model()
     return res
 
 
