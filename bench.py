@@ -7,15 +7,21 @@ customization of one metric + the negative-cycle scan (the plain one-metric
 form), inputs resident in HBM; value = lower triangles / step time.
 
 The same run also reports, as extra fields:
-  * ET queries on that metric: 100,000 random s-t pairs in one launch, and
-    single-query launches (p50/p99 latency);
+  * ET queries on that metric: 100,000 random s-t pairs in one launch (with
+    their algorithmic bytes against the HBM peak), and single-query launches
+    (p50/p99 latency);
+  * partial updates of 1, 10 and 1,000 arcs, the CCH potential, 2,000 paths
+    with their vertices, and the metric-independent index built on the host
+    and on the GPU;
+  * the serving loop on the 1.1M-point graph (16 resident metrics, Poisson
+    queries at 0.2-2 M/s interleaved with updates), p50/p99;
   * the batched form: 64 metrics (slopes 5-40, own height fields, one with an
     injected 3-arc cycle of total -1) customized in one level sweep on the
     321,000-point graph (seed 2), against 64 plain single-metric customizations;
   * parity of every GPU result checked bit-for-bit against the CPU oracle;
-  * the CPU baseline: the reference's own C++ customization (cpp/cch.cpp,
-    compiled from its sources) on the host, or the oracle port where the
-    reference was not built.
+  * the CPU baseline: the reference's own C++ customization and ET queries
+    (cpp/cch.cpp, compiled from its sources) on the host, or the oracle port
+    where the reference was not built.
 
 Prints ONE JSON line.
 """

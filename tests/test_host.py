@@ -138,3 +138,13 @@ def test_cch_nb_host_prerequisites(c0):
     assert np.array_equal(parent, st.parent) and np.array_equal(e_arc, st.e_arc)
     assert np.array_equal(e_up, st.e_up)
     assert cch_nb.count_triangles(g.n, first, head) == st.triangles
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py runs on one GPU; asked for more it exits with a message and no
+    JSON line."""
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, str(ROOT / "bench.py"), "--gpus", "2"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode != 0 and "one GPU" in r.stderr and r.stdout.strip() == ""
