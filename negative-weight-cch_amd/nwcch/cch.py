@@ -59,7 +59,12 @@ def _int_weights(w) -> np.ndarray:
 
 
 class CCH:
-    def __init__(self, g, xy=None, leaf=64, method="flow", log=print, order=None, device=None):
+    def __init__(self, g, xy=None, leaf=64, method="flow", log=print, order=None, device=None,
+                 gpu_index=False):
+        """The reference's constructor (python/cch.py:112); device = the GPU
+        ordinal (default: the current one); gpu_index=True builds the triangle
+        index and the update side lists on that GPU (NWCCH_TOPO_DEVICE_INDEX)
+        instead of the host -- the same arrays, faster on big graphs."""
         t0 = time.perf_counter()
         n, off, tgt, _, gxy = _as_csr(g)
         self.n = n
@@ -81,7 +86,12 @@ class CCH:
         self.rank = rank
         self.t_order = time.perf_counter() - t0
         t0 = time.perf_counter()
-        self.topo = _lib.Topology.from_graph(n, off, tgt, rank)
+        if gpu_index:
+            import torch
+            dev_i = torch.cuda.current_device() if device is None else int(device)
+            self.topo = _lib.Topology.from_graph(n, off, tgt, rank, device_index=dev_i)
+        else:
+            self.topo = _lib.Topology.from_graph(n, off, tgt, rank)
         ex = self.topo.export()
         self.t_contract = time.perf_counter() - t0
         self.first, self.head, self.parent = ex["first"], ex["head"], ex["parent"]

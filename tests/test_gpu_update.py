@@ -142,17 +142,19 @@ def _golden():
         return {k: d[k] for k in d.files}
 
 
+@pytest.mark.parametrize("gpu_index", [False, True])
 @pytest.mark.parametrize("name", ["grid", "small0", "small1", "small2"])
-def test_dropin_matches_reference_python_vectors(name):
+def test_dropin_matches_reference_python_vectors(name, gpu_index):
     """The CCH drop-in on the GPU against vectors the reference's python/cch.py
     wrote: customized values, every update's evaluated count and changed list,
-    the final values, and both witness forms of a negative cycle."""
+    the final values, and both witness forms of a negative cycle -- with the
+    triangle index and side lists built on the host or on the GPU."""
     G = _golden()
-    print(f"case: golden vectors {name}", flush=True)
+    print(f"case: golden vectors {name} gpu_index={gpu_index}", flush=True)
     off, tgt, w, rank = (G[f"{name}/{k}"] for k in ("off", "tgt", "w", "rank"))
     n = len(rank)
     g = graphs.RoadGraph(n, off, tgt, w.astype(np.int32), np.zeros(n), np.zeros(n))
-    c = CCH(g, order=rank, log=lambda *_: None)
+    c = CCH(g, order=rank, log=lambda *_: None, gpu_index=gpu_index)
     assert c.customize(g) == []
     ud = c.ud.cpu().numpy()
     assert np.array_equal(ud[:, 0], G[f"{name}/up0"]) and np.array_equal(ud[:, 1], G[f"{name}/dn0"])
@@ -165,7 +167,7 @@ def test_dropin_matches_reference_python_vectors(name):
     ud = c.ud.cpu().numpy()
     assert np.array_equal(ud[:, 0], G[f"{name}/up_end"]) and np.array_equal(ud[:, 1], G[f"{name}/dn_end"])
     gn = g.with_weights(G[f"{name}/neg_w"])
-    c2 = CCH(gn, order=rank, log=lambda *_: None)
+    c2 = CCH(gn, order=rank, log=lambda *_: None, gpu_index=gpu_index)
     assert c2.customize(gn) == G[f"{name}/neg_witnesses"].tolist()
     assert c2.customize(gn, stop_on_negative_cycle=True) == G[f"{name}/neg_stop_witness"].tolist()
 
