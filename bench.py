@@ -498,6 +498,16 @@ def batched(args, out):
     timed(step, 2)
     dt.check_status(negative_cycle=True)
     t_b = timed(step, args.batch_steps)
+    t_bg = None
+    if not args.no_graph:
+        from nwcch.device import StepGraph
+        bgraph = StepGraph(step)
+        timed(bgraph.replay, 1)
+        t_bg = timed(bgraph.replay, args.batch_steps)
+        dt.check_status(negative_cycle=True)
+    t_b_eager = t_b
+    if t_bg is not None:
+        t_b = min(t_b, t_bg)
     ud1 = torch.empty((topo.mp, 2), dtype=torch.int32, device=dt.dev)
     s1 = torch.empty(2, dtype=torch.int64, device=dt.dev)
     rows = [Wd[k].contiguous() for k in range(K)]
@@ -537,6 +547,7 @@ def batched(args, out):
         "workload": f"config 2: n={g.n} m={g.m} m+={topo.mp} T={topo.triangles} depth={topo.depth}; "
                     f"{K} metrics in one level sweep + per-metric scan",
         "metrics": K, "ms_per_step": 1000 * t_b, "triangles_per_s": K * topo.triangles / t_b,
+        "step_ms": {"eager": 1000 * t_b_eager, "hip_graph": None if t_bg is None else 1000 * t_bg},
         "plain_ms_for_all_metrics": 1000 * t_s, "plain_triangles_per_s": K * topo.triangles / t_s,
         "speedup_vs_plain": t_s / t_b,
         "roofline": {"kernel": "k_customize_batch_level", "bound": "hbm", "achieved": b_ach,
