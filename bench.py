@@ -583,6 +583,9 @@ def serving(args, out):
         if not srv.add_metric(k, W[k]):
             raise RuntimeError(f"metric {k} was not admitted")
         w0[k] = W[k].copy()
+    # warm-up: a short query-only run (first launches, lazy allocations); not reported
+    warm = serve.poisson_workload(g, K, rate=args.serve_rates[0], duration=0.05, seed=29, update_rate=0)
+    srv.run(warm, max_seconds=60, max_inflight=args.serve_inflight)
     runs = []
     all_reqs = []
     for i, rate in enumerate(args.serve_rates):

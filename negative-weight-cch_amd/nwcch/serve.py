@@ -70,6 +70,7 @@ class _Metric:
     version: int = 0
     history: list = dataclasses.field(default_factory=list)  # accepted updates, in order
     busy_update: object = None     # in-flight update state
+    res_host: torch.Tensor | None = None   # pinned int64[4]: the verdict of the running update
 
 
 class _Launch:
@@ -139,6 +140,8 @@ class Server:
         slot = self.free_table.pop(0)
         self.h_table[slot] = ud.data_ptr()
         m = _Metric(mid, slot, wd, ud, np.asarray(w, np.int32).copy())
+        # allocated once: a pinned allocation per update would stall the serving loop
+        m.res_host = torch.empty(4, dtype=torch.int64, pin_memory=True)
         if self.snapshot:
             m.w_spare, m.ud_spare = torch.empty_like(wd), torch.empty_like(ud)
         self.metrics[mid] = m
@@ -308,7 +311,7 @@ class Server:
                 res = self.dt.update(m.w_spare, m.ud_spare, r.edges, r.weights, stream=s)
             else:
                 res = self.dt.update(m.w, m.ud, r.edges, r.weights, stream=s)
-            host = torch.empty(4, dtype=torch.int64, pin_memory=True)
+            host = m.res_host      # one update of a metric at a time
             host.copy_(res, non_blocking=True)
             ev = torch.cuda.Event(enable_timing=True)
             ev.record(s)
