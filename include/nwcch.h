@@ -83,6 +83,14 @@ int nwcch_topo_create_ex(int64_t n, const int64_t* off, const int32_t* tgt, cons
 int nwcch_topo_from_csr(int64_t n, const int64_t* first, const int32_t* head, int64_t m,
                         const int64_t* e_arc, const int8_t* e_up, nwcch_topo** out);
 
+/* nwcch_topo_from_csr with the options of nwcch_topo_create_ex
+ * (NWCCH_TOPO_DEVICE_INDEX: the triangle index on GPU `device`).  A CSR that
+ * is not a chordal completion is still rejected (NWCCH_ERR_CHORDAL): the
+ * device counts are checked against sum d(d-1)/2 before anything else. */
+int nwcch_topo_from_csr_ex(int64_t n, const int64_t* first, const int32_t* head, int64_t m,
+                           const int64_t* e_arc, const int8_t* e_up, int32_t flags, int device,
+                           nwcch_topo** out);
+
 void nwcch_topo_destroy(nwcch_topo* t);
 
 /* out8 = {n, m, mp, triangles, depth, levels, loop_edges, mapped_edges}

@@ -128,12 +128,19 @@ extern "C" int nwcch_topo_create_ex(int64_t n, const int64_t* off, const int32_t
 
 extern "C" int nwcch_topo_from_csr(int64_t n, const int64_t* first, const int32_t* head, int64_t m,
                                    const int64_t* e_arc, const int8_t* e_up, nwcch_topo** out) {
+  return nwcch_topo_from_csr_ex(n, first, head, m, e_arc, e_up, 0, 0, out);
+}
+
+extern "C" int nwcch_topo_from_csr_ex(int64_t n, const int64_t* first, const int32_t* head, int64_t m,
+                                      const int64_t* e_arc, const int8_t* e_up, int32_t flags, int device,
+                                      nwcch_topo** out) {
   if (!out) { set_error("null out"); return NWCCH_ERR_ARG; }
   *out = nullptr;
   if (!valid_csr(n, first) || (first[n] > 0 && !head) || m < 0 || (m > 0 && (!e_arc || !e_up))) {
     set_error("nwcch_topo_from_csr: bad CSR or null argument");
     return NWCCH_ERR_ARG;
   }
+  if (flags & ~NWCCH_TOPO_DEVICE_INDEX) { set_error("nwcch_topo_from_csr_ex: unknown flags"); return NWCCH_ERR_ARG; }
   nwcch_topo* h = new nwcch_topo();
   try {
     Topology& t = h->t;
@@ -154,7 +161,12 @@ extern "C" int nwcch_topo_from_csr(int64_t n, const int64_t* first, const int32_
       if (t.e_arc[k] < -1 || t.e_arc[k] >= t.mp) throw std::runtime_error("e_arc out of range");
       if (t.e_arc[k] < 0) t.loop_edges.push_back((i32)k);
     }
-    build_index(t);
+    if (flags & NWCCH_TOPO_DEVICE_INDEX) {
+      build_meta(t);
+      build_schedule(t, count_triangles_device(t, device));
+    } else {
+      build_index(t);
+    }
   } catch (const std::exception& e) {
     delete h;
     return map_error(e);

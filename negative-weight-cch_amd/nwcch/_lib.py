@@ -39,6 +39,7 @@ def lib():
             "nwcch_topo_create": ([i64, p, p, p, C.POINTER(vp)], C.c_int),
             "nwcch_topo_create_ex": ([i64, p, p, p, i32, C.c_int, C.POINTER(vp)], C.c_int),
             "nwcch_topo_from_csr": ([i64, p, p, i64, p, p, C.POINTER(vp)], C.c_int),
+            "nwcch_topo_from_csr_ex": ([i64, p, p, i64, p, p, i32, C.c_int, C.POINTER(vp)], C.c_int),
             "nwcch_topo_destroy": ([vp], None),
             "nwcch_topo_info": ([vp, p], C.c_int),
             "nwcch_topo_export": ([vp, p, p, p, p, p, p, p], C.c_int),
@@ -112,7 +113,7 @@ def exported_symbols():
             "nwcch_customize_batch", "nwcch_negcycle_scan_batch", "nwcch_batch_slab_width",
             "nwcch_set_tuning", "nwcch_get_tuning", "nwcch_query_multi", "nwcch_potential",
             "nwcch_query_path", "nwcch_topo_create_ex", "nwcch_dev_export_triangles",
-            "nwcch_dev_export_sides"]
+            "nwcch_dev_export_sides", "nwcch_topo_from_csr_ex"]
 
 
 # ------------------------------------------------------------ host topology
@@ -148,15 +149,21 @@ class Topology:
         return t
 
     @classmethod
-    def from_csr(cls, n, first, head, e_arc=None, e_up=None):
+    def from_csr(cls, n, first, head, e_arc=None, e_up=None, device_index=None):
+        """device_index = a GPU ordinal: that GPU builds the triangle index."""
         first = np.ascontiguousarray(first, np.int64)
         head = np.ascontiguousarray(head, np.int32)
         m = 0 if e_arc is None else int(len(e_arc))
         ea = None if e_arc is None else np.ascontiguousarray(e_arc, np.int64)
         eu = None if e_up is None else np.ascontiguousarray(e_up, np.int8)
         h = C.c_void_p()
-        check(lib().nwcch_topo_from_csr(n, ptr(first), ptr(head), m, ptr(ea), ptr(eu), C.byref(h)),
-              "nwcch_topo_from_csr")
+        if device_index is None:
+            check(lib().nwcch_topo_from_csr(n, ptr(first), ptr(head), m, ptr(ea), ptr(eu), C.byref(h)),
+                  "nwcch_topo_from_csr")
+        else:
+            check(lib().nwcch_topo_from_csr_ex(n, ptr(first), ptr(head), m, ptr(ea), ptr(eu),
+                                               NWCCH_TOPO_DEVICE_INDEX, int(device_index), C.byref(h)),
+                  "nwcch_topo_from_csr_ex")
         t = cls(h)
         t.rank = None
         return t

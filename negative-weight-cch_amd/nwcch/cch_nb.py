@@ -102,9 +102,12 @@ def device_topology(n, first, head, e_arc=None, e_up=None) -> DeviceTopology:
         for k, v in _cache.items():
             if k[0] == skey:
                 return v
+    # the triangle index is built on the GPU that will hold the topology (the
+    # same arrays as a host build, without the host's O(T) passes)
     topo = _lib.Topology.from_csr(int(n), np.asarray(first), np.asarray(head),
                                   None if e_arc is None else np.asarray(e_arc),
-                                  None if e_up is None else np.asarray(e_up))
+                                  None if e_up is None else np.asarray(e_up),
+                                  device_index=torch.cuda.current_device())
     dt = DeviceTopology(topo)
     if len(_cache) >= _CACHE_MAX:
         _cache.pop(next(iter(_cache)))

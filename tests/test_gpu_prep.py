@@ -89,3 +89,22 @@ def test_device_index_without_triangles():
     """A star has no lower triangles (T = 0); vertex 4 has no arcs."""
     g = graphs.from_arcs(5, [(0, 1, 3), (1, 0, 3), (1, 2, -1), (3, 1, 4)])
     _check_same(g, np.array([0, 4, 1, 2, 3], np.int32), "star")
+
+
+def test_device_index_from_csr(c0):
+    """The CSR form (the arrays cch_nb.customize takes): the device-built index
+    equals the host's, and a CSR that is not a chordal completion is rejected
+    by the device count check as by the host build."""
+    g, rank = c0
+    print("case: device index from CSR", flush=True)
+    th = _lib.Topology.from_graph(g.n, g.off, g.tgt, rank)
+    ex = th.export()
+    td = _lib.Topology.from_csr(g.n, ex["first"], ex["head"], ex["e_arc"], ex["e_up"], device_index=0)
+    sh, oh, tab = th.export_triangles()
+    sd, od, _ = td.export_triangles(with_list=False)
+    assert np.array_equal(sh, sd) and np.array_equal(oh, od)
+    assert np.array_equal(DeviceTopology(td).export_triangles(), tab)
+    first = np.array([0, 2, 2, 2], np.int64)   # 0 -> {1, 2} needs the arc (1, 2)
+    head = np.array([1, 2], np.int32)
+    with pytest.raises(_lib.NwcchError, match="chordal"):
+        _lib.Topology.from_csr(3, first, head, device_index=0)
