@@ -152,8 +152,16 @@ def plain(args, out):
         timed(graph.replay, args.warmup)
         t_graph = timed(graph.replay, args.steps)
     dt.check_status()
-    # device time of the customization alone, per call (back-to-back launches)
-    cust_ms = event_ms(lambda: dt.customize(w, out=ud), args.steps)
+    # device time of the customization alone, per call: replayed from a HIP
+    # graph (eager launches of a sweep this fast are bound by the host's issue
+    # rate), or back-to-back eager launches without graphs
+    if args.no_graph:
+        cust_ms = event_ms(lambda: dt.customize(w, out=ud), args.steps)
+    else:
+        cgraph = StepGraph(lambda: dt.customize(w, out=ud))
+        cgraph.replay()
+        cust_ms = event_ms(cgraph.replay, args.steps)
+        del cgraph
     scan_ms = event_ms(lambda: dt.negcycle_scan(ud, out=scan), args.steps)
     # per-launch durations of the level kernel: HIP events recorded by the
     # library around each launch, on the stream the launch goes to
@@ -168,8 +176,8 @@ def plain(args, out):
     B = cust_bytes(g.m, topo.mp, topo.triangles, topo.n_mapped)
     B_lv = level_bytes(topo.mp, topo.triangles)
     cmed = statistics.median(cust_ms)
-    # average level launch: the back-to-back customization minus its base pass
-    # (events around every launch would add their own gaps)
+    # average level launch: the customization minus its base pass, over the
+    # level launches (events around every launch would add their own gaps)
     lv_avg_us = 1000.0 * (cmed - base_ms) / n_lv
     achieved = (B_lv / n_lv) / (lv_avg_us * 1e-6) / 1e9
     flat = dt.get_tuning("flat_min_tri")

@@ -229,6 +229,42 @@ __global__ void __launch_bounds__(256) k_customize_flat(i64 q0, i64 nt, const in
     up = inf_add(A.y, B.x, st);  // y -> x -> z
     dn = inf_add(B.y, A.x, st);  // z -> x -> y
   }
+  if constexpr (kStore == 0) {
+    // Segmented prefix minimum with DPP lane moves (no LDS round trips): lane l
+    // ends with the minimum over [s, l], s = the first lane of its arc's run
+    // (runs are contiguous); a step merges the source lane only when that lane
+    // lies in the same run.  Four row shifts build each 16-lane row, then
+    // row_bcast:15 and row_bcast:31 carry rows 0 and 0-1 into the rows above.
+    // The run's last lane folds the minimum into the arc.
+    const unsigned long long H = __ballot(lane == 0 || cprev != c);   // run starts (lane 0 always)
+    const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+    const int s0 = 63 - __clzll(H & upto);
+    const int row = lane >> 4;
+#define NWCCH_SEG_STEP(CTRL, ROWS, SRC)                                             \
+    {                                                                              \
+      const int32_t u2 = __builtin_amdgcn_update_dpp(kInf, up, CTRL, ROWS, 0xf, false); \
+      const int32_t d2 = __builtin_amdgcn_update_dpp(kInf, dn, CTRL, ROWS, 0xf, false); \
+      if ((SRC) >= s0) {                                                           \
+        up = min(up, u2);                                                          \
+        dn = min(dn, d2);                                                          \
+      }                                                                            \
+    }
+    NWCCH_SEG_STEP(0x111, 0xf, lane - 1)          // row_shr:1
+    NWCCH_SEG_STEP(0x112, 0xf, lane - 2)          // row_shr:2
+    NWCCH_SEG_STEP(0x114, 0xf, lane - 4)          // row_shr:4
+    NWCCH_SEG_STEP(0x118, 0xf, lane - 8)          // row_shr:8
+    NWCCH_SEG_STEP(0x142, 0xa, 16 * row - 1)      // row_bcast:15 into rows 1 and 3
+    NWCCH_SEG_STEP(0x143, 0xc, 31)                // row_bcast:31 into rows 2 and 3
+#undef NWCCH_SEG_STEP
+    const bool tail = valid && (lane == 63 || ((H >> (lane + 1)) & 1ull));
+    if (tail) {
+      int32_t* v = reinterpret_cast<int32_t*>(ud) + 2 * (i64)c;
+      if (up != kInf) atomicMin(v, up);
+      if (dn != kInf) atomicMin(v + 1, dn);
+    }
+    if (st) atomicOr(status, st);
+    return;
+  }
   int open = kStore ? (valid && c_out == c) : 1;  // the arc continues past this wavefront
   // an arc's lanes are contiguous: if lane+o has the same arc, so has every lane between
 #pragma unroll

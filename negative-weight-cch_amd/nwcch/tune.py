@@ -29,15 +29,27 @@ def _ms(fn, reps: int) -> float:
     return statistics.median(a.elapsed_time(b) for a, b in evs)
 
 
+def _graph_ms(fn, reps: int) -> float:
+    """Median time of `fn` replayed as a HIP graph: the sweep as deployed, with
+    no host launch cost in the measurement (the level launches of a small graph
+    can be shorter than the host needs to issue them)."""
+    from .device import StepGraph
+    g = StepGraph(fn)
+    return _ms(g.replay, reps)
+
+
 FLAT_OFF = 1 << 30
 
 
 def tune_customize(dt: DeviceTopology, w, candidates=(1, 2, 4, 8, 16),
-                   flat=(0, 64, 128, 256, FLAT_OFF), reps: int = 5, margin: float = 0.02) -> dict:
+                   flat=(0, 64, 128, 256, FLAT_OFF), reps: int = 5, margin: float = 0.02,
+                   graph: bool = True) -> dict:
     """Single-metric level sweep: triangles per lane of the team kernel, then
     the heaviest-arc size from which a level runs triangle-parallel instead
-    (FLAT_OFF = never, 0 = every level).  A later candidate replaces an
-    earlier one only if it is faster by more than `margin` (timing noise)."""
+    (FLAT_OFF = never, 0 = every level), then the store mode and XCD slicing.
+    A later candidate replaces an earlier one only if it is faster by more
+    than `margin` (timing noise).  graph=True times each candidate as a
+    captured HIP graph, False as eager launches."""
     w = dt.weights(w) if not (isinstance(w, torch.Tensor) and w.is_cuda) else w
     ref = None
     out = torch.empty((dt.mp, 2), dtype=torch.int32, device=dt.dev)
@@ -46,7 +58,7 @@ def tune_customize(dt: DeviceTopology, w, candidates=(1, 2, 4, 8, 16),
         nonlocal ref
         dt.set_tuning(key, value)
         out.fill_(0)
-        t = _ms(lambda: dt.customize(w, out=out), reps)
+        t = (_graph_ms if graph else _ms)(lambda: dt.customize(w, out=out), reps)
         if ref is None:
             ref = out.clone()
         elif not torch.equal(ref, out):
