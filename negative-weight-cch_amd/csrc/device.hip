@@ -220,7 +220,8 @@ __global__ void __launch_bounds__(256) k_customize_flat(i64 q0, i64 nt, const in
       if (lane == 63 && i + 1 < nt) c_out = tri_c[q + 1];
     }
   }
-  const int32_t cprev = __shfl_up(c, 1, 64);
+  // the left neighbour's arc: wave_shr:1, a DPP move across the whole wavefront
+  const int32_t cprev = kStore ? __shfl_up(c, 1, 64) : __builtin_amdgcn_update_dpp(-2, c, 0x138, 0xf, 0xf, false);
   const bool head = valid && (lane == 0 || cprev != c);
   int2 base = make_int2(kInf, kInf);
   if (valid) {
