@@ -100,6 +100,7 @@ __global__ void __launch_bounds__(256) k_base_batch(i64 mp, int32_t K, int32_t s
 // level's arcs are sorted by triangle count, descending) and owns blocks
 // [blk[j], blk[j+1]); every block is 16 wavefronts.
 constexpr int kWaveClasses = 5;
+constexpr int kUnroll = 4;      // triangles per lane with their gathers in flight at once (8 measured 15 % slower)
 constexpr int kSlabWidth = 64;  // default metrics per slab
 struct BatchLaunch {
   i64 cls[kWaveClasses + 1];
@@ -109,7 +110,7 @@ struct BatchLaunch {
 
 // One level of the batched sweep.  A wavefront holds KL metrics x (64 / KL)
 // triangle sub-lanes; WPA wavefronts of one block split an arc's triangles,
-// each lane keeping four (a, b) pairs -- eight gathers -- in flight per step;
+// each lane keeping kUnroll (a, b) pairs -- twice as many gathers -- in flight per step;
 // sub-lanes meet through cross-lane minima and wavefronts through LDS.  Same
 // no-atomics argument as the single-metric level kernel.  bpos (optional)
 // renumbers the schedule positions of each wave class (arc-id order), and with
@@ -157,17 +158,17 @@ __global__ void __launch_bounds__(1024) k_customize_batch_level(BatchLaunch L, i
     const int32_t stride = WPA * S;
     const i64 t1 = r.tri_begin + r.tri_count;
     i64 q = r.tri_begin + chunk * S + ts;
-    for (; q + 3 * stride < t1; q += 4 * stride) {
-      int2 ab[4], A[4], B[4];
+    for (; q + (kUnroll - 1) * stride < t1; q += kUnroll * stride) {
+      int2 ab[kUnroll], A[kUnroll], B[kUnroll];
 #pragma unroll
-      for (int u = 0; u < 4; u++) ab[u] = tri_ab[q + u * stride];
+      for (int u = 0; u < kUnroll; u++) ab[u] = tri_ab[q + u * stride];
 #pragma unroll
-      for (int u = 0; u < 4; u++) {
+      for (int u = 0; u < kUnroll; u++) {
         A[u] = slab[(i64)ab[u].x * KL];
         B[u] = slab[(i64)ab[u].y * KL];
       }
 #pragma unroll
-      for (int u = 0; u < 4; u++) {
+      for (int u = 0; u < kUnroll; u++) {
         up = min(up, inf_add_b(A[u].y, B[u].x, st));
         dn = min(dn, inf_add_b(B[u].y, A[u].x, st));
       }
