@@ -134,6 +134,14 @@ int nwcch_dev_export_sides(const nwcch_dev* h, int64_t* side_off, int32_t* side_
  * CCH.customize (python/cch.py:249-287, without the witness list) and
  * CCH::customize / customize_parallel (cpp/cch.cpp:437-463, 496-529). */
 int nwcch_customize(nwcch_dev* h, const int32_t* w, int32_t* ud, void* stream);
+/* The base pass and the level launches (one per ET level) are captured once per
+ * stream into a HIP graph and replayed; the buffers of each call reach the
+ * kernels through a per-stream argument slot written by a one-thread kernel in
+ * stream order, so any w and ud may be passed.  Inside a caller's own stream
+ * capture the launches are recorded one by one into that capture.  Tuning
+ * changes drop the captured graphs (the device is synchronised then).  Up to
+ * 256 streams per handle keep their own slot; a further stream takes the
+ * oldest slot over after a device synchronisation. */
 
 /* Negative-cycle scan of ud[mp][2] (any metric, no handle needed): out2
  * (device int64[2]) = {number of arcs with up + down < 0 (INF never counts),
@@ -268,6 +276,8 @@ int nwcch_status(nwcch_dev* h, uint32_t* out, int clear, void* stream);
  *                   schedule order (default).
  *   "batch_xcd"     batched sweep: 1 = each XCD takes one contiguous slice of a
  *                   wave class, 0 = workgroups interleaved (default).
+ *   "customize_graph"  1 = nwcch_customize replays a captured graph per
+ *                   stream (default), 0 = it launches kernel by kernel.
  *   "query_wpb"     queries (wavefronts) per workgroup of the query kernels:
  *                   1 (default), 2 or 4, fewer where their labels would pass
  *                   64 KB of LDS.

@@ -112,7 +112,9 @@ const std::vector<LevelLaunch>& level_launches(DevTopo& d) {
 // INF if none.  One launch over all arcs; the level kernels start from it.
 __global__ void __launch_bounds__(256) k_base_all(i64 mp, const int32_t* __restrict__ ae_off,
                                                   const int32_t* __restrict__ ae_idx,
-                                                  const int32_t* __restrict__ w, int2* ud, uint32_t* status) {
+                                                  const CustPtrs* __restrict__ P, uint32_t* status) {
+  const int32_t* __restrict__ w = P->w;
+  int2* ud = P->ud;
   uint32_t st = 0;
   const i64 stride = (i64)gridDim.x * blockDim.x;
   for (i64 c = (i64)blockIdx.x * blockDim.x + threadIdx.x; c < mp; c += stride) {
@@ -130,6 +132,13 @@ __global__ void __launch_bounds__(256) k_base_all(i64 mp, const int32_t* __restr
   if (st) atomicOr(status, st);
 }
 
+// The buffers of one customization call into its argument slot (a kernel, so
+// the values travel with the launch in stream order).
+__global__ void k_set_cust_args(CustPtrs* P, const int32_t* w, int2* ud) {
+  P->w = w;
+  P->ud = ud;
+}
+
 // One level of customization: F_c of every arc of the level, i.e. the min of its
 // base value and its lower triangles (python/cch.py:279-285;
 // python/cch_nb.py:543-562 in pull form).  A team of TS lanes owns an arc, each
@@ -138,8 +147,9 @@ __global__ void __launch_bounds__(256) k_base_all(i64 mp, const int32_t* __restr
 // other -- their inputs have lower endpoints of smaller ET height -- so there are
 // no atomics and every arc is written exactly once.
 __global__ void __launch_bounds__(256) k_customize_level(LevelLaunch L, const PosRec* __restrict__ prec,
-                                                         const int2* __restrict__ tri_ab, int2* ud,
-                                                         uint32_t* status) {
+                                                         const int2* __restrict__ tri_ab,
+                                                         const CustPtrs* __restrict__ P, uint32_t* status) {
+  int2* ud = P->ud;
   const int32_t b = blockIdx.x;
   int k = 0;
   while (k < kTeamClasses - 1 && b >= L.blk[k + 1]) k++;
@@ -201,8 +211,10 @@ __global__ void __launch_bounds__(256) k_customize_level(LevelLaunch L, const Po
 // works through one contiguous eighth of the level.
 template <int kStore>
 __global__ void __launch_bounds__(256) k_customize_flat(i64 q0, i64 nt, const int2* __restrict__ tri_ab,
-                                                        const int32_t* __restrict__ tri_c, int2* ud,
-                                                        uint32_t* status, int xcd_slices) {
+                                                        const int32_t* __restrict__ tri_c,
+                                                        const CustPtrs* __restrict__ P, uint32_t* status,
+                                                        int xcd_slices) {
+  int2* ud = P->ud;
   const int lane = threadIdx.x & 63;
   const unsigned per = gridDim.x >> 3;
   const unsigned slice = xcd_slices ? (blockIdx.x & 7u) * per + (blockIdx.x >> 3) : blockIdx.x;
@@ -1095,6 +1107,8 @@ int fail(const std::exception& e) {
   return NWCCH_ERR_RUNTIME;
 }
 
+void drop_cust_graphs(DevTopo& d);
+
 }  // namespace
 
 void nwcch::down_csr(const Topology& t, std::vector<i64>& off, std::vector<int32_t>& arc) {
@@ -1287,9 +1301,12 @@ extern "C" int nwcch_dev_create(const nwcch_topo* topo, int device, nwcch_dev** 
     d.arc_pos = dev_upload(d, t.arc_pos.data(), t.arc_pos.size());
     d.status = dev_alloc<uint32_t>(d, 1);
     HIP_OK(hipMemset(d.status, 0, sizeof(uint32_t)));
+    d.cust_args = dev_alloc<CustPtrs>(d, kCustSlots);
+    HIP_OK(hipStreamCreateWithFlags(&d.cap_stream, hipStreamNonBlocking));
     d.scan = dev_alloc<unsigned long long>(d, 4);
     HIP_OK(hipDeviceSynchronize());
   } catch (const std::exception& e) {
+    if (h->d.cap_stream) (void)hipStreamDestroy(h->d.cap_stream);
     for (void* p : h->d.owned) (void)hipFree(p);
     delete h;
     return fail(e);
@@ -1302,6 +1319,8 @@ extern "C" void nwcch_dev_destroy(nwcch_dev* h) {
   if (!h) return;
   (void)hipSetDevice(h->d.device);
   (void)hipDeviceSynchronize();
+  drop_cust_graphs(h->d);
+  if (h->d.cap_stream) (void)hipStreamDestroy(h->d.cap_stream);
   for (hipEvent_t e : h->d.ev_pool) (void)hipEventDestroy(e);
   for (auto& st : h->d.stage)
     if (st.ev) (void)hipEventDestroy(st.ev);
@@ -1356,45 +1375,122 @@ extern "C" int nwcch_dev_export_sides(const nwcch_dev* h, int64_t* side_off, int
   return NWCCH_OK;
 }
 
+namespace {
+
+// The base pass and the level launches of one customization, reading the
+// metric buffers from argument slot P.
+void launch_customize(DevTopo& d, const CustPtrs* P, hipStream_t s) {
+  {
+    ProfScope ps(d, s, kKOther);
+    const i64 blocks = std::min<i64>(std::max<i64>((d.mp + 255) / 256, 1), 16384);
+    hipLaunchKernelGGL(k_base_all, dim3((unsigned)blocks), dim3(256), 0, s, d.mp, d.ae_off, d.ae_idx, P, d.status);
+  }
+  const std::vector<LevelLaunch>& launches = level_launches(d);
+  for (size_t i = 0; i < d.lv.size(); i++) {
+    const LevelRange& r = d.lv[i];
+    if (r.end == r.begin || r.triangles == 0) continue;  // base values are final
+    ProfScope ps(d, s, kKCustomize);
+    if (d.pos_cnt[r.begin] >= d.flat_min_tri) {  // positions of a level: heaviest first
+      const i64 q0 = d.lv_tri0[i], nt = d.lv_tri0[i + 1] - q0;
+      const i64 nb = (nt + 255) / 256;
+      const i64 grid = d.flat_xcd ? 8 * ((nb + 7) / 8) : nb;  // XCD slices: 8 equal shares
+      const int2* tab = d.tri_ab;
+      const int32_t* tc = d.tri_c;
+      if (d.flat_store == 1)
+        hipLaunchKernelGGL(k_customize_flat<1>, dim3((unsigned)grid), dim3(256), 0, s, q0, nt, tab, tc, P, d.status,
+                           d.flat_xcd);
+      else if (d.flat_store == 2)
+        hipLaunchKernelGGL(k_customize_flat<2>, dim3((unsigned)grid), dim3(256), 0, s, q0, nt, tab, tc, P, d.status,
+                           d.flat_xcd);
+      else
+        hipLaunchKernelGGL(k_customize_flat<0>, dim3((unsigned)grid), dim3(256), 0, s, q0, nt, tab, tc, P, d.status,
+                           d.flat_xcd);
+      continue;
+    }
+    const LevelLaunch& L = launches[i];
+    hipLaunchKernelGGL(k_customize_level, dim3(L.blk[kTeamClasses]), dim3(256), 0, s, L, d.prec, d.tri_ab, P,
+                       d.status);
+  }
+  HIP_OK(hipGetLastError());
+}
+
+// Argument slot of a stream: one per stream, so calls on different streams never
+// share one.
+bool stream_capturing(hipStream_t s);
+void drop_slot_graph(DevTopo& d, int i);
+
+int cust_slot(DevTopo& d, hipStream_t s) {
+  for (size_t i = 0; i < d.cust_stream.size(); i++)
+    if (d.cust_stream[i] == (void*)s) return (int)i;
+  if ((int)d.cust_stream.size() < kCustSlots) {
+    d.cust_stream.push_back((void*)s);
+    d.cust_exec.push_back(nullptr);
+    d.cust_graph.push_back(nullptr);
+    return (int)d.cust_stream.size() - 1;
+  }
+  // every slot taken: hand the oldest one over, once the device has finished
+  // with it (impossible while the caller captures a graph)
+  if (stream_capturing(s))
+    throw std::runtime_error("nwcch_customize: more streams than argument slots while capturing");
+  HIP_OK(hipDeviceSynchronize());
+  const int i = d.cust_next++ % kCustSlots;
+  drop_slot_graph(d, i);
+  d.cust_stream[i] = (void*)s;
+  return i;
+}
+
+bool stream_capturing(hipStream_t s) {
+  hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(s, &st) != hipSuccess) return true;   // unknown: launch directly
+  return st != hipStreamCaptureStatusNone;
+}
+
+// Drops every captured customization (launch shapes changed, or the handle
+// goes away).  Waits for the device first: a graph may still be running.
+void drop_slot_graph(DevTopo& d, int i) {
+  if (d.cust_exec[i]) (void)hipGraphExecDestroy(d.cust_exec[i]);
+  if (d.cust_graph[i]) (void)hipGraphDestroy(d.cust_graph[i]);
+  d.cust_exec[i] = nullptr;
+  d.cust_graph[i] = nullptr;
+}
+
+void drop_cust_graphs(DevTopo& d) {
+  bool any = false;
+  for (hipGraphExec_t e : d.cust_exec) any |= e != nullptr;
+  if (!any) return;
+  (void)hipDeviceSynchronize();
+  for (size_t i = 0; i < d.cust_exec.size(); i++) drop_slot_graph(d, (int)i);
+}
+
+}  // namespace
+
 extern "C" int nwcch_customize(nwcch_dev* h, const int32_t* w, int32_t* ud, void* stream) {
   if (!h || !w || !ud) { set_error("null argument"); return NWCCH_ERR_ARG; }
   try {
     DevTopo& d = h->d;
     hipStream_t s = (hipStream_t)stream;
-    int2* u = reinterpret_cast<int2*>(ud);
-    {
-      ProfScope ps(d, s, kKOther);
-      const i64 blocks = std::min<i64>(std::max<i64>((d.mp + 255) / 256, 1), 16384);
-      hipLaunchKernelGGL(k_base_all, dim3((unsigned)blocks), dim3(256), 0, s, d.mp, d.ae_off, d.ae_idx, w, u,
-                         d.status);
+    const int slot = cust_slot(d, s);
+    CustPtrs* P = d.cust_args + slot;
+    hipLaunchKernelGGL(k_set_cust_args, dim3(1), dim3(1), 0, s, P, w, reinterpret_cast<int2*>(ud));
+    if (!d.customize_graph || d.prof_on || stream_capturing(s)) {
+      launch_customize(d, P, s);   // per-launch profiling, or inside the caller's own capture
+      return NWCCH_OK;
     }
-    const std::vector<LevelLaunch>& launches = level_launches(d);
-    for (size_t i = 0; i < d.lv.size(); i++) {
-      const LevelRange& r = d.lv[i];
-      if (r.end == r.begin || r.triangles == 0) continue;  // base values are final
-      ProfScope ps(d, s, kKCustomize);
-      if (d.pos_cnt[r.begin] >= d.flat_min_tri) {  // positions of a level: heaviest first
-        const i64 q0 = d.lv_tri0[i], nt = d.lv_tri0[i + 1] - q0;
-        const i64 nb = (nt + 255) / 256;
-        const i64 grid = d.flat_xcd ? 8 * ((nb + 7) / 8) : nb;  // XCD slices: 8 equal shares
-        const int2* tab = d.tri_ab;
-        const int32_t* tc = d.tri_c;
-        if (d.flat_store == 1)
-          hipLaunchKernelGGL(k_customize_flat<1>, dim3((unsigned)grid), dim3(256), 0, s, q0, nt, tab, tc, u, d.status,
-                             d.flat_xcd);
-        else if (d.flat_store == 2)
-          hipLaunchKernelGGL(k_customize_flat<2>, dim3((unsigned)grid), dim3(256), 0, s, q0, nt, tab, tc, u, d.status,
-                             d.flat_xcd);
-        else
-          hipLaunchKernelGGL(k_customize_flat<0>, dim3((unsigned)grid), dim3(256), 0, s, q0, nt, tab, tc, u, d.status,
-                             d.flat_xcd);
-        continue;
+    if (!d.cust_exec[slot]) {
+      // capture the launch sequence once on the internal stream, for this slot
+      HIP_OK(hipStreamBeginCapture(d.cap_stream, hipStreamCaptureModeThreadLocal));
+      try {
+        launch_customize(d, P, d.cap_stream);
+      } catch (...) {
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(d.cap_stream, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
       }
-      const LevelLaunch& L = launches[i];
-      hipLaunchKernelGGL(k_customize_level, dim3(L.blk[kTeamClasses]), dim3(256), 0, s, L, d.prec, d.tri_ab,
-                         u, d.status);
+      HIP_OK(hipStreamEndCapture(d.cap_stream, &d.cust_graph[slot]));
+      HIP_OK(hipGraphInstantiate(&d.cust_exec[slot], d.cust_graph[slot], nullptr, nullptr, 0));
     }
-    HIP_OK(hipGetLastError());
+    HIP_OK(hipGraphLaunch(d.cust_exec[slot], s));
   } catch (const std::exception& e) {
     return fail(e);
   }
@@ -1565,6 +1661,12 @@ extern "C" int nwcch_potential(nwcch_dev* h, const int32_t* ud, int32_t* p, void
 extern "C" int nwcch_set_tuning(nwcch_dev* h, const char* key, int32_t value) {
   if (!h || !key) { set_error("null argument"); return NWCCH_ERR_ARG; }
   const std::string k(key);
+  drop_cust_graphs(h->d);   // captured customizations hold the old launch shapes
+  if (k == "customize_graph") {
+    if (value < 0 || value > 1) { set_error("customize_graph is 0 or 1"); return NWCCH_ERR_ARG; }
+    h->d.customize_graph = value;
+    return NWCCH_OK;
+  }
   if (k == "upd_persistent_max") {
     if (value < 0) { set_error("upd_persistent_max must be >= 0"); return NWCCH_ERR_ARG; }
     h->d.upd_persistent_max = value;
@@ -1622,6 +1724,7 @@ extern "C" int nwcch_get_tuning(const nwcch_dev* h, const char* key, int32_t* va
   else if (k == "flat_min_tri") *value = h->d.flat_min_tri;
   else if (k == "flat_xcd") *value = h->d.flat_xcd;
   else if (k == "query_wpb") *value = h->d.query_wpb;
+  else if (k == "customize_graph") *value = h->d.customize_graph;
   else if (k == "flat_store") *value = h->d.flat_store;
   else if (k == "query_deep_max") *value = h->d.query_deep_max;
   else if (k == "query_chunk") *value = h->d.query_chunk;

@@ -30,6 +30,14 @@ struct VInfo {
   int32_t first, deg, parent, depth;
 };
 
+// The metric buffers of one customization call, read by its kernels from
+// device memory, so that a captured launch sequence serves any buffers.
+struct CustPtrs {
+  const int32_t* w;
+  int2* ud;
+};
+constexpr int kCustSlots = 256;  // streams with their own argument slot and graph
+
 // Launch geometry of one level of the single-metric sweep: team class k
 // (64 >> k lanes per arc) owns schedule positions [cls[k], cls[k+1]) and blocks
 // [blk[k], blk[k+1]).
@@ -135,6 +143,15 @@ struct DevTopo {
 
   // optional launch profiling: every launch is bracketed by two events
   struct Rec { int kernel; hipEvent_t a, b; };
+  // customization replayed from a HIP graph per stream: slot i holds the
+  // arguments of calls on cust_stream[i] and that stream's instantiated graph
+  CustPtrs* cust_args = nullptr;                 // [kCustSlots] device
+  std::vector<void*> cust_stream;                // stream of each slot in use
+  std::vector<hipGraphExec_t> cust_exec;         // per slot, nullptr until built
+  std::vector<hipGraph_t> cust_graph;
+  int32_t cust_next = 0;                         // next slot handed over once all are taken
+  hipStream_t cap_stream = nullptr;              // internal stream the graphs are captured on
+  int32_t customize_graph = 1;                   // 0: launch the kernels one by one
   bool prof_on = false;
   std::vector<hipEvent_t> ev_pool;
   size_t ev_used = 0;

@@ -227,3 +227,40 @@ def test_potential_equals_johnson(c0, which):
     assert np.array_equal(p, want)
     tails = np.repeat(np.arange(g.n), np.diff(g.off))
     assert (g.w.astype(np.int64) + p[tails] - p[g.tgt] >= 0).all()
+
+
+def test_customize_graph_paths(c0):
+    """nwcch_customize replays a captured launch sequence per stream, with the
+    buffers of each call passed through an argument slot: calls with changing
+    weights and outputs, on two streams at once, equal the launch-by-launch
+    path and the oracle."""
+    g, rank = c0
+    print("case: customize graph paths on config 0", flush=True)
+    topo = _lib.Topology.from_graph(g.n, g.off, g.tgt, rank)
+    dt = DeviceTopology(topo)
+    st = O.Structure(g.n, g.off, g.tgt, rank)
+    rng = np.random.default_rng(4)
+    ws = [(g.w + rng.integers(0, 40, g.m)).astype(np.int32) for _ in range(4)]
+    want = [st.customize(w)[:2] for w in ws]
+    wd = [dt.weights(w) for w in ws]
+    outs = [torch.full((topo.mp, 2), -5, dtype=torch.int32, device=dt.dev) for _ in range(4)]
+    for rep in range(2):
+        for k in range(4):                       # one stream, new buffers every call
+            dt.customize(wd[k], out=outs[k])
+        torch.cuda.synchronize()
+        for k in range(4):
+            got = outs[k].cpu().numpy()
+            assert np.array_equal(got[:, 0], want[k][0]) and np.array_equal(got[:, 1], want[k][1]), (rep, k)
+            outs[k].fill_(-5)
+    s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+    torch.cuda.synchronize()
+    for k in range(4):                           # two streams interleaved
+        dt.customize(wd[k], out=outs[k], stream=s1 if k % 2 == 0 else s2)
+    torch.cuda.synchronize()
+    for k in range(4):
+        got = outs[k].cpu().numpy()
+        assert np.array_equal(got[:, 0], want[k][0]) and np.array_equal(got[:, 1], want[k][1]), k
+    dt.set_tuning("customize_graph", 0)
+    assert torch.equal(dt.customize(wd[1]), outs[1])
+    dt.set_tuning("customize_graph", 1)
+    assert dt.status() == 0
