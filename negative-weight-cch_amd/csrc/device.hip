@@ -1302,11 +1302,9 @@ extern "C" int nwcch_dev_create(const nwcch_topo* topo, int device, nwcch_dev** 
     d.status = dev_alloc<uint32_t>(d, 1);
     HIP_OK(hipMemset(d.status, 0, sizeof(uint32_t)));
     d.cust_args = dev_alloc<CustPtrs>(d, kCustSlots);
-    HIP_OK(hipStreamCreateWithFlags(&d.cap_stream, hipStreamNonBlocking));
     d.scan = dev_alloc<unsigned long long>(d, 4);
     HIP_OK(hipDeviceSynchronize());
   } catch (const std::exception& e) {
-    if (h->d.cap_stream) (void)hipStreamDestroy(h->d.cap_stream);
     for (void* p : h->d.owned) (void)hipFree(p);
     delete h;
     return fail(e);
@@ -1320,7 +1318,6 @@ extern "C" void nwcch_dev_destroy(nwcch_dev* h) {
   (void)hipSetDevice(h->d.device);
   (void)hipDeviceSynchronize();
   drop_cust_graphs(h->d);
-  if (h->d.cap_stream) (void)hipStreamDestroy(h->d.cap_stream);
   for (hipEvent_t e : h->d.ev_pool) (void)hipEventDestroy(e);
   for (auto& st : h->d.stage)
     if (st.ev) (void)hipEventDestroy(st.ev);
@@ -1477,17 +1474,27 @@ extern "C" int nwcch_customize(nwcch_dev* h, const int32_t* w, int32_t* ud, void
       return NWCCH_OK;
     }
     if (!d.cust_exec[slot]) {
-      // capture the launch sequence once on the internal stream, for this slot
-      HIP_OK(hipStreamBeginCapture(d.cap_stream, hipStreamCaptureModeThreadLocal));
-      try {
-        launch_customize(d, P, d.cap_stream);
-      } catch (...) {
-        hipGraph_t g = nullptr;
-        (void)hipStreamEndCapture(d.cap_stream, &g);
-        if (g) (void)hipGraphDestroy(g);
-        throw;
+      // capture the launch sequence once for this slot, on a stream that lives
+      // only for the capture (a lasting extra stream would share a hardware
+      // queue with the caller's streams)
+      hipStream_t cap = nullptr;
+      HIP_OK(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+      hipGraph_t g = nullptr;
+      hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
+      if (e == hipSuccess) {
+        try {
+          launch_customize(d, P, cap);
+        } catch (...) {
+          (void)hipStreamEndCapture(cap, &g);
+          if (g) (void)hipGraphDestroy(g);
+          (void)hipStreamDestroy(cap);
+          throw;
+        }
+        e = hipStreamEndCapture(cap, &g);
       }
-      HIP_OK(hipStreamEndCapture(d.cap_stream, &d.cust_graph[slot]));
+      (void)hipStreamDestroy(cap);
+      HIP_OK(e);
+      d.cust_graph[slot] = g;
       HIP_OK(hipGraphInstantiate(&d.cust_exec[slot], d.cust_graph[slot], nullptr, nullptr, 0));
     }
     HIP_OK(hipGraphLaunch(d.cust_exec[slot], s));

@@ -150,7 +150,6 @@ struct DevTopo {
   std::vector<hipGraphExec_t> cust_exec;         // per slot, nullptr until built
   std::vector<hipGraph_t> cust_graph;
   int32_t cust_next = 0;                         // next slot handed over once all are taken
-  hipStream_t cap_stream = nullptr;              // internal stream the graphs are captured on
   int32_t customize_graph = 1;                   // 0: launch the kernels one by one
   bool prof_on = false;
   std::vector<hipEvent_t> ev_pool;
