@@ -220,7 +220,8 @@ int nwcch_query_path(nwcch_dev* h, const int32_t* ud, const int32_t* s_rank, con
  * sweeps over the hierarchy.  Replaces cch_potential (cpp/cch.cpp:635-655) and
  * CCH.potential (python/cch.py:734-755, which returns it by original id).
  * Valid for metrics without negative cycles; a value below -(2^31-1) raises
- * NWCCH_STATUS_OVERFLOW. */
+ * NWCCH_STATUS_OVERFLOW.  Its launches are replayed from a per-stream captured
+ * graph, as nwcch_customize's are. */
 int nwcch_potential(nwcch_dev* h, const int32_t* ud, int32_t* p, void* stream);
 
 /* Build (once, on the host, or on the device for a topology made with
@@ -276,8 +277,9 @@ int nwcch_status(nwcch_dev* h, uint32_t* out, int clear, void* stream);
  *                   schedule order (default).
  *   "batch_xcd"     batched sweep: 1 = each XCD takes one contiguous slice of a
  *                   wave class, 0 = workgroups interleaved (default).
- *   "customize_graph"  1 = nwcch_customize replays a captured graph per
- *                   stream (default), 0 = it launches kernel by kernel.
+ *   "customize_graph"  1 = nwcch_customize and nwcch_potential replay a
+ *                   captured graph per stream (default), 0 = they launch
+ *                   kernel by kernel.
  *   "query_wpb"     queries (wavefronts) per workgroup of the query kernels:
  *                   1 (default), 2 or 4, fewer where their labels would pass
  *                   64 KB of LDS.

@@ -112,9 +112,9 @@ const std::vector<LevelLaunch>& level_launches(DevTopo& d) {
 // INF if none.  One launch over all arcs; the level kernels start from it.
 __global__ void __launch_bounds__(256) k_base_all(i64 mp, const int32_t* __restrict__ ae_off,
                                                   const int32_t* __restrict__ ae_idx,
-                                                  const CustPtrs* __restrict__ P, uint32_t* status) {
-  const int32_t* __restrict__ w = P->w;
-  int2* ud = P->ud;
+                                                  const SlotPtrs* __restrict__ P, uint32_t* status) {
+  const int32_t* __restrict__ w = static_cast<const int32_t*>(P->in);
+  int2* ud = static_cast<int2*>(P->out);
   uint32_t st = 0;
   const i64 stride = (i64)gridDim.x * blockDim.x;
   for (i64 c = (i64)blockIdx.x * blockDim.x + threadIdx.x; c < mp; c += stride) {
@@ -132,11 +132,11 @@ __global__ void __launch_bounds__(256) k_base_all(i64 mp, const int32_t* __restr
   if (st) atomicOr(status, st);
 }
 
-// The buffers of one customization call into its argument slot (a kernel, so
-// the values travel with the launch in stream order).
-__global__ void k_set_cust_args(CustPtrs* P, const int32_t* w, int2* ud) {
-  P->w = w;
-  P->ud = ud;
+// The buffers of one call into its argument slot (a kernel, so the values
+// travel with the launch in stream order).
+__global__ void k_set_slot(SlotPtrs* P, const void* in, void* out) {
+  P->in = in;
+  P->out = out;
 }
 
 // One level of customization: F_c of every arc of the level, i.e. the min of its
@@ -148,8 +148,8 @@ __global__ void k_set_cust_args(CustPtrs* P, const int32_t* w, int2* ud) {
 // no atomics and every arc is written exactly once.
 __global__ void __launch_bounds__(256) k_customize_level(LevelLaunch L, const PosRec* __restrict__ prec,
                                                          const int2* __restrict__ tri_ab,
-                                                         const CustPtrs* __restrict__ P, uint32_t* status) {
-  int2* ud = P->ud;
+                                                         const SlotPtrs* __restrict__ P, uint32_t* status) {
+  int2* ud = static_cast<int2*>(P->out);
   const int32_t b = blockIdx.x;
   int k = 0;
   while (k < kTeamClasses - 1 && b >= L.blk[k + 1]) k++;
@@ -212,9 +212,9 @@ __global__ void __launch_bounds__(256) k_customize_level(LevelLaunch L, const Po
 template <int kStore>
 __global__ void __launch_bounds__(256) k_customize_flat(i64 q0, i64 nt, const int2* __restrict__ tri_ab,
                                                         const int32_t* __restrict__ tri_c,
-                                                        const CustPtrs* __restrict__ P, uint32_t* status,
+                                                        const SlotPtrs* __restrict__ P, uint32_t* status,
                                                         int xcd_slices) {
-  int2* ud = P->ud;
+  int2* ud = static_cast<int2*>(P->out);
   const int lane = threadIdx.x & 63;
   const unsigned per = gridDim.x >> 3;
   const unsigned slice = xcd_slices ? (blockIdx.x & 7u) * per + (blockIdx.x >> 3) : blockIdx.x;
@@ -461,15 +461,19 @@ __global__ void k_scan_finish(const unsigned long long* src, long long* dst) {
 // into it, and the level pushes only into higher heads; downward, a tail pulls
 // from heads of higher levels, already final.  Labels are indexed by rank and
 // formed in 64 bits; a value below -(2^31-1) raises the overflow bit.
-__global__ void __launch_bounds__(256) k_pot_init(int32_t* p, i64 n) {
+__global__ void __launch_bounds__(256) k_pot_init(const SlotPtrs* __restrict__ P, i64 n) {
+  int32_t* p = static_cast<int32_t*>(P->out);
   const i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0;
 }
 
 __global__ void __launch_bounds__(256) k_pot_level(const int32_t* __restrict__ sched_arc,
                                                    const int32_t* __restrict__ tail,
-                                                   const int32_t* __restrict__ head, const int2* __restrict__ ud,
-                                                   int32_t* p, i64 begin, i64 end, int down, uint32_t* status) {
+                                                   const int32_t* __restrict__ head,
+                                                   const SlotPtrs* __restrict__ P, i64 begin, i64 end, int down,
+                                                   uint32_t* status) {
+  const int2* __restrict__ ud = static_cast<const int2*>(P->in);
+  int32_t* p = static_cast<int32_t*>(P->out);
   const i64 q = begin + (i64)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= end) return;
   const int32_t a = sched_arc[q];
@@ -1301,7 +1305,8 @@ extern "C" int nwcch_dev_create(const nwcch_topo* topo, int device, nwcch_dev** 
     d.arc_pos = dev_upload(d, t.arc_pos.data(), t.arc_pos.size());
     d.status = dev_alloc<uint32_t>(d, 1);
     HIP_OK(hipMemset(d.status, 0, sizeof(uint32_t)));
-    d.cust_args = dev_alloc<CustPtrs>(d, kCustSlots);
+    d.cust.args = dev_alloc<SlotPtrs>(d, kCustSlots);
+    d.pot.args = dev_alloc<SlotPtrs>(d, kCustSlots);
     d.scan = dev_alloc<unsigned long long>(d, 4);
     HIP_OK(hipDeviceSynchronize());
   } catch (const std::exception& e) {
@@ -1376,7 +1381,7 @@ namespace {
 
 // The base pass and the level launches of one customization, reading the
 // metric buffers from argument slot P.
-void launch_customize(DevTopo& d, const CustPtrs* P, hipStream_t s) {
+void launch_customize(DevTopo& d, const SlotPtrs* P, hipStream_t s) {
   {
     ProfScope ps(d, s, kKOther);
     const i64 blocks = std::min<i64>(std::max<i64>((d.mp + 255) / 256, 1), 16384);
@@ -1411,28 +1416,27 @@ void launch_customize(DevTopo& d, const CustPtrs* P, hipStream_t s) {
   HIP_OK(hipGetLastError());
 }
 
-// Argument slot of a stream: one per stream, so calls on different streams never
-// share one.
+// Argument slot of a stream in a cache: one per stream, so calls on different
+// streams never share one.
 bool stream_capturing(hipStream_t s);
-void drop_slot_graph(DevTopo& d, int i);
+void drop_slot_graph(GraphCache& c, int i);
 
-int cust_slot(DevTopo& d, hipStream_t s) {
-  for (size_t i = 0; i < d.cust_stream.size(); i++)
-    if (d.cust_stream[i] == (void*)s) return (int)i;
-  if ((int)d.cust_stream.size() < kCustSlots) {
-    d.cust_stream.push_back((void*)s);
-    d.cust_exec.push_back(nullptr);
-    d.cust_graph.push_back(nullptr);
-    return (int)d.cust_stream.size() - 1;
+int cache_slot(GraphCache& c, hipStream_t s) {
+  for (size_t i = 0; i < c.stream.size(); i++)
+    if (c.stream[i] == (void*)s) return (int)i;
+  if ((int)c.stream.size() < kCustSlots) {
+    c.stream.push_back((void*)s);
+    c.exec.push_back(nullptr);
+    c.graph.push_back(nullptr);
+    return (int)c.stream.size() - 1;
   }
   // every slot taken: hand the oldest one over, once the device has finished
   // with it (impossible while the caller captures a graph)
-  if (stream_capturing(s))
-    throw std::runtime_error("nwcch_customize: more streams than argument slots while capturing");
+  if (stream_capturing(s)) throw std::runtime_error("more streams than argument slots while capturing");
   HIP_OK(hipDeviceSynchronize());
-  const int i = d.cust_next++ % kCustSlots;
-  drop_slot_graph(d, i);
-  d.cust_stream[i] = (void*)s;
+  const int i = c.next++ % kCustSlots;
+  drop_slot_graph(c, i);
+  c.stream[i] = (void*)s;
   return i;
 }
 
@@ -1442,21 +1446,61 @@ bool stream_capturing(hipStream_t s) {
   return st != hipStreamCaptureStatusNone;
 }
 
-// Drops every captured customization (launch shapes changed, or the handle
-// goes away).  Waits for the device first: a graph may still be running.
-void drop_slot_graph(DevTopo& d, int i) {
-  if (d.cust_exec[i]) (void)hipGraphExecDestroy(d.cust_exec[i]);
-  if (d.cust_graph[i]) (void)hipGraphDestroy(d.cust_graph[i]);
-  d.cust_exec[i] = nullptr;
-  d.cust_graph[i] = nullptr;
+void drop_slot_graph(GraphCache& c, int i) {
+  if (c.exec[i]) (void)hipGraphExecDestroy(c.exec[i]);
+  if (c.graph[i]) (void)hipGraphDestroy(c.graph[i]);
+  c.exec[i] = nullptr;
+  c.graph[i] = nullptr;
 }
 
+// Drops every captured launch sequence (launch shapes changed, or the handle
+// goes away).  Waits for the device first: a graph may still be running.
 void drop_cust_graphs(DevTopo& d) {
   bool any = false;
-  for (hipGraphExec_t e : d.cust_exec) any |= e != nullptr;
+  for (GraphCache* c : {&d.cust, &d.pot})
+    for (hipGraphExec_t e : c->exec) any |= e != nullptr;
   if (!any) return;
   (void)hipDeviceSynchronize();
-  for (size_t i = 0; i < d.cust_exec.size(); i++) drop_slot_graph(d, (int)i);
+  for (GraphCache* c : {&d.cust, &d.pot})
+    for (size_t i = 0; i < c->exec.size(); i++) drop_slot_graph(*c, (int)i);
+}
+
+// One call through a cache: its buffers into the stream's slot, then the
+// stream's captured launch sequence (captured on first use, on a stream that
+// lives only for the capture -- a lasting extra stream would share a hardware
+// queue with the caller's streams).  With per-launch profiling on, graphs off,
+// or inside the caller's own capture, the launches go out one by one.
+template <class Launch>
+void run_cached(DevTopo& d, GraphCache& c, hipStream_t s, const void* in, void* out, Launch launch) {
+  const int slot = cache_slot(c, s);
+  SlotPtrs* P = c.args + slot;
+  hipLaunchKernelGGL(k_set_slot, dim3(1), dim3(1), 0, s, P, in, out);
+  if (!d.customize_graph || d.prof_on || stream_capturing(s)) {
+    launch(P, s);
+    return;
+  }
+  if (!c.exec[slot]) {
+    hipStream_t cap = nullptr;
+    HIP_OK(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+      try {
+        launch(P, cap);
+      } catch (...) {
+        (void)hipStreamEndCapture(cap, &g);
+        if (g) (void)hipGraphDestroy(g);
+        (void)hipStreamDestroy(cap);
+        throw;
+      }
+      e = hipStreamEndCapture(cap, &g);
+    }
+    (void)hipStreamDestroy(cap);
+    HIP_OK(e);
+    c.graph[slot] = g;
+    HIP_OK(hipGraphInstantiate(&c.exec[slot], c.graph[slot], nullptr, nullptr, 0));
+  }
+  HIP_OK(hipGraphLaunch(c.exec[slot], s));
 }
 
 }  // namespace
@@ -1465,39 +1509,8 @@ extern "C" int nwcch_customize(nwcch_dev* h, const int32_t* w, int32_t* ud, void
   if (!h || !w || !ud) { set_error("null argument"); return NWCCH_ERR_ARG; }
   try {
     DevTopo& d = h->d;
-    hipStream_t s = (hipStream_t)stream;
-    const int slot = cust_slot(d, s);
-    CustPtrs* P = d.cust_args + slot;
-    hipLaunchKernelGGL(k_set_cust_args, dim3(1), dim3(1), 0, s, P, w, reinterpret_cast<int2*>(ud));
-    if (!d.customize_graph || d.prof_on || stream_capturing(s)) {
-      launch_customize(d, P, s);   // per-launch profiling, or inside the caller's own capture
-      return NWCCH_OK;
-    }
-    if (!d.cust_exec[slot]) {
-      // capture the launch sequence once for this slot, on a stream that lives
-      // only for the capture (a lasting extra stream would share a hardware
-      // queue with the caller's streams)
-      hipStream_t cap = nullptr;
-      HIP_OK(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
-      hipGraph_t g = nullptr;
-      hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
-      if (e == hipSuccess) {
-        try {
-          launch_customize(d, P, cap);
-        } catch (...) {
-          (void)hipStreamEndCapture(cap, &g);
-          if (g) (void)hipGraphDestroy(g);
-          (void)hipStreamDestroy(cap);
-          throw;
-        }
-        e = hipStreamEndCapture(cap, &g);
-      }
-      (void)hipStreamDestroy(cap);
-      HIP_OK(e);
-      d.cust_graph[slot] = g;
-      HIP_OK(hipGraphInstantiate(&d.cust_exec[slot], d.cust_graph[slot], nullptr, nullptr, 0));
-    }
-    HIP_OK(hipGraphLaunch(d.cust_exec[slot], s));
+    run_cached(d, d.cust, (hipStream_t)stream, w, ud,
+               [&](const SlotPtrs* P, hipStream_t s) { launch_customize(d, P, s); });
   } catch (const std::exception& e) {
     return fail(e);
   }
@@ -1645,20 +1658,20 @@ extern "C" int nwcch_potential(nwcch_dev* h, const int32_t* ud, int32_t* p, void
   if (!h || !ud || !p) { set_error("null argument"); return NWCCH_ERR_ARG; }
   try {
     DevTopo& d = h->d;
-    hipStream_t s = (hipStream_t)stream;
-    const int2* u = reinterpret_cast<const int2*>(ud);
-    hipLaunchKernelGGL(k_pot_init, dim3((unsigned)std::max<i64>((d.n + 255) / 256, 1)), dim3(256), 0, s, p, d.n);
-    const int32_t L = (int32_t)d.lv.size();
-    for (int pass = 0; pass < 2; pass++) {
-      for (int32_t i = 0; i < L; i++) {
-        const LevelRange& r = d.lv[pass == 0 ? i : L - 1 - i];
-        if (r.end == r.begin) continue;
-        ProfScope ps(d, s, kKOther);
-        hipLaunchKernelGGL(k_pot_level, dim3((unsigned)((r.end - r.begin + 255) / 256)), dim3(256), 0, s,
-                           d.sched_arc, d.tail, d.head, u, p, r.begin, r.end, pass, d.status);
+    run_cached(d, d.pot, (hipStream_t)stream, ud, p, [&](const SlotPtrs* P, hipStream_t s) {
+      hipLaunchKernelGGL(k_pot_init, dim3((unsigned)std::max<i64>((d.n + 255) / 256, 1)), dim3(256), 0, s, P, d.n);
+      const int32_t L = (int32_t)d.lv.size();
+      for (int pass = 0; pass < 2; pass++) {
+        for (int32_t i = 0; i < L; i++) {
+          const LevelRange& r = d.lv[pass == 0 ? i : L - 1 - i];
+          if (r.end == r.begin) continue;
+          ProfScope ps(d, s, kKOther);
+          hipLaunchKernelGGL(k_pot_level, dim3((unsigned)((r.end - r.begin + 255) / 256)), dim3(256), 0, s,
+                             d.sched_arc, d.tail, d.head, P, r.begin, r.end, pass, d.status);
+        }
       }
-    }
-    HIP_OK(hipGetLastError());
+      HIP_OK(hipGetLastError());
+    });
   } catch (const std::exception& e) {
     return fail(e);
   }

@@ -30,13 +30,24 @@ struct VInfo {
   int32_t first, deg, parent, depth;
 };
 
-// The metric buffers of one customization call, read by its kernels from
-// device memory, so that a captured launch sequence serves any buffers.
-struct CustPtrs {
-  const int32_t* w;
-  int2* ud;
+// The input and output buffer of one call, read by its kernels from device
+// memory, so that a captured launch sequence serves any buffers
+// (customization: w -> ud; CCH potential: ud -> p).
+struct SlotPtrs {
+  const void* in;
+  void* out;
 };
 constexpr int kCustSlots = 256;  // streams with their own argument slot and graph
+
+// Launch sequences replayed from HIP graphs, one per stream: slot i holds the
+// buffers of calls on stream[i] and that stream's instantiated graph.
+struct GraphCache {
+  SlotPtrs* args = nullptr;                // [kCustSlots] device
+  std::vector<void*> stream;               // stream of each slot in use
+  std::vector<hipGraphExec_t> exec;        // per slot, nullptr until built
+  std::vector<hipGraph_t> graph;
+  int32_t next = 0;                        // next slot handed over once all are taken
+};
 
 // Launch geometry of one level of the single-metric sweep: team class k
 // (64 >> k lanes per arc) owns schedule positions [cls[k], cls[k+1]) and blocks
@@ -143,13 +154,8 @@ struct DevTopo {
 
   // optional launch profiling: every launch is bracketed by two events
   struct Rec { int kernel; hipEvent_t a, b; };
-  // customization replayed from a HIP graph per stream: slot i holds the
-  // arguments of calls on cust_stream[i] and that stream's instantiated graph
-  CustPtrs* cust_args = nullptr;                 // [kCustSlots] device
-  std::vector<void*> cust_stream;                // stream of each slot in use
-  std::vector<hipGraphExec_t> cust_exec;         // per slot, nullptr until built
-  std::vector<hipGraph_t> cust_graph;
-  int32_t cust_next = 0;                         // next slot handed over once all are taken
+  // customization and CCH potential replayed from HIP graphs per stream
+  GraphCache cust, pot;
   int32_t customize_graph = 1;                   // 0: launch the kernels one by one
   bool prof_on = false;
   std::vector<hipEvent_t> ev_pool;

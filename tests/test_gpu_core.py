@@ -227,6 +227,12 @@ def test_potential_equals_johnson(c0, which):
     assert np.array_equal(p, want)
     tails = np.repeat(np.arange(g.n), np.diff(g.off))
     assert (g.w.astype(np.int64) + p[tails] - p[g.tgt] >= 0).all()
+    # the replayed launch sequence with other buffers, and launch by launch
+    ud2 = dt.customize(g.w)
+    assert torch.equal(dt.potential(ud2), dt.potential(ud))
+    dt.set_tuning("customize_graph", 0)
+    assert np.array_equal(dt.potential(ud).cpu().numpy(), pr)
+    dt.set_tuning("customize_graph", 1)
 
 
 def test_customize_graph_paths(c0):
