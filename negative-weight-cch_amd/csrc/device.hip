@@ -1456,6 +1456,7 @@ void drop_slot_graph(GraphCache& c, int i) {
 // Drops every captured launch sequence (launch shapes changed, or the handle
 // goes away).  Waits for the device first: a graph may still be running.
 void drop_cust_graphs(DevTopo& d) {
+  std::lock_guard<std::mutex> lock(d.graph_mu);
   bool any = false;
   for (GraphCache* c : {&d.cust, &d.pot})
     for (hipGraphExec_t e : c->exec) any |= e != nullptr;
@@ -1472,6 +1473,7 @@ void drop_cust_graphs(DevTopo& d) {
 // or inside the caller's own capture, the launches go out one by one.
 template <class Launch>
 void run_cached(DevTopo& d, GraphCache& c, hipStream_t s, const void* in, void* out, Launch launch) {
+  std::lock_guard<std::mutex> lock(d.graph_mu);
   const int slot = cache_slot(c, s);
   SlotPtrs* P = c.args + slot;
   hipLaunchKernelGGL(k_set_slot, dim3(1), dim3(1), 0, s, P, in, out);

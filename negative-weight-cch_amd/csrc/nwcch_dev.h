@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -156,6 +157,7 @@ struct DevTopo {
   struct Rec { int kernel; hipEvent_t a, b; };
   // customization and CCH potential replayed from HIP graphs per stream
   GraphCache cust, pot;
+  std::mutex graph_mu;                           // guards both caches (calls from several host threads)
   int32_t customize_graph = 1;                   // 0: launch the kernels one by one
   bool prof_on = false;
   std::vector<hipEvent_t> ev_pool;
