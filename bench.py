@@ -258,8 +258,10 @@ def plain(args, out):
 
     cpu = None
     if not args.no_cpu:
-        ncq = min(args.queries, 20000)
-        cpu = cpu_baseline(g, rank, topo, queries=(qs_h[:ncq], qt_h[:ncq], dist[:ncq].cpu().numpy()))
+        # a sample of about 10 s of host work: 15 customizations each way and
+        # 3 x 50,000 reference queries
+        ncq = min(args.queries, 50000)
+        cpu = cpu_baseline(g, rank, topo, reps=15, queries=(qs_h[:ncq], qt_h[:ncq], dist[:ncq].cpu().numpy()))
     prep = preprocessing(g, rank, topo, host, dt, updates["prepare_s"])
 
     out.update({
