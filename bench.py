@@ -845,7 +845,7 @@ def main():
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--serve-rates", type=float, nargs="+", default=[200_000, 1_000_000, 2_000_000])
     ap.add_argument("--serve-seconds", type=float, default=0.25)
-    ap.add_argument("--serve-inflight", type=int, default=2, help="query launches on the device at once")
+    ap.add_argument("--serve-inflight", type=int, default=3, help="query launches on the device at once")
     ap.add_argument("--serve-streams", type=int, default=2, help="streams the query launches rotate over")
     ap.add_argument("--serve-update-rate", type=float, default=40.0)
     args = ap.parse_args()

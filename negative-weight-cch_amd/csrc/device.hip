@@ -1689,6 +1689,11 @@ extern "C" int nwcch_set_tuning(nwcch_dev* h, const char* key, int32_t value) {
     h->d.customize_graph = value;
     return NWCCH_OK;
   }
+  if (k == "upd_max_blocks") {
+    if (value < 1 || value > 65536) { set_error("upd_max_blocks is 1..65536"); return NWCCH_ERR_ARG; }
+    h->d.upd_max_blocks = value;
+    return NWCCH_OK;
+  }
   if (k == "upd_persistent_max") {
     if (value < 0) { set_error("upd_persistent_max must be >= 0"); return NWCCH_ERR_ARG; }
     h->d.upd_persistent_max = value;
@@ -1751,6 +1756,7 @@ extern "C" int nwcch_get_tuning(const nwcch_dev* h, const char* key, int32_t* va
   else if (k == "query_deep_max") *value = h->d.query_deep_max;
   else if (k == "query_chunk") *value = h->d.query_chunk;
   else if (k == "upd_persistent_max") *value = h->d.upd_persistent_max;
+  else if (k == "upd_max_blocks") *value = h->d.upd_max_blocks;
   else { set_error("unknown tuning key: " + k); return NWCCH_ERR_ARG; }
   return NWCCH_OK;
 }

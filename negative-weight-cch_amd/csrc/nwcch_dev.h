@@ -90,6 +90,7 @@ struct DevTopo {
   int32_t batch_tri_per_lane = 8;            // batched level kernel
   int32_t batch_order = 0;                   // 1: arcs of a wave class in id order (bpos)
   int32_t batch_xcd = 0;                     // 1: each XCD takes a contiguous slice of a class
+  int32_t upd_max_blocks = 1024;             // workgroups of a per-height update launch, at most
   int32_t upd_persistent_max = 2;            // updates of at most this many changed arcs run
                                              // in one persistent workgroup (mode auto)
   int32_t query_chunk = 0;                   // queries read ancestor chains 32 at a time
@@ -133,8 +134,9 @@ struct DevTopo {
   int32_t* qcnt = nullptr;           // [levels] bucket fill
   int32_t* chg = nullptr;            // [mp] changed arcs of the last update (unordered)
   long long* upd_res = nullptr;      // [4] evaluated, changed, negative changed, first negative
-  // staging of the changes: a ring of slots (device + pinned host buffers,
-  // grown on demand), each reused only after its previous copy has completed
+  // staging of the changes: a ring of slots (device + pinned host buffers of
+  // kStageCap0 changes from nwcch_update_prepare on, grown for larger
+  // updates), each reused only after its previous copy has completed
   struct Stage {
     i64* e = nullptr;
     int32_t* w = nullptr;
@@ -144,6 +146,7 @@ struct DevTopo {
     hipEvent_t ev = nullptr;
   };
   static constexpr int kStages = 8;
+  static constexpr i64 kStageCap0 = 4096;
   Stage stage[kStages];
   int stage_next = 0;
   std::vector<void*> owned_host;     // pinned host allocations

@@ -327,6 +327,24 @@ extern "C" int nwcch_update_prepare(nwcch_dev* h, nwcch_topo* topo) {
     d.qcnt = up_alloc<int32_t>(d, t.levels);
     d.chg = up_alloc<int32_t>(d, t.mp);
     d.upd_res = up_alloc<long long>(d, 4);
+    // staging slots sized up front: growing one later synchronises the
+    // caller's stream and allocates pinned memory in the middle of its work
+    for (auto& sg : d.stage) {
+      const i64 cap = DevTopo::kStageCap0;
+      sg.e = up_alloc<i64>(d, cap);
+      sg.w = up_alloc<int32_t>(d, cap);
+      void* he = nullptr;
+      void* hw = nullptr;
+      HIP_OK(hipHostMalloc(&he, cap * sizeof(i64), hipHostMallocDefault));
+      HIP_OK(hipHostMalloc(&hw, cap * sizeof(int32_t), hipHostMallocDefault));
+      d.owned_host.push_back(he);
+      d.owned_host.push_back(hw);
+      sg.he = (i64*)he;
+      sg.hw = (int32_t*)hw;
+      sg.cap = cap;
+      HIP_OK(hipEventCreateWithFlags(&sg.ev, hipEventDisableTiming));
+      HIP_OK(hipEventRecord(sg.ev, 0));
+    }
     HIP_OK(hipDeviceSynchronize());
     d.has_update = true;
   } catch (const std::exception& e) {
@@ -362,8 +380,7 @@ extern "C" int nwcch_update(nwcch_dev* h, int32_t* w, int32_t* ud, const int64_t
     // waits on the host only when kStages earlier ones are still queued.
     DevTopo::Stage& sg = d.stage[d.stage_next];
     d.stage_next = (d.stage_next + 1) % DevTopo::kStages;
-    if (sg.ev) HIP_OK(hipEventSynchronize(sg.ev));
-    else HIP_OK(hipEventCreateWithFlags(&sg.ev, hipEventDisableTiming));
+    HIP_OK(hipEventSynchronize(sg.ev));
     if (k > sg.cap) {
       i64 cap = std::max<i64>(k, 2 * sg.cap);
       HIP_OK(hipStreamSynchronize(s));
@@ -410,7 +427,7 @@ extern "C" int nwcch_update(nwcch_dev* h, int32_t* w, int32_t* ud, const int64_t
       } else {
         for (int32_t L = L0; L < d.levels; L++) {
           const i64 cap = d.lv[L].end - d.lv[L].begin;
-          const int blocks = (int)std::max<i64>(1, std::min<i64>((cap + 3) / 4, 1024));
+          const int blocks = (int)std::max<i64>(1, std::min<i64>((cap + 3) / 4, d.upd_max_blocks));
           ProfScope ps(d, s, kKUpdate);
           hipLaunchKernelGGL(k_upd_level, dim3(blocks), dim3(256), 0, s, A, L);
         }

@@ -157,7 +157,7 @@ class Server:
             self.used -= self.metric_bytes()
 
     # ------------------------------------------------------------ serving
-    def run(self, requests: list[Request], max_seconds: float = 60.0, max_inflight: int = 2) -> dict:
+    def run(self, requests: list[Request], max_seconds: float = 60.0, max_inflight: int = 3) -> dict:
         """Serve `requests` in real time (arrival times in seconds from the start
         of the call); returns statistics and writes the results into the
         Request objects.  At most `max_inflight` query launches are on the
