@@ -277,6 +277,8 @@ int nwcch_status(nwcch_dev* h, uint32_t* out, int clear, void* stream);
  *                   schedule order (default).
  *   "batch_xcd"     batched sweep: 1 = each XCD takes one contiguous slice of a
  *                   wave class, 0 = workgroups interleaved (default).
+ *   "upd_max_blocks"  workgroups of one per-height launch of a large update,
+ *                   at most (default 1024).
  *   "customize_graph"  1 = nwcch_customize and nwcch_potential replay a
  *                   captured graph per stream (default), 0 = they launch
  *                   kernel by kernel.
