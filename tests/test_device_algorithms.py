@@ -1,0 +1,94 @@
+"""CPU restatements of two device algorithms, checked against brute force and
+the host build: the segmented prefix minimum of the level kernel (DPP row
+shifts and row broadcasts over a 64-lane wavefront) and the pull-form
+triangle enumeration of the GPU-built index (down lists, binary search in the
+lower row, x ascending).  The GPU tests check the kernels themselves bit for
+bit; these pin the networks they implement."""
+import numpy as np
+
+from conftest import order_of, small_graphs
+from nwcch import _lib
+
+INF = 2**31 - 1
+
+
+def _dpp(vals, op, row_mask):
+    """One DPP move over a wave64: row_shr:n inside 16-lane rows, or
+    row_bcast:15 / row_bcast:31; lanes without a source or outside row_mask
+    receive INF (the identity of min)."""
+    out = np.full(64, INF, np.int64)
+    for lane in range(64):
+        row = lane // 16
+        if not (row_mask >> row) & 1:
+            continue
+        if op[0] == "shr":
+            if lane % 16 >= op[1]:
+                out[lane] = vals[lane - op[1]]
+        elif op[0] == "bcast15":
+            if row >= 1:
+                out[lane] = vals[16 * row - 1]
+        elif row >= 2:                                   # bcast31
+            out[lane] = vals[31]
+    return out
+
+
+def _source(lane, op):
+    if op[0] == "shr":
+        return lane - op[1]
+    return 16 * (lane // 16) - 1 if op[0] == "bcast15" else 31
+
+
+STEPS = ((("shr", 1), 0xF), (("shr", 2), 0xF), (("shr", 4), 0xF), (("shr", 8), 0xF),
+         (("bcast15",), 0xA), (("bcast31",), 0xC))
+
+
+def test_segmented_prefix_min_network():
+    """After the six steps, merging a source lane only when it lies in the same
+    run, lane l holds the minimum over [start of its run, l]."""
+    rng = np.random.default_rng(0)
+    for trial in range(3000):
+        cuts = np.sort(rng.choice(np.arange(1, 64), size=rng.integers(0, 40), replace=False))
+        arc = np.zeros(64, np.int64)
+        for k in cuts:
+            arc[k:] += 1
+        valid = rng.integers(1, 65)
+        arc[valid:] = -1                                  # lanes past the level's end
+        v = rng.integers(-100, 100, 64)
+        v[rng.random(64) < 0.1] = INF
+        v[valid:] = INF
+        starts = [lane for lane in range(64) if lane == 0 or arc[lane - 1] != arc[lane]]
+        s0 = np.array([max(h for h in starts if h <= lane) for lane in range(64)])
+        x = v.copy()
+        for op, mask in STEPS:
+            moved = _dpp(x, op, mask)
+            for lane in range(64):
+                if _source(lane, op) >= s0[lane]:
+                    x[lane] = min(x[lane], moved[lane])
+        for lane in range(64):
+            assert x[lane] == v[s0[lane]:lane + 1].min(), (trial, lane)
+
+
+def test_pull_enumeration_equals_host_index():
+    """For every arc (y, z): the entries a = (x, y) of y's down list, x
+    ascending, whose row x holds z after a -- exactly the host's run of that
+    arc, in the same order."""
+    g = small_graphs()[0]
+    t = _lib.Topology.from_graph(g.n, g.off, g.tgt, order_of(g))
+    ex = t.export()
+    first, head = ex["first"], ex["head"]
+    tail = np.repeat(np.arange(t.n), np.diff(first))
+    down = [[] for _ in range(t.n)]
+    for a in range(t.mp):                                 # ascending arc id = ascending x
+        down[head[a]].append(a)
+    sched, toff, tab = t.export_triangles()
+    for p in range(t.mp):
+        j = sched[p]
+        y, z = tail[j], head[j]
+        run = []
+        for a in down[y]:
+            x = tail[a]
+            row = head[a + 1:first[x + 1]]
+            k = np.searchsorted(row, z)
+            if k < len(row) and row[k] == z:
+                run.append((a, a + 1 + k))
+        assert [tuple(r) for r in tab[toff[p]:toff[p + 1]]] == run, p
