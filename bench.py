@@ -850,8 +850,8 @@ def main():
     ap.add_argument("--serve-update-rate", type=float, default=40.0)
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU design is "
-              "described in DESIGN.md and is not built", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); no path sharded over "
+              "several GPUs is built", file=sys.stderr)
         sys.exit(2)
     out = {
         "metric": "triangles/s customize; ET queries/s + p99 latency, batched/served vs plain GPU",
