@@ -394,14 +394,77 @@ def preprocessing(g, rank, topo, host, dt, t_upd_h):
     so_d, st_d = dd.export_sides()
     same &= bool(np.array_equal(so_h, so_d) and np.array_equal(st_h, st_d))
     del dd, td
+    # everything after the order on the GPU: chordal completion and arc map too
+    ts, t_topo_s = wall(lambda: _lib.Topology.from_graph(g.n, g.off, g.tgt, rank, device_index=0,
+                                                         device_symbolic=0))
+    same_s = same_structure(topo, ts)
+    del ts
     res = {"host_index": {"topology_s": host["topology_s"], "update_index_s": t_upd_h},
            "device_index": {"topology_s": t_topo_d, "dev_create_s": t_dev_d, "update_index_s": t_upd_d},
+           "device_symbolic_and_index": {"topology_s": t_topo_s, "identical": same_s},
            "identical": same,
            "what": "topology = symbolic contraction + arc map + triangle index + schedule (device index: "
-                   "triangle counts on the GPU, lists filled by dev_create); update_index = side lists"}
+                   "triangle counts on the GPU, lists filled by dev_create; device symbolic: chordal "
+                   "completion and arc map on the GPU); update_index = side lists"}
     log(f"preprocessing: host topology {host['topology_s']:.3f} s + side lists {t_upd_h:.3f} s; device-index "
-        f"topology {t_topo_d:.3f} s + dev_create {t_dev_d:.3f} s + side lists {t_upd_d:.3f} s; identical {same}")
+        f"topology {t_topo_d:.3f} s + dev_create {t_dev_d:.3f} s + side lists {t_upd_d:.3f} s; identical {same}; "
+        f"device symbolic + index topology {t_topo_s:.3f} s, identical {same_s}")
     return res
+
+
+def same_structure(a, b):
+    """Two topologies hold the same contraction, arc map, levels and schedule."""
+    import numpy as np
+    ea, eb = a.export(), b.export()
+    if (a.n, a.m, a.mp, a.triangles, a.levels) != (b.n, b.m, b.mp, b.triangles, b.levels):
+        return False
+    if any(not np.array_equal(ea[k], eb[k]) for k in ea):
+        return False
+    sa, oa, _ = a.export_triangles(with_list=False)
+    sb, ob, _ = b.export_triangles(with_list=False)
+    return bool(np.array_equal(sa, sb) and np.array_equal(oa, ob))
+
+
+def prep(args, out):
+    """Everything after the order for one configuration, timed three ways: on
+    the host; with the triangle index on the GPU; with the chordal completion,
+    the arc map and the triangle index on the GPU (one line per phase on
+    stderr).  All three must give the same structure."""
+    import numpy as np
+    import torch
+
+    from nwcch import _lib, graphs
+
+    os.environ["NWCCH_PROGRESS"] = "1"
+    cfg = args.config
+    t0 = time.perf_counter()
+    g = graphs.make_config(cfg, progress=lambda msg: log(f"config {cfg}: {msg}"))
+    log(f"config {cfg}: graph n={g.n} m={g.m} generated in {time.perf_counter() - t0:.1f}s")
+    t0 = time.perf_counter()
+    rank = _lib.order(g.n, g.off, g.tgt, g.x, g.y, 64)
+    log(f"config {cfg}: nested dissection {time.perf_counter() - t0:.1f}s")
+    torch.cuda.init()
+    torch.empty(1, device="cuda:0")
+    sm = graphs.ev_grid(6, 0)     # first use of the GPU by the library, outside the timings
+    _lib.Topology.from_graph(sm.n, sm.off, sm.tgt, np.arange(sm.n, dtype=np.int32), device_index=0,
+                             device_symbolic=0)
+    res, ref = {}, None
+    for name, kw in (("host", {}), ("device_index", {"device_index": 0}),
+                     ("device_symbolic_and_index", {"device_index": 0, "device_symbolic": 0})):
+        log(f"config {cfg}: topology, {name}")
+        t0 = time.perf_counter()
+        t = _lib.Topology.from_graph(g.n, g.off, g.tgt, rank, **kw)
+        dt = time.perf_counter() - t0
+        res[name] = {"topology_s": dt}
+        if ref is None:
+            ref = t
+            res["structure"] = {"n": g.n, "m": g.m, "mp": t.mp, "triangles": t.triangles, "depth": t.depth,
+                                "levels": t.levels}
+        else:
+            res[name]["identical"] = same_structure(ref, t)
+            del t
+        log(f"config {cfg}: {name} {dt:.2f} s {res[name]}")
+    out["prep"] = res
 
 
 def cpu_model():
@@ -836,6 +899,8 @@ def main():
     ap.add_argument("--no-graph", action="store_true", help="skip the HIP graph replay of the step")
     ap.add_argument("--full-scale", action="store_true",
                     help="run the 23.9M-point road graph (config 4) instead of the default workloads")
+    ap.add_argument("--prep", action="store_true",
+                    help="time everything after the order for --config on the host and on the GPU")
     ap.add_argument("--full-queries", type=int, default=1_000_000)
     ap.add_argument("--full-check", type=int, default=100, help="queries checked against Johnson + Dijkstra")
     ap.add_argument("--no-batched", action="store_true")
@@ -861,6 +926,10 @@ def main():
     }
     if args.full_scale:
         full_scale(args, out)
+        print(json.dumps(out), flush=True)
+        return
+    if args.prep:
+        prep(args, out)
         print(json.dumps(out), flush=True)
         return
     plain(args, out)

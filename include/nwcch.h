@@ -72,8 +72,19 @@ int nwcch_topo_create(int64_t n, const int64_t* off, const int32_t* tgt, const i
  * lists in device memory, and nwcch_update_prepare fills the side lists
  * (python/cch.py:309-324) there.  Every array equals the host-built one.
  * nwcch_topo_export_triangles then serves sched_arc and tri_off only;
- * nwcch_dev_export_triangles reads the list back.  A GPU is needed. */
+ * nwcch_dev_export_triangles reads the list back.  A GPU is needed.
+ *
+ * flags & NWCCH_TOPO_DEVICE_SYMBOLIC: the symbolic contraction
+ * (python/cch.py:136-166, python/cch_nb.py:431-481, cpp/cch.cpp:377-420) and
+ * the original-arc map (python/cch.py:188-204, python/cch_nb.py:496-515) are
+ * built on GPU `device`.  The host computes the elimination tree from the
+ * original graph first (Liu's union-find walk, near-linear); the GPU then
+ * builds the rows by ET height, leaves first.  first/head/e_arc/e_up are
+ * identical to the host build.  An elimination tree deeper than 15,360 is
+ * contracted on the host instead (the GPU's per-vertex bitmap lives in LDS).
+ * Either flag may be given without the other.  A GPU is needed. */
 #define NWCCH_TOPO_DEVICE_INDEX 1
+#define NWCCH_TOPO_DEVICE_SYMBOLIC 2
 int nwcch_topo_create_ex(int64_t n, const int64_t* off, const int32_t* tgt, const int32_t* rank,
                          int32_t flags, int device, nwcch_topo** out);
 

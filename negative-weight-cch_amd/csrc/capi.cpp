@@ -97,17 +97,27 @@ extern "C" int nwcch_topo_create_ex(int64_t n, const int64_t* off, const int32_t
     set_error("nwcch_topo_create: bad CSR or null argument");
     return NWCCH_ERR_ARG;
   }
-  if (flags & ~NWCCH_TOPO_DEVICE_INDEX) { set_error("nwcch_topo_create_ex: unknown flags"); return NWCCH_ERR_ARG; }
+  if (flags & ~(NWCCH_TOPO_DEVICE_INDEX | NWCCH_TOPO_DEVICE_SYMBOLIC)) {
+    set_error("nwcch_topo_create_ex: unknown flags");
+    return NWCCH_ERR_ARG;
+  }
   nwcch_topo* h = new nwcch_topo();
   try {
     Phases ph;
     std::vector<i64> noff;
     std::vector<i32> nbr;
     undirected_csr(n, off, tgt, noff, nbr);
-    symbolic(h->t, n, noff, nbr, rank);
-    ph.done("symbolic contraction");
-    map_arcs(h->t, off, tgt, off[n]);
-    ph.done("arc map");
+    ph.done("undirected graph");
+    if ((flags & NWCCH_TOPO_DEVICE_SYMBOLIC) && symbolic_device(h->t, n, noff, nbr, rank, device)) {
+      ph.done("symbolic contraction on the device");
+      map_arcs_device(h->t, off, tgt, off[n], device);
+      ph.done("arc map on the device");
+    } else {
+      symbolic(h->t, n, noff, nbr, rank);
+      ph.done("symbolic contraction");
+      map_arcs(h->t, off, tgt, off[n]);
+      ph.done("arc map");
+    }
     if (flags & NWCCH_TOPO_DEVICE_INDEX) {
       build_meta(h->t);
       std::vector<i32> cnt = count_triangles_device(h->t, device);

@@ -95,5 +95,11 @@ void build_side_index(Topology& t);
 // Device builders (device.hip): the lower-triangle count of every arc computed
 // on GPU `device` from the upward CSR alone.
 std::vector<i32> count_triangles_device(const Topology& t, int device);
+// (symbolic.hip) The chordal completion built on GPU `device` from the
+// elimination tree; false, with t untouched, when the tree is deeper than the
+// kernel's LDS bitmap covers (the caller then runs `symbolic`).
+bool symbolic_device(Topology& t, i64 n, const std::vector<i64>& noff, const std::vector<i32>& nbr,
+                     const i32* rank, int device);
+void map_arcs_device(Topology& t, const i64* off, const i32* tgt, i64 m, int device);
 
 }  // namespace nwcch

@@ -15,6 +15,7 @@ INF = 2147483647
 
 NWCCH_OK = 0
 NWCCH_TOPO_DEVICE_INDEX = 1
+NWCCH_TOPO_DEVICE_SYMBOLIC = 2
 STATUS_BITS = {1: "int32 overflow", 2: "INT32_MIN weight", 4: "query vertex out of range",
                8: "update frontier full or path longer than max_len",
                16: "path unpacking passed 4n + 16 vertices (negative cycle)"}
@@ -129,20 +130,28 @@ class Topology:
          self.n_loops, self.n_mapped) = (int(v) for v in info)
 
     @classmethod
-    def from_graph(cls, n, off, tgt, rank, device_index=None):
+    def from_graph(cls, n, off, tgt, rank, device_index=None, device_symbolic=None):
         """device_index = a GPU ordinal: that GPU builds the lower-triangle index
-        (NWCCH_TOPO_DEVICE_INDEX); the host then keeps no triangle list."""
+        (NWCCH_TOPO_DEVICE_INDEX); the host then keeps no triangle list.
+        device_symbolic = a GPU ordinal: that GPU builds the chordal completion
+        and the arc map (NWCCH_TOPO_DEVICE_SYMBOLIC)."""
         off = np.ascontiguousarray(off, np.int64)
         tgt = np.ascontiguousarray(tgt, np.int32)
         rank = np.ascontiguousarray(rank, np.int32)
         if off.shape != (n + 1,) or rank.shape != (n,):
             raise ValueError("off must be [n+1] and rank [n]")
+        if (device_index is not None and device_symbolic is not None
+                and int(device_index) != int(device_symbolic)):
+            raise ValueError("device_index and device_symbolic name different GPUs")
+        flags = ((0 if device_index is None else NWCCH_TOPO_DEVICE_INDEX)
+                 | (0 if device_symbolic is None else NWCCH_TOPO_DEVICE_SYMBOLIC))
+        dev = device_index if device_index is not None else device_symbolic
         h = C.c_void_p()
-        if device_index is None:
+        if flags == 0:
             check(lib().nwcch_topo_create(n, ptr(off), ptr(tgt), ptr(rank), C.byref(h)), "nwcch_topo_create")
         else:
-            check(lib().nwcch_topo_create_ex(n, ptr(off), ptr(tgt), ptr(rank), NWCCH_TOPO_DEVICE_INDEX,
-                                             int(device_index), C.byref(h)), "nwcch_topo_create_ex")
+            check(lib().nwcch_topo_create_ex(n, ptr(off), ptr(tgt), ptr(rank), flags, int(dev), C.byref(h)),
+                  "nwcch_topo_create_ex")
         t = cls(h)
         t.rank = rank.copy()
         t.device_index = device_index

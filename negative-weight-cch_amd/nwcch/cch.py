@@ -62,9 +62,11 @@ class CCH:
     def __init__(self, g, xy=None, leaf=64, method="flow", log=print, order=None, device=None,
                  gpu_index=False):
         """The reference's constructor (python/cch.py:112); device = the GPU
-        ordinal (default: the current one); gpu_index=True builds the triangle
-        index and the update side lists on that GPU (NWCCH_TOPO_DEVICE_INDEX)
-        instead of the host -- the same arrays, faster on big graphs."""
+        ordinal (default: the current one); gpu_index=True builds everything
+        after the order -- the chordal completion, the arc map, the triangle
+        index and the update side lists -- on that GPU
+        (NWCCH_TOPO_DEVICE_SYMBOLIC | NWCCH_TOPO_DEVICE_INDEX) instead of the
+        host: the same arrays, faster on big graphs."""
         t0 = time.perf_counter()
         n, off, tgt, _, gxy = _as_csr(g)
         self.n = n
@@ -89,7 +91,7 @@ class CCH:
         if gpu_index:
             import torch
             dev_i = torch.cuda.current_device() if device is None else int(device)
-            self.topo = _lib.Topology.from_graph(n, off, tgt, rank, device_index=dev_i)
+            self.topo = _lib.Topology.from_graph(n, off, tgt, rank, device_index=dev_i, device_symbolic=dev_i)
         else:
             self.topo = _lib.Topology.from_graph(n, off, tgt, rank)
         ex = self.topo.export()

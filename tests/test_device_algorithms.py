@@ -92,3 +92,74 @@ def test_pull_enumeration_equals_host_index():
             if k < len(row) and row[k] == z:
                 run.append((a, a + 1 + k))
         assert [tuple(r) for r in tab[toff[p]:toff[p + 1]]] == run, p
+
+
+def _undirected(g):
+    nb = [set() for _ in range(g.n)]
+    for u in range(g.n):
+        for v in g.tgt[g.off[u]:g.off[u + 1]]:
+            if v != u:
+                nb[u].add(int(v))
+                nb[int(v)].add(u)
+    return nb
+
+
+def test_symbolic_by_ancestor_bitmap_equals_host():
+    """The device symbolic contraction's construction, restated: the elimination
+    tree by Liu's union-find walk over the original graph, then by height,
+    leaves first, every row as a bitmap over ancestor distances (bit i = the
+    ancestor at distance i + 1) filled from the original upper neighbours and
+    the children's rows minus their first entry, read out in bit order.  It
+    equals the host contraction (first, head) on the reference's test family
+    and on an identity order (a deep tree)."""
+    cases = [(g, order_of(g)) for g in small_graphs()[::3]]
+    g = small_graphs()[1]
+    cases.append((g, np.arange(g.n, dtype=np.int32)))
+    for g, rank in cases:
+        n = g.n
+        inv = np.empty(n, np.int64)
+        inv[rank] = np.arange(n)
+        nb = _undirected(g)
+        parent = np.full(n, -1)
+        anc = np.full(n, -1)
+        for x in range(n):
+            for u in nb[inv[x]]:
+                r = rank[u]
+                if r >= x:
+                    continue
+                while anc[r] != -1 and anc[r] != x:
+                    anc[r], r = x, anc[r]
+                if anc[r] == -1:
+                    anc[r] = parent[r] = x
+        depth = np.zeros(n, np.int64)
+        height = np.zeros(n, np.int64)
+        for x in range(n - 1, -1, -1):
+            if parent[x] >= 0:
+                depth[x] = depth[parent[x]] + 1
+        for x in range(n):
+            if parent[x] >= 0:
+                height[parent[x]] = max(height[parent[x]], height[x] + 1)
+        kids = [[] for _ in range(n)]
+        for x in range(n):
+            if parent[x] >= 0:
+                kids[parent[x]].append(x)
+        rows = [None] * n
+        for x in sorted(range(n), key=lambda v: height[v]):
+            D = depth[x]
+            bits = np.zeros(D, bool)
+            ids = np.full(D, -1)
+            cand = [rank[u] for u in nb[inv[x]] if rank[u] > x]
+            for c in kids[x]:
+                assert rows[c][0] == x
+                cand += rows[c][1:]
+            for w in cand:
+                b = D - 1 - depth[w]
+                assert 0 <= b < D
+                assert ids[b] in (-1, w)
+                bits[b], ids[b] = True, w
+            rows[x] = [int(w) for w in ids[bits]]
+        t = _lib.Topology.from_graph(n, g.off, g.tgt, rank)
+        ex = t.export()
+        assert np.array_equal(ex["first"], np.concatenate([[0], np.cumsum([len(r) for r in rows])]))
+        assert np.array_equal(ex["head"], np.concatenate([np.asarray(r, np.int64) for r in rows] + [[]]))
+        assert np.array_equal(ex["parent"], parent)

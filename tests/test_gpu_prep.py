@@ -2,7 +2,8 @@
 triangle counts, the pull-form triangle lists (python/cch.py:168-186, by_c
 grouping cch.py:308) and the update side lists (python/cch.py:309-324) equal the
 host-built arrays entry for entry, so customization, updates and path
-unpacking give identical results either way."""
+unpacking give identical results either way.  The same holds for the chordal
+completion and the arc map built on the GPU (NWCCH_TOPO_DEVICE_SYMBOLIC)."""
 import numpy as np
 import pytest
 import torch
@@ -108,3 +109,55 @@ def test_device_index_from_csr(c0):
     head = np.array([1, 2], np.int32)
     with pytest.raises(_lib.NwcchError, match="chordal"):
         _lib.Topology.from_csr(3, first, head, device_index=0)
+
+
+def _check_symbolic(g, rank, what, index=False):
+    """NWCCH_TOPO_DEVICE_SYMBOLIC: the chordal completion and the arc map built
+    on the GPU equal the host's (python/cch.py:136-166, 188-204), array for
+    array, and so does everything derived from them."""
+    print("case: device symbolic", what, g.n, flush=True)
+    th = _lib.Topology.from_graph(g.n, g.off, g.tgt, rank)
+    td = _lib.Topology.from_graph(g.n, g.off, g.tgt, rank, device_symbolic=0,
+                                  device_index=0 if index else None)
+    assert (th.n, th.m, th.mp, th.triangles, th.depth, th.levels, th.n_loops, th.n_mapped) == \
+           (td.n, td.m, td.mp, td.triangles, td.depth, td.levels, td.n_loops, td.n_mapped)
+    eh, ed = th.export(), td.export()
+    for k in eh:
+        assert np.array_equal(eh[k], ed[k]), k
+    sh, oh, tab = th.export_triangles()
+    sd, od, tabd = td.export_triangles(with_list=not index)
+    assert np.array_equal(sh, sd) and np.array_equal(oh, od)
+    if not index:
+        assert np.array_equal(tab, tabd)
+    dh, dd = DeviceTopology(th), DeviceTopology(td)
+    assert torch.equal(dd.customize(g.w), dh.customize(g.w))
+    assert dd.status() == 0 and dh.status() == 0
+
+
+def test_device_symbolic_small_graphs():
+    for i, g in enumerate(small_graphs()):
+        _check_symbolic(g, order_of(g), f"small {i}")
+
+
+def test_device_symbolic_configs(c0):
+    g, rank = c0
+    _check_symbolic(g, rank, "config 0")
+    _check_symbolic(g, rank, "config 0 with the device index", index=True)
+    g = graphs.make_config(1)
+    _check_symbolic(g, _lib.order(g.n, g.off, g.tgt, g.x, g.y, 64), "config 1", index=True)
+
+
+def test_device_symbolic_loops_and_isolated_vertices():
+    """A self-loop maps to no shortcut (e_arc = -1); vertex 4 has no arcs."""
+    g = graphs.from_arcs(5, [(0, 1, 3), (1, 0, 3), (1, 2, -1), (3, 1, 4), (2, 2, 5)])
+    _check_symbolic(g, np.array([0, 4, 1, 2, 3], np.int32), "star with a loop")
+
+
+def test_device_symbolic_deep_trees():
+    """A path under the identity order is one chain: depth n - 1.  Depth
+    14,999 uses the largest LDS bitmap the kernel takes (15,000 levels of one
+    vertex); depth 15,999 is past it and is contracted on the host."""
+    for n in (15000, 16000):
+        u = np.arange(n - 1)
+        g = graphs.from_arcs(n, [(int(a), int(a) + 1, 1) for a in u] + [(int(a) + 1, int(a), 2) for a in u])
+        _check_symbolic(g, np.arange(n, dtype=np.int32), f"path of {n}")

@@ -110,10 +110,16 @@ def test_device_index_flags_are_checked():
     rc = _lib.lib().nwcch_topo_create_ex(g.n, _lib.ptr(g.off), _lib.ptr(g.tgt), _lib.ptr(rank), 6, 0,
                                          ctypes.byref(h))
     assert rc == -1 and b"unknown flags" in _lib.lib().nwcch_last_error()
+    ex = _lib.Topology.from_graph(g.n, g.off, g.tgt, rank).export()
+    rc = _lib.lib().nwcch_topo_from_csr_ex(g.n, _lib.ptr(ex["first"]), _lib.ptr(ex["head"]), 0, None, None,
+                                           _lib.NWCCH_TOPO_DEVICE_SYMBOLIC, 0, ctypes.byref(h))
+    assert rc == -1 and b"unknown flags" in _lib.lib().nwcch_last_error()   # a CSR needs no contraction
     import torch
     if not torch.cuda.is_available():
         with pytest.raises(_lib.NwcchError):
             _lib.Topology.from_graph(g.n, g.off, g.tgt, rank, device_index=0)
+        with pytest.raises(_lib.NwcchError):
+            _lib.Topology.from_graph(g.n, g.off, g.tgt, rank, device_symbolic=0)
 
 
 def test_non_chordal_csr_is_rejected():
