@@ -22,7 +22,10 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cstdint>
+#include <cstdio>
+#include <cstdlib>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -47,6 +50,23 @@ using namespace nwcch;
   } while (0)
 
 namespace {
+
+// With NWCCH_PROGRESS=1, a line on stderr per finished step.
+struct Steps {
+  bool on;
+  std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+  Steps() {
+    const char* e = std::getenv("NWCCH_PROGRESS");
+    on = e && e[0] == '1';
+  }
+  void done(const char* what) {
+    if (!on) return;
+    const auto t1 = std::chrono::steady_clock::now();
+    std::fprintf(stderr, "  device symbolic: %s %.2f s\n", what, std::chrono::duration<double>(t1 - t0).count());
+    std::fflush(stderr);
+    t0 = t1;
+  }
+};
 
 template <class T>
 struct DevBuf {
@@ -87,7 +107,8 @@ __device__ __forceinline__ i32 wave_incl_scan(i32 v, int lane) {
 // becomes -1, *flag is set and the host runs the pass again with a larger
 // buffer (readers skip such rows, so every access stays in bounds).
 __global__ void __launch_bounds__(64) k_sym_level(const i32* __restrict__ vtx, i32 nv, i32 n, i32 maxd,
-                                                  const i64* __restrict__ uo_off, const i32* __restrict__ uo,
+                                                  const i32* __restrict__ inv, const i64* __restrict__ noff,
+                                                  const i32* __restrict__ nbr, const i32* __restrict__ rank,
                                                   const i64* __restrict__ kid_off, const i32* __restrict__ kid,
                                                   const i32* __restrict__ vdepth, i32* len, i64* loc, i32* buf,
                                                   i64 cap, unsigned long long* cursor, uint32_t* flag) {
@@ -109,7 +130,11 @@ __global__ void __launch_bounds__(64) k_sym_level(const i32* __restrict__ vtx, i
       atomicOr(&bits[b >> 5], 1u << (b & 31));
       ids[b] = w;
     };
-    for (i64 j = uo_off[x] + lane; j < uo_off[x + 1]; j += 64) mark(uo[j]);
+    const i32 v = inv[x];  // original neighbours of higher rank
+    for (i64 j = noff[v] + lane; j < noff[v + 1]; j += 64) {
+      const i32 r = rank[nbr[j]];
+      if (r > x) mark(r);
+    }
     for (i64 k = kid_off[x]; k < kid_off[x + 1]; k++) {
       const i32 c = kid[k];
       const i64 lc = loc[c];
@@ -227,7 +252,9 @@ bool symbolic_device(Topology& t, i64 n, const std::vector<i64>& noff, const std
     if (r < 0 || r >= n || inv[r] >= 0) throw std::runtime_error("order must be a permutation");
     inv[r] = (i32)v;
   }
+  Steps steps;
   std::vector<i32> parent = elimination_tree(n, noff, nbr, rank, inv);
+  steps.done("elimination tree (host)");
   std::vector<i32> depth(n, 0), height(n, 0);
   i32 maxd = 0, maxh = 0;
   for (i64 x = n - 1; x >= 0; x--)
@@ -241,43 +268,36 @@ bool symbolic_device(Topology& t, i64 n, const std::vector<i64>& noff, const std
     if (p >= 0 && height[p] < height[x] + 1) height[p] = height[x] + 1;
     maxh = std::max(maxh, height[x]);
   }
-  // original upward neighbours in rank space, children lists, vertices by height
-  std::vector<i64> uo_off(n + 1, 0), kid_off(n + 1, 0), lv_off((size_t)maxh + 2, 0);
+  // children lists and vertices by height (the original neighbours are read
+  // on the device through inv and rank)
+  std::vector<i64> kid_off(n + 1, 0), lv_off((size_t)maxh + 2, 0);
   for (i64 x = 0; x < n; x++) {
-    const i32 v = inv[x];
-    i64 c = 0;
-    for (i64 j = noff[v]; j < noff[v + 1]; j++) c += rank[nbr[j]] > x;
-    uo_off[x + 1] = uo_off[x] + c;
     if (parent[x] >= 0) kid_off[parent[x] + 1]++;
     lv_off[height[x] + 1]++;
   }
   for (i64 x = 0; x < n; x++) kid_off[x + 1] += kid_off[x];
   for (i32 h = 0; h <= maxh; h++) lv_off[h + 1] += lv_off[h];
-  std::vector<i32> uo(uo_off[n]), kid(std::max<i64>(kid_off[n], 1)), lv_vtx(n);
+  std::vector<i32> kid(std::max<i64>(kid_off[n], 1)), lv_vtx(n);
   {
     std::vector<i64> kp(kid_off.begin(), kid_off.end() - 1), lp(lv_off.begin(), lv_off.end() - 1);
     for (i64 x = 0; x < n; x++) {
-      const i32 v = inv[x];
-      i64 o = uo_off[x];
-      for (i64 j = noff[v]; j < noff[v + 1]; j++) {
-        const i32 r = rank[nbr[j]];
-        if (r > x) uo[o++] = r;
-      }
       if (parent[x] >= 0) kid[kp[parent[x]]++] = (i32)x;
       lv_vtx[lp[height[x]]++] = (i32)x;
     }
   }
 
+  steps.done("depths, heights, child and level lists (host)");
   HIP_OK(hipSetDevice(device));
-  DevBuf<i64> d_uo_off(uo_off.data(), uo_off.size()), d_kid_off(kid_off.data(), kid_off.size());
-  DevBuf<i32> d_uo(uo.data(), uo.size()), d_kid(kid.data(), kid.size()), d_depth(depth.data(), depth.size());
+  DevBuf<i64> d_noff(noff.data(), noff.size()), d_kid_off(kid_off.data(), kid_off.size());
+  DevBuf<i32> d_nbr(nbr.data(), nbr.size()), d_inv(inv.data(), inv.size()), d_rank(rank, (size_t)n);
+  DevBuf<i32> d_kid(kid.data(), kid.size()), d_depth(depth.data(), depth.size());
   DevBuf<i32> d_vtx(lv_vtx.data(), lv_vtx.size()), d_len(n);
   DevBuf<i64> d_loc(n);
   DevBuf<unsigned long long> d_cursor(1);
   DevBuf<uint32_t> d_flag(1);
   const size_t lds = sizeof(uint32_t) * ((size_t)(maxd + 31) / 32 + (size_t)maxd);
   // staging buffer: a first guess, doubled until every row fits
-  i64 cap = std::max<i64>(4 * uo_off[n] + n, 1 << 16);
+  i64 cap = std::max<i64>(2 * noff[n] + n, 1 << 16);
   unsigned long long total = 0;
   std::vector<i32> len(n);
   for (;;) {
@@ -289,8 +309,8 @@ bool symbolic_device(Topology& t, i64 n, const std::vector<i64>& noff, const std
       if (nv == 0) continue;
       const unsigned grid = (unsigned)std::min<i64>(nv, 8192);
       hipLaunchKernelGGL(k_sym_level, dim3(grid), dim3(64), lds, 0, d_vtx.p + v0, (i32)nv, (i32)n, maxd,
-                         d_uo_off.p, d_uo.p, d_kid_off.p, d_kid.p, d_depth.p, d_len.p, d_loc.p, d_buf.p, cap,
-                         d_cursor.p, d_flag.p);
+                         d_inv.p, d_noff.p, d_nbr.p, d_rank.p, d_kid_off.p, d_kid.p, d_depth.p, d_len.p, d_loc.p,
+                         d_buf.p, cap, d_cursor.p, d_flag.p);
     }
     HIP_OK(hipGetLastError());
     uint32_t flag = 0;
@@ -300,6 +320,7 @@ bool symbolic_device(Topology& t, i64 n, const std::vector<i64>& noff, const std
       cap = std::max<i64>(2 * cap, (i64)total);
       continue;
     }
+    steps.done("rows by height (device)");
     if ((i64)total >= ((i64)1 << 31) - 1) throw std::runtime_error("too many shortcut arcs for int32 ids");
     HIP_OK(hipMemcpy(len.data(), d_len.p, (size_t)n * sizeof(i32), hipMemcpyDeviceToHost));
     t.first.assign(n + 1, 0);
@@ -315,6 +336,7 @@ bool symbolic_device(Topology& t, i64 n, const std::vector<i64>& noff, const std
     }
     t.head.resize(t.mp);
     if (t.mp) HIP_OK(hipMemcpy(t.head.data(), d_head.p, (size_t)t.mp * sizeof(i32), hipMemcpyDeviceToHost));
+    steps.done("CSR order and copy back");
     break;
   }
   t.n = n;
@@ -325,6 +347,7 @@ bool symbolic_device(Topology& t, i64 n, const std::vector<i64>& noff, const std
     if (p != parent[x]) throw std::runtime_error("device symbolic contraction: rows disagree with the elimination tree");
   }
   t.parent = std::move(parent);
+  steps.done("check against the tree");
   return true;
 }
 
