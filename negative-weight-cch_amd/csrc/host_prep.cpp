@@ -48,16 +48,31 @@ void undirected_csr(i64 n, const i64* off, const i32* tgt, std::vector<i64>& nof
       i64 v = tgt[k];
       if (v != u) { tmp[pos[u]++] = (i32)v; tmp[pos[v]++] = (i32)u; }
     }
+  // each list sorted and deduplicated in place (vertex ranges on threads), then
+  // the lists packed in vertex order
   noff.assign(n + 1, 0);
-  nbr.clear();
-  nbr.reserve(deg[n]);
-  for (i64 u = 0; u < n; u++) {
-    std::sort(tmp.begin() + deg[u], tmp.begin() + deg[u + 1]);
-    i32 last = -1;
-    for (i64 k = deg[u]; k < deg[u + 1]; k++)
-      if (tmp[k] != last) { nbr.push_back(tmp[k]); last = tmp[k]; }
-    noff[u + 1] = (i64)nbr.size();
-  }
+  std::vector<i64> kept(n);
+  int nth = (int)std::thread::hardware_concurrency();
+  if (const char* e = std::getenv("NWCCH_THREADS")) nth = std::atoi(e);
+  nth = (int)std::max<i64>(1, std::min<i64>({(i64)nth, 16, n / 65536 + 1}));
+  auto each = [&](auto&& fn) {
+    std::vector<std::thread> th;
+    for (int w = 1; w < nth; w++) th.emplace_back(fn, n * w / nth, n * (w + 1) / nth);
+    fn((i64)0, n / nth);
+    for (auto& x : th) x.join();
+  };
+  each([&](i64 u0, i64 u1) {
+    for (i64 u = u0; u < u1; u++) {
+      auto b = tmp.begin() + deg[u], e = tmp.begin() + deg[u + 1];
+      std::sort(b, e);
+      kept[u] = std::unique(b, e) - b;
+    }
+  });
+  for (i64 u = 0; u < n; u++) noff[u + 1] = noff[u] + kept[u];
+  nbr.assign(noff[n], 0);
+  each([&](i64 u0, i64 u1) {
+    for (i64 u = u0; u < u1; u++) std::copy_n(tmp.begin() + deg[u], kept[u], nbr.begin() + noff[u]);
+  });
 }
 
 // ------------------------------------------------- inertial-flow bisection
