@@ -28,6 +28,7 @@
 #include <cstdlib>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "nwcch_topo.h"
@@ -108,7 +109,7 @@ __device__ __forceinline__ i32 wave_incl_scan(i32 v, int lane) {
 // buffer (readers skip such rows, so every access stays in bounds).
 __global__ void __launch_bounds__(64) k_sym_level(const i32* __restrict__ vtx, i32 nv, i32 n, i32 maxd,
                                                   const i32* __restrict__ inv, const i64* __restrict__ noff,
-                                                  const i32* __restrict__ nbr, const i32* __restrict__ rank,
+                                                  const i32* __restrict__ rnbr,
                                                   const i64* __restrict__ kid_off, const i32* __restrict__ kid,
                                                   const i32* __restrict__ vdepth, i32* len, i64* loc, i32* buf,
                                                   i64 cap, unsigned long long* cursor, uint32_t* flag) {
@@ -132,7 +133,7 @@ __global__ void __launch_bounds__(64) k_sym_level(const i32* __restrict__ vtx, i
     };
     const i32 v = inv[x];  // original neighbours of higher rank
     for (i64 j = noff[v] + lane; j < noff[v + 1]; j += 64) {
-      const i32 r = rank[nbr[j]];
+      const i32 r = rnbr[j];
       if (r > x) mark(r);
     }
     for (i64 k = kid_off[x]; k < kid_off[x + 1]; k++) {
@@ -218,16 +219,16 @@ __global__ void __launch_bounds__(256) k_map_arcs(i32 n, const i64* __restrict__
 
 namespace nwcch {
 
-// Elimination tree of the chordal completion under `rank`, from the original
-// graph (Liu's algorithm): parent[x] = the smallest higher rank adjacent to x
-// in the completion, -1 for roots.
-static std::vector<i32> elimination_tree(i64 n, const std::vector<i64>& noff, const std::vector<i32>& nbr,
-                                         const i32* rank, const std::vector<i32>& inv) {
+// Elimination tree of the chordal completion, from the original graph with
+// its neighbour lists in rank space (Liu's algorithm): parent[x] = the
+// smallest higher rank adjacent to x in the completion, -1 for roots.
+static std::vector<i32> elimination_tree(i64 n, const std::vector<i64>& noff, const std::vector<i32>& rnbr,
+                                         const std::vector<i32>& inv) {
   std::vector<i32> parent(n, -1), anc(n, -1);
   for (i64 x = 0; x < n; x++) {
     const i32 v = inv[x];
     for (i64 j = noff[v]; j < noff[v + 1]; j++) {
-      i32 r = rank[nbr[j]];
+      i32 r = rnbr[j];
       if (r >= x) continue;
       while (anc[r] != -1 && anc[r] != x) {
         const i32 next = anc[r];
@@ -253,7 +254,22 @@ bool symbolic_device(Topology& t, i64 n, const std::vector<i64>& noff, const std
     inv[r] = (i32)v;
   }
   Steps steps;
-  std::vector<i32> parent = elimination_tree(n, noff, nbr, rank, inv);
+  // the neighbour lists with ranks in place of vertex ids (vertex ranges on threads)
+  std::vector<i32> rnbr(nbr.size());
+  {
+    int nth = (int)std::thread::hardware_concurrency();
+    if (const char* e = std::getenv("NWCCH_THREADS")) nth = std::atoi(e);
+    nth = (int)std::max<i64>(1, std::min<i64>({(i64)nth, 16, n / 65536 + 1}));
+    auto relabel = [&](i64 v0, i64 v1) {
+      for (i64 j = noff[v0]; j < noff[v1]; j++) rnbr[j] = rank[nbr[j]];
+    };
+    std::vector<std::thread> th;
+    for (int w = 1; w < nth; w++) th.emplace_back(relabel, n * w / nth, n * (w + 1) / nth);
+    relabel(0, n / nth);
+    for (auto& x : th) x.join();
+  }
+  steps.done("neighbour lists in rank space (host)");
+  std::vector<i32> parent = elimination_tree(n, noff, rnbr, inv);
   steps.done("elimination tree (host)");
   std::vector<i32> depth(n, 0), height(n, 0);
   i32 maxd = 0, maxh = 0;
@@ -289,7 +305,7 @@ bool symbolic_device(Topology& t, i64 n, const std::vector<i64>& noff, const std
   steps.done("depths, heights, child and level lists (host)");
   HIP_OK(hipSetDevice(device));
   DevBuf<i64> d_noff(noff.data(), noff.size()), d_kid_off(kid_off.data(), kid_off.size());
-  DevBuf<i32> d_nbr(nbr.data(), nbr.size()), d_inv(inv.data(), inv.size()), d_rank(rank, (size_t)n);
+  DevBuf<i32> d_rnbr(rnbr.data(), rnbr.size()), d_inv(inv.data(), inv.size());
   DevBuf<i32> d_kid(kid.data(), kid.size()), d_depth(depth.data(), depth.size());
   DevBuf<i32> d_vtx(lv_vtx.data(), lv_vtx.size()), d_len(n);
   DevBuf<i64> d_loc(n);
@@ -309,7 +325,7 @@ bool symbolic_device(Topology& t, i64 n, const std::vector<i64>& noff, const std
       if (nv == 0) continue;
       const unsigned grid = (unsigned)std::min<i64>(nv, 8192);
       hipLaunchKernelGGL(k_sym_level, dim3(grid), dim3(64), lds, 0, d_vtx.p + v0, (i32)nv, (i32)n, maxd,
-                         d_inv.p, d_noff.p, d_nbr.p, d_rank.p, d_kid_off.p, d_kid.p, d_depth.p, d_len.p, d_loc.p,
+                         d_inv.p, d_noff.p, d_rnbr.p, d_kid_off.p, d_kid.p, d_depth.p, d_len.p, d_loc.p,
                          d_buf.p, cap, d_cursor.p, d_flag.p);
     }
     HIP_OK(hipGetLastError());
