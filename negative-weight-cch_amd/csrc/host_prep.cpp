@@ -679,15 +679,43 @@ void build_schedule(Topology& t, const std::vector<i32>& cnt) {
   for (i32 c : cnt) { tot += c; maxc = std::max<i64>(maxc, c); }
   if (tot != t.triangles) throw std::runtime_error("upward CSR is not chordal");
   // schedule: arcs grouped by the ET height of their tail, within a level by
-  // lower-triangle count, descending (big arcs first), ties by arc id -- two
-  // stable counting sorts, by count descending and then by level
+  // lower-triangle count, descending (big arcs first), ties by arc id -- a
+  // stable counting sort of the arc ids by level, then one by count descending
+  // inside every level (levels on threads)
   std::vector<i32> ids(mp);
   std::iota(ids.begin(), ids.end(), 0);
   std::vector<i64> key(mp);
-  for (i64 a = 0; a < mp; a++) key[a] = maxc - cnt[a];
-  ids = counting_sort(ids, key, maxc + 1);
-  for (i64 a = 0; a < mp; a++) key[a] = t.vlevel[t.arc_tail[a]];
+  std::vector<i64> lstart((size_t)t.levels + 1, 0);
+  for (i64 a = 0; a < mp; a++) {
+    key[a] = t.vlevel[t.arc_tail[a]];
+    lstart[key[a] + 1]++;
+  }
+  for (i32 L = 0; L < t.levels; L++) lstart[L + 1] += lstart[L];
   t.sched_arc = counting_sort(ids, key, t.levels);
+  {
+    int nth = (int)std::thread::hardware_concurrency();
+    if (const char* e = std::getenv("NWCCH_THREADS")) nth = std::atoi(e);
+    nth = (int)std::max<i64>(1, std::min<i64>({(i64)nth, 16, mp / 65536 + 1}));
+    std::atomic<i32> next{0};
+    auto work = [&] {
+      std::vector<i64> start(maxc + 2);
+      std::vector<i32> out;
+      for (i32 L; (L = next.fetch_add(1)) < t.levels;) {
+        i32* seg = t.sched_arc.data() + lstart[L];
+        const i64 len = lstart[L + 1] - lstart[L];
+        std::fill(start.begin(), start.end(), 0);
+        for (i64 i = 0; i < len; i++) start[maxc - cnt[seg[i]] + 1]++;
+        for (i64 k = 0; k <= maxc; k++) start[k + 1] += start[k];
+        out.resize(len);
+        for (i64 i = 0; i < len; i++) out[start[maxc - cnt[seg[i]]]++] = seg[i];
+        std::copy(out.begin(), out.end(), seg);
+      }
+    };
+    std::vector<std::thread> th;
+    for (int w = 1; w < nth; w++) th.emplace_back(work);
+    work();
+    for (auto& x : th) x.join();
+  }
   t.arc_pos.resize(mp);
   for (i64 p = 0; p < mp; p++) t.arc_pos[t.sched_arc[p]] = (i32)p;
   t.tri_off.assign(mp + 1, 0);
