@@ -29,6 +29,23 @@
 
 namespace nwcch {
 
+namespace {
+
+// fn(lo, hi) over contiguous slices of [0, n) on up to 16 threads (NWCCH_THREADS
+// caps it; small inputs stay on the calling thread).  fn must not throw.
+template <class F>
+void parallel_ranges(i64 n, F&& fn) {
+  int nth = (int)std::thread::hardware_concurrency();
+  if (const char* e = std::getenv("NWCCH_THREADS")) nth = std::atoi(e);
+  nth = (int)std::max<i64>(1, std::min<i64>({(i64)nth, 16, n / 65536 + 1}));
+  std::vector<std::thread> th;
+  for (int w = 1; w < nth; w++) th.emplace_back(fn, n * w / nth, n * (w + 1) / nth);
+  fn((i64)0, n / nth);
+  for (auto& x : th) x.join();
+}
+
+}  // namespace
+
 // ------------------------------------------------------------ undirected CSR
 
 void undirected_csr(i64 n, const i64* off, const i32* tgt, std::vector<i64>& noff,
@@ -52,16 +69,7 @@ void undirected_csr(i64 n, const i64* off, const i32* tgt, std::vector<i64>& nof
   // the lists packed in vertex order
   noff.assign(n + 1, 0);
   std::vector<i64> kept(n);
-  int nth = (int)std::thread::hardware_concurrency();
-  if (const char* e = std::getenv("NWCCH_THREADS")) nth = std::atoi(e);
-  nth = (int)std::max<i64>(1, std::min<i64>({(i64)nth, 16, n / 65536 + 1}));
-  auto each = [&](auto&& fn) {
-    std::vector<std::thread> th;
-    for (int w = 1; w < nth; w++) th.emplace_back(fn, n * w / nth, n * (w + 1) / nth);
-    fn((i64)0, n / nth);
-    for (auto& x : th) x.join();
-  };
-  each([&](i64 u0, i64 u1) {
+  parallel_ranges(n, [&](i64 u0, i64 u1) {
     for (i64 u = u0; u < u1; u++) {
       auto b = tmp.begin() + deg[u], e = tmp.begin() + deg[u + 1];
       std::sort(b, e);
@@ -70,7 +78,7 @@ void undirected_csr(i64 n, const i64* off, const i32* tgt, std::vector<i64>& nof
   });
   for (i64 u = 0; u < n; u++) noff[u + 1] = noff[u] + kept[u];
   nbr.assign(noff[n], 0);
-  each([&](i64 u0, i64 u1) {
+  parallel_ranges(n, [&](i64 u0, i64 u1) {
     for (i64 u = u0; u < u1; u++) std::copy_n(tmp.begin() + deg[u], kept[u], nbr.begin() + noff[u]);
   });
 }
@@ -573,17 +581,24 @@ void map_arcs(Topology& t, const i64* off, const i32* tgt, i64 m) {
   t.e_arc.assign(m, -1);
   t.e_up.assign(m, 0);
   t.loop_edges.clear();
-  for (i64 u = 0; u < t.n; u++) {
-    const i64 ru = t.rank[u];
-    for (i64 k = off[u]; k < off[u + 1]; k++) {
-      const i64 rv = t.rank[tgt[k]];
-      if (ru == rv) { t.loop_edges.push_back((i32)k); continue; }
-      i64 a = ru < rv ? find_arc(t, ru, rv) : find_arc(t, rv, ru);
-      if (a < 0) throw std::runtime_error("original arc missing from the contraction");
-      t.e_arc[k] = a;
-      t.e_up[k] = ru < rv ? 1 : 0;
+  std::atomic<bool> missing{false};
+  parallel_ranges(t.n, [&](i64 u0, i64 u1) {  // each arc written once; loops stay -1
+    for (i64 u = u0; u < u1; u++) {
+      const i64 ru = t.rank[u];
+      for (i64 k = off[u]; k < off[u + 1]; k++) {
+        const i64 rv = t.rank[tgt[k]];
+        if (ru == rv) continue;
+        const i64 a = ru < rv ? find_arc(t, ru, rv) : find_arc(t, rv, ru);
+        if (a < 0) missing.store(true, std::memory_order_relaxed);
+        t.e_arc[k] = a;
+        t.e_up[k] = ru < rv ? 1 : 0;
+      }
     }
-  }
+  });
+  if (missing.load()) throw std::runtime_error("original arc missing from the contraction");
+  for (i64 u = 0; u < t.n; u++)
+    for (i64 k = off[u]; k < off[u + 1]; k++)
+      if (t.rank[u] == t.rank[tgt[k]]) t.loop_edges.push_back((i32)k);
 }
 
 // ---------------------------------------------- triangle index + schedule
@@ -643,22 +658,31 @@ void build_meta(Topology& t) {
 
 // Lower-triangle count of every arc (pull form: triangles whose top arc is c),
 // by a merge pointer along row head[i] for each arc i of row x.
+// Rows on threads; an arc gains triangles from several rows, so the counts are
+// added atomically (the sums do not depend on the order).
 std::vector<i32> count_triangles_host(const Topology& t) {
   std::vector<i32> cnt(t.mp, 0);
-  for (i64 x = 0; x < t.n; x++) {
-    const i64 s = t.first[x], e = t.first[x + 1];
-    for (i64 i = s; i < e; i++) {
-      const i64 y = t.head[i];
-      i64 j = t.first[y];
-      const i64 je = t.first[y + 1];
-      for (i64 k = i + 1; k < e; k++) {
-        const i32 z = t.head[k];
-        while (j < je && t.head[j] < z) j++;
-        if (j >= je || t.head[j] != z) throw std::runtime_error("upward CSR is not chordal");
-        cnt[j]++;
+  std::atomic<bool> bad{false};
+  parallel_ranges(t.n, [&](i64 x0, i64 x1) {
+    for (i64 x = x0; x < x1 && !bad.load(std::memory_order_relaxed); x++) {
+      const i64 s = t.first[x], e = t.first[x + 1];
+      for (i64 i = s; i < e; i++) {
+        const i64 y = t.head[i];
+        i64 j = t.first[y];
+        const i64 je = t.first[y + 1];
+        for (i64 k = i + 1; k < e; k++) {
+          const i32 z = t.head[k];
+          while (j < je && t.head[j] < z) j++;
+          if (j >= je || t.head[j] != z) {
+            bad.store(true, std::memory_order_relaxed);
+            break;
+          }
+          __atomic_fetch_add(&cnt[j], 1, __ATOMIC_RELAXED);
+        }
       }
     }
-  }
+  });
+  if (bad.load()) throw std::runtime_error("upward CSR is not chordal");
   return cnt;
 }
 
