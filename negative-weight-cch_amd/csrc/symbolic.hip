@@ -28,10 +28,9 @@
 #include <cstdlib>
 #include <stdexcept>
 #include <string>
-#include <thread>
 #include <vector>
 
-#include "nwcch_topo.h"
+#include "nwcch_dev.h"
 
 namespace nwcch {
 
@@ -67,26 +66,6 @@ struct Steps {
     std::fflush(stderr);
     t0 = t1;
   }
-};
-
-template <class T>
-struct DevBuf {
-  T* p = nullptr;
-  explicit DevBuf(size_t count) {
-    if (hipMalloc((void**)&p, std::max<size_t>(count, 1) * sizeof(T)) != hipSuccess) {
-      p = nullptr;
-      throw std::runtime_error("hipMalloc of a device symbolic-contraction buffer failed");
-    }
-  }
-  DevBuf(const T* host, size_t count) : DevBuf(count) {
-    if (count && hipMemcpy(p, host, count * sizeof(T), hipMemcpyHostToDevice) != hipSuccess)
-      throw std::runtime_error("upload of a device symbolic-contraction buffer failed");
-  }
-  ~DevBuf() {
-    if (p) (void)hipFree(p);
-  }
-  DevBuf(const DevBuf&) = delete;
-  DevBuf& operator=(const DevBuf&) = delete;
 };
 
 __device__ __forceinline__ i32 wave_sum(i32 v) {
@@ -256,18 +235,9 @@ bool symbolic_device(Topology& t, i64 n, const std::vector<i64>& noff, const std
   Steps steps;
   // the neighbour lists with ranks in place of vertex ids (vertex ranges on threads)
   std::vector<i32> rnbr(nbr.size());
-  {
-    int nth = (int)std::thread::hardware_concurrency();
-    if (const char* e = std::getenv("NWCCH_THREADS")) nth = std::atoi(e);
-    nth = (int)std::max<i64>(1, std::min<i64>({(i64)nth, 16, n / 65536 + 1}));
-    auto relabel = [&](i64 v0, i64 v1) {
-      for (i64 j = noff[v0]; j < noff[v1]; j++) rnbr[j] = rank[nbr[j]];
-    };
-    std::vector<std::thread> th;
-    for (int w = 1; w < nth; w++) th.emplace_back(relabel, n * w / nth, n * (w + 1) / nth);
-    relabel(0, n / nth);
-    for (auto& x : th) x.join();
-  }
+  parallel_ranges(n, [&](i64 v0, i64 v1) {
+    for (i64 j = noff[v0]; j < noff[v1]; j++) rnbr[j] = rank[nbr[j]];
+  });
   steps.done("neighbour lists in rank space (host)");
   std::vector<i32> parent = elimination_tree(n, noff, rnbr, inv);
   steps.done("elimination tree (host)");
@@ -304,20 +274,20 @@ bool symbolic_device(Topology& t, i64 n, const std::vector<i64>& noff, const std
 
   steps.done("depths, heights, child and level lists (host)");
   HIP_OK(hipSetDevice(device));
-  DevBuf<i64> d_noff(noff.data(), noff.size()), d_kid_off(kid_off.data(), kid_off.size());
-  DevBuf<i32> d_rnbr(rnbr.data(), rnbr.size()), d_inv(inv.data(), inv.size());
-  DevBuf<i32> d_kid(kid.data(), kid.size()), d_depth(depth.data(), depth.size());
-  DevBuf<i32> d_vtx(lv_vtx.data(), lv_vtx.size()), d_len(n);
-  DevBuf<i64> d_loc(n);
-  DevBuf<unsigned long long> d_cursor(1);
-  DevBuf<uint32_t> d_flag(1);
+  TempBuf<i64> d_noff(noff.data(), noff.size()), d_kid_off(kid_off.data(), kid_off.size());
+  TempBuf<i32> d_rnbr(rnbr.data(), rnbr.size()), d_inv(inv.data(), inv.size());
+  TempBuf<i32> d_kid(kid.data(), kid.size()), d_depth(depth.data(), depth.size());
+  TempBuf<i32> d_vtx(lv_vtx.data(), lv_vtx.size()), d_len(n);
+  TempBuf<i64> d_loc(n);
+  TempBuf<unsigned long long> d_cursor(1);
+  TempBuf<uint32_t> d_flag(1);
   const size_t lds = sizeof(uint32_t) * ((size_t)(maxd + 31) / 32 + (size_t)maxd);
   // staging buffer: a first guess, doubled until every row fits
   i64 cap = std::max<i64>(2 * noff[n] + n, 1 << 16);
   unsigned long long total = 0;
   std::vector<i32> len(n);
   for (;;) {
-    DevBuf<i32> d_buf((size_t)cap);
+    TempBuf<i32> d_buf((size_t)cap);
     HIP_OK(hipMemset(d_cursor.p, 0, sizeof(unsigned long long)));
     HIP_OK(hipMemset(d_flag.p, 0, sizeof(uint32_t)));
     for (i32 h = 0; h <= maxh; h++) {
@@ -343,8 +313,8 @@ bool symbolic_device(Topology& t, i64 n, const std::vector<i64>& noff, const std
     for (i64 x = 0; x < n; x++) t.first[x + 1] = t.first[x] + len[x];
     if (t.first[n] != (i64)total) throw std::runtime_error("device symbolic contraction: row sizes disagree");
     t.mp = t.first[n];
-    DevBuf<i64> d_first(t.first.data(), t.first.size());
-    DevBuf<i32> d_head((size_t)t.mp);
+    TempBuf<i64> d_first(t.first.data(), t.first.size());
+    TempBuf<i32> d_head((size_t)t.mp);
     if (n > 0) {
       const unsigned g = (unsigned)std::min<i64>((n + 255) / 256, 16384);
       hipLaunchKernelGGL(k_sym_scatter, dim3(g), dim3(256), 0, 0, (i32)n, d_first.p, d_loc.p, d_buf.p, d_head.p);
@@ -375,10 +345,10 @@ void map_arcs_device(Topology& t, const i64* off, const i32* tgt, i64 m, int dev
   t.e_up.assign(m, 0);
   t.loop_edges.clear();
   if (m > 0) {
-    DevBuf<i64> d_off(off, (size_t)n + 1), d_first(t.first.data(), t.first.size()), d_arc((size_t)m);
-    DevBuf<i32> d_tgt(tgt, (size_t)m), d_rank(t.rank.data(), t.rank.size()), d_head(t.head.data(), t.head.size());
-    DevBuf<int8_t> d_up((size_t)m);
-    DevBuf<uint32_t> d_bad(1);
+    TempBuf<i64> d_off(off, (size_t)n + 1), d_first(t.first.data(), t.first.size()), d_arc((size_t)m);
+    TempBuf<i32> d_tgt(tgt, (size_t)m), d_rank(t.rank.data(), t.rank.size()), d_head(t.head.data(), t.head.size());
+    TempBuf<int8_t> d_up((size_t)m);
+    TempBuf<uint32_t> d_bad(1);
     HIP_OK(hipMemset(d_bad.p, 0, sizeof(uint32_t)));
     const unsigned g = (unsigned)std::min<i64>(std::max<i64>((n + 255) / 256, 1), 16384);
     hipLaunchKernelGGL(k_map_arcs, dim3(g), dim3(256), 0, 0, (i32)n, d_off.p, d_tgt.p, d_rank.p, d_first.p,
