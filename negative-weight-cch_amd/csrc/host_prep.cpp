@@ -747,24 +747,50 @@ void build_schedule(Topology& t, const std::vector<i32>& cnt) {
 }
 
 // (a, b) pairs scattered into schedule order, x ascending within an arc.
+// Arc (y, z) collects its triangles from the rows x < y in ascending x.  The
+// tails y are cut into slices of about equal triangle counts, and the slices
+// go to threads: a thread walks the rows x in order but takes only the pairs
+// whose first arc (x, y) ends in its slice, so every arc is filled by one
+// thread, in ascending x, exactly as one sequential pass would fill it.
 void fill_triangles_host(Topology& t) {
+  const i64 n = t.n, mp = t.mp;
   t.tri_ab.assign(2 * t.triangles, 0);
-  std::vector<i64> fill(t.mp);
-  for (i64 a = 0; a < t.mp; a++) fill[a] = t.tri_off[t.arc_pos[a]];
-  for (i64 x = 0; x < t.n; x++) {
-    const i64 s = t.first[x], e = t.first[x + 1];
-    for (i64 i = s; i < e; i++) {
-      const i64 y = t.head[i];
-      i64 j = t.first[y];
-      for (i64 k = i + 1; k < e; k++) {
-        const i32 z = t.head[k];
-        while (t.head[j] < z) j++;
-        const i64 q = fill[j]++;
-        t.tri_ab[2 * q] = (i32)i;
-        t.tri_ab[2 * q + 1] = (i32)k;
+  std::vector<i64> fill(mp);
+  parallel_ranges(mp, [&](i64 a0, i64 a1) {
+    for (i64 a = a0; a < a1; a++) fill[a] = t.tri_off[t.arc_pos[a]];
+  });
+  std::vector<i64> below(n + 1, 0);  // triangles of the arcs whose tail is < y
+  for (i64 y = 0; y < n; y++) {
+    i64 c = 0;
+    for (i64 a = t.first[y]; a < t.first[y + 1]; a++) c += t.tri_off[t.arc_pos[a] + 1] - t.tri_off[t.arc_pos[a]];
+    below[y + 1] = below[y] + c;
+  }
+  constexpr i64 kSlices = 256;
+  std::vector<i64> ycut(kSlices + 1, n);
+  for (i64 s = 0; s < kSlices; s++)
+    ycut[s] = std::lower_bound(below.begin(), below.end(), t.triangles * s / kSlices) - below.begin();
+  ycut[0] = 0;
+  // a thread takes the slices [s0, s1), i.e. the tails [ycut[s0], ycut[s1])
+  parallel_ranges(kSlices, [&](i64 s0, i64 s1) {
+    const i64 ylo = ycut[s0], yhi = ycut[s1];
+    if (ylo >= yhi) return;
+    for (i64 x = 0; x < yhi; x++) {
+      const i64 s = t.first[x], e = t.first[x + 1];
+      const i64 i0 = std::lower_bound(t.head.begin() + s, t.head.begin() + e, (i32)ylo) - t.head.begin();
+      const i64 i1 = std::lower_bound(t.head.begin() + i0, t.head.begin() + e, (i32)yhi) - t.head.begin();
+      for (i64 i = i0; i < i1; i++) {
+        const i64 y = t.head[i];
+        i64 j = t.first[y];
+        for (i64 k = i + 1; k < e; k++) {
+          const i32 z = t.head[k];
+          while (t.head[j] < z) j++;
+          const i64 q = fill[j]++;
+          t.tri_ab[2 * q] = (i32)i;
+          t.tri_ab[2 * q + 1] = (i32)k;
+        }
       }
     }
-  }
+  }, 1);
   t.has_tri = true;
 }
 

@@ -18,13 +18,14 @@ namespace nwcch {
 using i32 = int32_t;
 using i64 = int64_t;
 
-// fn(lo, hi) over contiguous slices of [0, n) on up to 16 threads (NWCCH_THREADS
-// caps it; small inputs stay on the calling thread).  fn must not throw.
+// fn(lo, hi) over contiguous, disjoint slices of [0, n) on up to 16 threads
+// (NWCCH_THREADS caps it; one thread per `grain` items at most, so small
+// inputs stay on the calling thread).  fn must not throw.
 template <class F>
-void parallel_ranges(i64 n, F&& fn) {
+void parallel_ranges(i64 n, F&& fn, i64 grain = 65536) {
   int nth = (int)std::thread::hardware_concurrency();
   if (const char* e = std::getenv("NWCCH_THREADS")) nth = std::atoi(e);
-  nth = (int)std::max<i64>(1, std::min<i64>({(i64)nth, 16, n / 65536 + 1}));
+  nth = (int)std::max<i64>(1, std::min<i64>({(i64)nth, 16, n / grain + 1}));
   std::vector<std::thread> th;
   for (int w = 1; w < nth; w++) th.emplace_back(fn, n * w / nth, n * (w + 1) / nth);
   fn((i64)0, n / nth);
