@@ -154,3 +154,30 @@ def test_bench_refuses_more_than_one_gpu():
     r = subprocess.run([sys.executable, str(ROOT / "bench.py"), "--gpus", "2"], capture_output=True, text=True,
                        timeout=120)
     assert r.returncode != 0 and "one GPU" in r.stderr and r.stdout.strip() == ""
+
+
+def test_host_threads_do_not_change_the_result():
+    """The host build's passes on threads (nested dissection, undirected lists,
+    arc map, triangle counts, schedule sorts, triangle list) give what one
+    thread gives, on the 264,000-point road graph."""
+    import os
+    g = graphs.make_config(1)
+    old = os.environ.get("NWCCH_THREADS")
+    res = []
+    try:
+        for th in ("1", "8"):
+            os.environ["NWCCH_THREADS"] = th
+            rank = _lib.order(g.n, g.off, g.tgt, g.x, g.y, 64)
+            t = _lib.Topology.from_graph(g.n, g.off, g.tgt, rank)
+            res.append((rank, t.export(), t.export_triangles()))
+    finally:
+        if old is None:
+            os.environ.pop("NWCCH_THREADS", None)
+        else:
+            os.environ["NWCCH_THREADS"] = old
+    (r1, e1, x1), (r8, e8, x8) = res
+    assert np.array_equal(r1, r8)
+    for k in e1:
+        assert np.array_equal(e1[k], e8[k]), k
+    for a, b in zip(x1, x8):
+        assert np.array_equal(a, b)
