@@ -23,6 +23,10 @@ The same run also reports, as extra fields:
     (cpp/cch.cpp, compiled from its sources) on the host, or the oracle port
     where the reference was not built.
 
+Other modes: `--full-scale` runs the 23.4 M-vertex road graph (customization,
+batched sweep, queries, checks against the reference's C++); `--prep` times
+everything after the order for `--config` on the host and on the GPU.
+
 Prints ONE JSON line.
 """
 from __future__ import annotations
