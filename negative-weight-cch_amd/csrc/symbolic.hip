@@ -13,8 +13,9 @@
 //
 // The elimination tree itself comes first, on the host, from the original graph
 // by Liu's algorithm (a union-find walk with path compression, near-linear).
-// The GPU then builds the rows by ET height, leaves first: one wavefront per
-// vertex marks its candidates in an LDS bitmap, counts them, reserves its slice
+// The host also relabels the neighbour lists into rank space once, for that
+// walk and for the GPU.  The GPU then builds the rows by ET height, leaves
+// first: one wavefront per vertex marks its candidates in an LDS bitmap, counts them, reserves its slice
 // of a staging buffer with one atomic add and writes the row in rank order.  A
 // last kernel moves the rows into the CSR order, and another maps every
 // original arc to its shortcut (python/cch.py:188-204, python/cch_nb.py:496-515)
@@ -254,8 +255,8 @@ bool symbolic_device(Topology& t, i64 n, const std::vector<i64>& noff, const std
     if (p >= 0 && height[p] < height[x] + 1) height[p] = height[x] + 1;
     maxh = std::max(maxh, height[x]);
   }
-  // children lists and vertices by height (the original neighbours are read
-  // on the device through inv and rank)
+  // children lists and vertices by height (the device reads a vertex's
+  // original neighbours as ranks, through inv and the relabelled lists)
   std::vector<i64> kid_off(n + 1, 0), lv_off((size_t)maxh + 2, 0);
   for (i64 x = 0; x < n; x++) {
     if (parent[x] >= 0) kid_off[parent[x] + 1]++;
