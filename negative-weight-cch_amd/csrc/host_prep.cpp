@@ -1,13 +1,12 @@
 // Metric-independent CCH preprocessing on the host: undirected CSR, nested
-// dissection with inertial-flow bisection, symbolic contraction, the original-arc
-// to shortcut map, the pull-form lower-triangle index and the level schedule.
+// dissection with inertial-flow bisection, the pull-form lower-triangle index
+// and the level schedule (the symbolic contraction and the arc map are in
+// contract_host.cpp).
 //
 // Reference behaviour followed (not its code):
 //   undirected CSR          python/cch_nb.py:22-71, cpp/cch.cpp:97-130
 //   inertial-flow ND        python/inertial_flow.py:14-150, python/cch.py:42-106,
 //                           docs/technical_note.md:194-196
-//   symbolic contraction    python/cch.py:136-166, python/cch_nb.py:431-481
-//   edge -> arc map         python/cch.py:188-204, python/cch_nb.py:496-515
 //   lower triangles         python/cch.py:168-186 (by_c grouping: cch.py:308)
 //   levels                  cpp/cch.cpp:468-480
 #include "nwcch_topo.h"
@@ -508,89 +507,14 @@ void nested_dissection(i64 n, const std::vector<i64>& noff, const std::vector<i3
     if (rank[v] < 0) throw std::runtime_error("nested dissection left a vertex unranked");
 }
 
-// -------------------------------------------------------- symbolic contraction
-
-void symbolic(Topology& t, i64 n, const std::vector<i64>& noff, const std::vector<i32>& nbr,
-              const i32* rank) {
-  t.n = n;
-  t.rank.assign(rank, rank + n);
-  std::vector<i32> inv(n, -1);
-  for (i64 v = 0; v < n; v++) {
-    i32 r = rank[v];
-    if (r < 0 || r >= n || inv[r] >= 0) throw std::runtime_error("order must be a permutation");
-    inv[r] = (i32)v;
-  }
-  t.parent.assign(n, -1);
-  t.first.assign(n + 1, 0);
-  t.head.clear();
-  t.head.reserve(4 * n);
-  std::vector<i32> kid_head(n, -1), kid_next(n, -1), mark(n, -1), row;
-  for (i64 x = 0; x < n; x++) {
-    row.clear();
-    const i32 v = inv[x];
-    for (i64 j = noff[v]; j < noff[v + 1]; j++) {
-      i32 r = rank[nbr[j]];
-      if (r > x && mark[r] != x) { mark[r] = (i32)x; row.push_back(r); }
-    }
-    // upper neighbours of each child, minus x itself, join x's row
-    for (i32 c = kid_head[x]; c >= 0; c = kid_next[c])
-      for (i64 a = t.first[c]; a < t.first[c + 1]; a++) {
-        i32 r = t.head[a];
-        if (r != x && mark[r] != x) { mark[r] = (i32)x; row.push_back(r); }
-      }
-    std::sort(row.begin(), row.end());
-    t.head.insert(t.head.end(), row.begin(), row.end());
-    t.first[x + 1] = (i64)t.head.size();
-    if (!row.empty()) {
-      i32 p = row[0];
-      t.parent[x] = p;
-      kid_next[x] = kid_head[p];
-      kid_head[p] = (i32)x;
-    }
-  }
-  t.mp = (i64)t.head.size();
-  if (t.mp >= ((i64)1 << 31) - 1) throw std::runtime_error("too many shortcut arcs for int32 ids");
-}
-
-static i64 find_arc(const Topology& t, i64 lo, i64 hi) {
-  auto b = t.head.begin() + t.first[lo], e = t.head.begin() + t.first[lo + 1];
-  auto it = std::lower_bound(b, e, (i32)hi);
-  if (it == e || *it != hi) return -1;
-  return it - t.head.begin();
-}
-
-void map_arcs(Topology& t, const i64* off, const i32* tgt, i64 m) {
-  t.m = m;
-  t.e_arc.assign(m, -1);
-  t.e_up.assign(m, 0);
-  t.loop_edges.clear();
-  std::atomic<bool> missing{false};
-  parallel_ranges(t.n, [&](i64 u0, i64 u1) {  // each arc written once; loops stay -1
-    for (i64 u = u0; u < u1; u++) {
-      const i64 ru = t.rank[u];
-      for (i64 k = off[u]; k < off[u + 1]; k++) {
-        const i64 rv = t.rank[tgt[k]];
-        if (ru == rv) continue;
-        const i64 a = ru < rv ? find_arc(t, ru, rv) : find_arc(t, rv, ru);
-        if (a < 0) missing.store(true, std::memory_order_relaxed);
-        t.e_arc[k] = a;
-        t.e_up[k] = ru < rv ? 1 : 0;
-      }
-    }
-  });
-  if (missing.load()) throw std::runtime_error("original arc missing from the contraction");
-  for (i64 u = 0; u < t.n; u++)
-    for (i64 k = off[u]; k < off[u + 1]; k++)
-      if (t.rank[u] == t.rank[tgt[k]]) t.loop_edges.push_back((i32)k);
-}
-
 // ---------------------------------------------- triangle index + schedule
 
 void build_meta(Topology& t) {
   const i64 n = t.n, mp = t.mp;
   if ((i64)t.first.size() != n + 1 || (i64)t.first[n] != mp)
     throw std::runtime_error("first/head sizes disagree");
-  // elimination tree depth and height; parent = first upper neighbour
+  // the elimination tree: a row's smallest entry is the tail's parent; depths
+  // and heights (= schedule levels) from the forest's shape
   t.parent.assign(n, -1);
   for (i64 x = 0; x < n; x++) {
     if (t.first[x + 1] > t.first[x]) {
@@ -599,20 +523,13 @@ void build_meta(Topology& t) {
       t.parent[x] = p;
     }
   }
-  t.vdepth.assign(n, 0);
-  t.depth = 0;
-  for (i64 x = n - 1; x >= 0; x--) {
-    i32 p = t.parent[x];
-    if (p >= 0) { t.vdepth[x] = t.vdepth[p] + 1; t.depth = std::max(t.depth, t.vdepth[x]); }
+  {
+    Forest f = forest_shape(t.parent);
+    t.vdepth = std::move(f.depth);
+    t.vlevel = std::move(f.height);
+    t.depth = f.max_depth;
+    t.levels = f.max_height + 1;
   }
-  t.vlevel.assign(n, 0);
-  i32 maxlvl = 0;
-  for (i64 x = 0; x < n; x++) {
-    i32 p = t.parent[x];
-    if (p >= 0 && t.vlevel[p] < t.vlevel[x] + 1) t.vlevel[p] = t.vlevel[x] + 1;
-    maxlvl = std::max(maxlvl, t.vlevel[x]);
-  }
-  t.levels = maxlvl + 1;
   // T = sum over rows of d(d-1)/2 (every pair of upper neighbours is a triangle)
   t.triangles = 0;
   for (i64 x = 0; x < n; x++) {

@@ -94,11 +94,34 @@ struct Topology {
   std::vector<i32> arc_tail;    // [mp]
 };
 
-// Builders (host_prep.cpp).  They throw std::runtime_error on bad input.
+// The shape of an elimination forest: children lists (ascending), depth (root
+// = 0), height (leaf = 0) and the vertices grouped by height (ascending inside
+// a height).
+struct Forest {
+  std::vector<i64> kid_off;   // [n+1]
+  std::vector<i32> kid;
+  std::vector<i32> depth, height;
+  i32 max_depth = 0, max_height = 0;
+  std::vector<i64> h_off;     // [max_height+2]
+  std::vector<i32> h_vtx;     // [n]
+};
+
+// Builders (host_prep.cpp, contract_host.cpp).  They throw std::runtime_error
+// on bad input.
 void undirected_csr(i64 n, const i64* off, const i32* tgt, std::vector<i64>& noff,
                     std::vector<i32>& nbr);
 void nested_dissection(i64 n, const std::vector<i64>& noff, const std::vector<i32>& nbr,
                        const double* x, const double* y, int leaf, i32* rank_out);
+// inv[rank[v]] = v; throws unless rank is a permutation of [0, n)
+std::vector<i32> inverse_order(i64 n, const i32* rank);
+// the undirected neighbour lists with every vertex id replaced by its rank
+std::vector<i32> rank_space_lists(i64 n, const std::vector<i64>& noff, const std::vector<i32>& nbr,
+                                  const i32* rank);
+// elimination tree of the chordal completion from the rank-space lists (Liu's
+// walk): parent[x] = the smallest higher rank adjacent to x in the completion
+std::vector<i32> elimination_tree(i64 n, const std::vector<i64>& noff, const std::vector<i32>& rnbr,
+                                  const std::vector<i32>& inv);
+Forest forest_shape(const std::vector<i32>& parent);
 void symbolic(Topology& t, i64 n, const std::vector<i64>& noff, const std::vector<i32>& nbr,
               const i32* rank);
 void map_arcs(Topology& t, const i64* off, const i32* tgt, i64 m);

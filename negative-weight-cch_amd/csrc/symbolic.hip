@@ -199,79 +199,25 @@ __global__ void __launch_bounds__(256) k_map_arcs(i32 n, const i64* __restrict__
 
 namespace nwcch {
 
-// Elimination tree of the chordal completion, from the original graph with
-// its neighbour lists in rank space (Liu's algorithm): parent[x] = the
-// smallest higher rank adjacent to x in the completion, -1 for roots.
-static std::vector<i32> elimination_tree(i64 n, const std::vector<i64>& noff, const std::vector<i32>& rnbr,
-                                         const std::vector<i32>& inv) {
-  std::vector<i32> parent(n, -1), anc(n, -1);
-  for (i64 x = 0; x < n; x++) {
-    const i32 v = inv[x];
-    for (i64 j = noff[v]; j < noff[v + 1]; j++) {
-      i32 r = rnbr[j];
-      if (r >= x) continue;
-      while (anc[r] != -1 && anc[r] != x) {
-        const i32 next = anc[r];
-        anc[r] = (i32)x;
-        r = next;
-      }
-      if (anc[r] == -1) {
-        anc[r] = (i32)x;
-        parent[r] = (i32)x;
-      }
-    }
-  }
-  return parent;
-}
-
 bool symbolic_device(Topology& t, i64 n, const std::vector<i64>& noff, const std::vector<i32>& nbr,
                      const i32* rank, int device) {
-  if (n >= ((i64)1 << 31) - 1) throw std::runtime_error("too many vertices for int32 ids");
-  std::vector<i32> inv(n, -1);
-  for (i64 v = 0; v < n; v++) {
-    const i32 r = rank[v];
-    if (r < 0 || r >= n || inv[r] >= 0) throw std::runtime_error("order must be a permutation");
-    inv[r] = (i32)v;
-  }
+  const std::vector<i32> inv = inverse_order(n, rank);
   Steps steps;
   // the neighbour lists with ranks in place of vertex ids (vertex ranges on threads)
-  std::vector<i32> rnbr(nbr.size());
-  parallel_ranges(n, [&](i64 v0, i64 v1) {
-    for (i64 j = noff[v0]; j < noff[v1]; j++) rnbr[j] = rank[nbr[j]];
-  });
+  const std::vector<i32> rnbr = rank_space_lists(n, noff, nbr, rank);
   steps.done("neighbour lists in rank space (host)");
   std::vector<i32> parent = elimination_tree(n, noff, rnbr, inv);
   steps.done("elimination tree (host)");
-  std::vector<i32> depth(n, 0), height(n, 0);
-  i32 maxd = 0, maxh = 0;
-  for (i64 x = n - 1; x >= 0; x--)
-    if (parent[x] >= 0) {
-      depth[x] = depth[parent[x]] + 1;
-      maxd = std::max(maxd, depth[x]);
-    }
+  // children lists, depths, heights and vertices by height (the device reads a
+  // vertex's original neighbours as ranks, through inv and the relabelled lists)
+  const Forest f = forest_shape(parent);
+  const i32 maxd = f.max_depth, maxh = f.max_height;
   if (maxd > kSymMaxDepth) return false;  // the caller contracts on the host
-  for (i64 x = 0; x < n; x++) {
-    const i32 p = parent[x];
-    if (p >= 0 && height[p] < height[x] + 1) height[p] = height[x] + 1;
-    maxh = std::max(maxh, height[x]);
-  }
-  // children lists and vertices by height (the device reads a vertex's
-  // original neighbours as ranks, through inv and the relabelled lists)
-  std::vector<i64> kid_off(n + 1, 0), lv_off((size_t)maxh + 2, 0);
-  for (i64 x = 0; x < n; x++) {
-    if (parent[x] >= 0) kid_off[parent[x] + 1]++;
-    lv_off[height[x] + 1]++;
-  }
-  for (i64 x = 0; x < n; x++) kid_off[x + 1] += kid_off[x];
-  for (i32 h = 0; h <= maxh; h++) lv_off[h + 1] += lv_off[h];
-  std::vector<i32> kid(std::max<i64>(kid_off[n], 1)), lv_vtx(n);
-  {
-    std::vector<i64> kp(kid_off.begin(), kid_off.end() - 1), lp(lv_off.begin(), lv_off.end() - 1);
-    for (i64 x = 0; x < n; x++) {
-      if (parent[x] >= 0) kid[kp[parent[x]]++] = (i32)x;
-      lv_vtx[lp[height[x]]++] = (i32)x;
-    }
-  }
+  const std::vector<i64>& kid_off = f.kid_off;
+  const std::vector<i32>& kid = f.kid;
+  const std::vector<i32>& depth = f.depth;
+  const std::vector<i64>& lv_off = f.h_off;
+  const std::vector<i32>& lv_vtx = f.h_vtx;
 
   steps.done("depths, heights, child and level lists (host)");
   HIP_OK(hipSetDevice(device));
@@ -284,7 +230,9 @@ bool symbolic_device(Topology& t, i64 n, const std::vector<i64>& noff, const std
   TempBuf<uint32_t> d_flag(1);
   const size_t lds = sizeof(uint32_t) * ((size_t)(maxd + 31) / 32 + (size_t)maxd);
   // staging buffer: a first guess, doubled until every row fits
+  // (NWCCH_SYM_STAGE_CAP sets a smaller first guess, to exercise the rerun)
   i64 cap = std::max<i64>(2 * noff[n] + n, 1 << 16);
+  if (const char* e = std::getenv("NWCCH_SYM_STAGE_CAP")) cap = std::max<i64>(1, std::atoll(e));
   unsigned long long total = 0;
   std::vector<i32> len(n);
   for (;;) {

@@ -26,7 +26,7 @@ HIP_FLAGS = [
 HOST_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-march=x86-64-v2"]
 
 SOURCES_HIP = ["device.hip", "update.hip", "batch.hip", "symbolic.hip"]
-SOURCES_CPP = ["capi.cpp", "host_prep.cpp"]
+SOURCES_CPP = ["capi.cpp", "host_prep.cpp", "contract_host.cpp"]
 
 
 def hipcc() -> str:
@@ -65,6 +65,14 @@ def build(verbose: bool = False) -> Path:
             print(" ".join(cmd))
         _run(cmd)
         objs.append(str(o))
+    # the synthetic-graph generator's host helpers: their own library, no HIP
+    gen = PKG / "libnwgen.so"
+    gtmp = gen.with_suffix(".so.tmp")
+    cmd = ["g++", *HOST_FLAGS, "-shared", str(CSRC / "gen_host.cpp"), "-o", str(gtmp), "-pthread"]
+    if verbose:
+        print(" ".join(cmd))
+    _run(cmd)
+    os.replace(gtmp, gen)
     tmp = LIB.with_suffix(".so.tmp")
     cmd = [hc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *objs]
     if verbose:
