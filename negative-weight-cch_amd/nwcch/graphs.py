@@ -83,31 +83,61 @@ def from_arcs(n: int, arcs, xy=None, name="") -> RoadGraph:
     return RoadGraph(n, off, tgt, w, x, y, None, name)
 
 
+_GEN = None
+
+
+def _gen():
+    """The generator's native helpers (libnwgen.so, built with the package):
+    threaded sorts, Kruskal's spanning forest, Tarjan's SCC and the CSR build."""
+    global _GEN
+    if _GEN is None:
+        import ctypes
+        from pathlib import Path
+        path = Path(__file__).resolve().parent / "libnwgen.so"
+        if not path.exists():
+            raise RuntimeError(f"{path} is missing: build the package first (__graft_entry__.build())")
+        lib = ctypes.CDLL(str(path))
+        P = ctypes.c_void_p
+        I = ctypes.c_int64
+        for name, args, res in (("nwgen_unique_first", [P, I, P], I), ("nwgen_unique_sorted", [P, I], I),
+                                ("nwgen_mst", [I, P, P, I, P, P, P], ctypes.c_int),
+                                ("nwgen_csr", [I, P, P, I, P, P], ctypes.c_int),
+                                ("nwgen_largest_scc", [I, P, P, I, P], I)):
+            f = getattr(lib, name)
+            f.argtypes, f.restype = args, res
+        _GEN = lib
+    return _GEN
+
+
+def _ptr(a):
+    return a.ctypes.data
+
+
 def _csr_from_pairs(n, u, v):
-    """Sort directed pairs by (u, v) and return off, tgt, permutation."""
-    key = u.astype(np.int64) * n + v.astype(np.int64)
-    perm = np.argsort(key, kind="stable")
-    u_s = u[perm]
-    off = np.zeros(n + 1, np.int64)
-    np.add.at(off, u_s.astype(np.int64) + 1, 1)
-    return np.cumsum(off), v[perm].astype(np.int32), perm
+    """Sort distinct directed pairs by (u, v): off int64[n+1], tgt int32[m]."""
+    u = np.ascontiguousarray(u, dtype=np.int32)
+    v = np.ascontiguousarray(v, dtype=np.int32)
+    off = np.empty(n + 1, np.int64)
+    tgt = np.empty(len(u), np.int32)
+    _gen().nwgen_csr(n, _ptr(u), _ptr(v), len(u), _ptr(off), _ptr(tgt))
+    return off, tgt
 
 
 def _largest_scc(n, u, v):
-    from scipy.sparse import csr_matrix
-    from scipy.sparse.csgraph import connected_components
-    a = csr_matrix((np.ones(len(u), np.int8), (u, v)), shape=(n, n))
-    _, lab = connected_components(a, directed=True, connection="strong")
-    big = np.argmax(np.bincount(lab))
-    keep = lab == big
+    u = np.ascontiguousarray(u, dtype=np.int32)
+    v = np.ascontiguousarray(v, dtype=np.int32)
+    keep8 = np.empty(n, np.uint8)
+    _gen().nwgen_largest_scc(n, _ptr(u), _ptr(v), len(u), _ptr(keep8))
+    keep = keep8.view(bool)
     newid = -np.ones(n, np.int64)
     newid[keep] = np.arange(int(keep.sum()))
     sel = keep[u] & keep[v]
     return keep, newid, sel
 
 
-def _energy(x, y, h, u, v, alpha=ALPHA, slope=SLOPE):
-    ln = np.hypot(x[v] - x[u], y[v] - y[u])
+def _energy(x, y, h, u, v, alpha=ALPHA, slope=SLOPE, ln=None):
+    if ln is None:
+        ln = np.hypot(x[v] - x[u], y[v] - y[u])
     w = np.rint(alpha * ln).astype(np.int64) + slope * (h[v].astype(np.int64) - h[u].astype(np.int64))
     assert np.abs(w).max(initial=0) < 2 ** 30, "weights must stay well inside int32"
     return w.astype(np.int32)
@@ -144,7 +174,7 @@ def grid_graph(side=64, seed=0, drop=0.10, bumps=8, bump_sigma=5.0, hmax=400,
     u, v = newid[u[sel]], newid[v[sel]]
     x, y, h = x[keepv], y[keepv], h[keepv]
     n = int(keepv.sum())
-    off, tgt, perm = _csr_from_pairs(n, u, v)
+    off, tgt = _csr_from_pairs(n, u, v)
     tails = np.repeat(np.arange(n), np.diff(off))
     w = _energy(x, y, h, tails, tgt.astype(np.int64), alpha, slope)
     g = RoadGraph(n, off, tgt, w, x, y, h, f"grid{side}_s{seed}")
@@ -181,52 +211,161 @@ def sample_field(f, box, x, y):
     return v
 
 
+_TILES = {}   # the point set and its tile index, shared with forked workers
+
+
 def _tile_edges(job):
-    """Delaunay edges of one tile: the triangulation of the tile's points plus a
-    margin, keeping the edges whose midpoint lies in the tile (half-open, so
-    every edge has one owner).  Returns sorted global (i, j) pairs, i < j."""
+    """Delaunay edges owned by one tile, triangulated with `margin` metres of
+    neighbouring points.  An owned edge (its midpoint in the tile, half-open,
+    so every edge has one owner) is kept only if one of its triangles has its
+    circumdisk inside the region whose points were all included (every point
+    lies in the box, so only the part of the disk inside the box counts): that
+    disk is then empty of every point, so the edge is an edge of the global
+    Delaunay triangulation.  Every global edge with its midpoint in the tile is present
+    in the local triangulation (an empty circle stays empty when points are
+    removed).  Returns the sorted distinct global (i, j) pairs, i < j, or None
+    when some owned edge is not certified (the caller retries with a wider
+    margin).  With certify False the owned edges are returned unchecked."""
     from scipy.spatial import Delaunay
-    idx, P, x0, x1, y0, y1 = job
+    i, j, margin, certify = job
+    T = _TILES
+    pts, order, bounds, tiles, w, box = T["pts"], T["order"], T["bounds"], T["tiles"], T["w"], T["box"]
+    x0 = i * w if i else -np.inf
+    x1 = (i + 1) * w if i < tiles - 1 else np.inf
+    y0 = j * w if j else -np.inf
+    y1 = (j + 1) * w if j < tiles - 1 else np.inf
+    k = int(np.ceil(margin / w))
+    parts = [order[bounds[a * tiles + b]:bounds[a * tiles + b + 1]]
+             for a in range(max(0, i - k), min(tiles, i + k + 1)) for b in range(max(0, j - k), min(tiles, j + k + 1))]
+    cand = np.sort(np.concatenate(parts))
+    P = pts[cand]
+    X0, X1, Y0, Y1 = x0 - margin, x1 + margin, y0 - margin, y1 + margin
+    keep = (P[:, 0] >= X0) & (P[:, 0] < X1) & (P[:, 1] >= Y0) & (P[:, 1] < Y1)
+    idx, P = cand[keep], P[keep]
     if len(idx) < 3:
         return np.empty((0, 2), np.int64)
     tri = Delaunay(P).simplices
+    # circumcircles, in coordinates relative to each triangle's first corner
+    A = P[tri[:, 0]]
+    bx, by = (P[tri[:, 1]] - A).T
+    cx, cy = (P[tri[:, 2]] - A).T
+    d = 2.0 * (bx * cy - by * cx)
+    with np.errstate(divide="ignore", invalid="ignore", over="ignore"):
+        ux = (cy * (bx * bx + by * by) - by * (cx * cx + cy * cy)) / d
+        uy = (bx * (cx * cx + cy * cy) - cx * (bx * bx + by * by)) / d
+        r = np.hypot(ux, uy) * (1 + 1e-9) + 1e-6
+        ux += A[:, 0]
+        uy += A[:, 1]
+        # every point lies in [0, box]^2: the disk must miss the parts of the
+        # box outside the included region (up to four strips), each tested
+        # exactly as disk against rectangle
+        inside = np.isfinite(ux) & np.isfinite(uy) & np.isfinite(r)
+        r2 = r * r
+        strips = [(rx0, rx1, ry0, ry1) for rx0, rx1, ry0, ry1 in
+                  ((0.0, X0, 0.0, box), (X1, box, 0.0, box), (0.0, box, 0.0, Y0), (0.0, box, Y1, box))
+                  if rx1 >= rx0 and ry1 >= ry0]   # a strip is empty where the region reaches past the box
+        for rx0, rx1, ry0, ry1 in strips:
+            dx = np.maximum(np.maximum(rx0 - ux, ux - rx1), 0.0)
+            dy = np.maximum(np.maximum(ry0 - uy, uy - ry1), 0.0)
+            inside &= dx * dx + dy * dy > r2
+    nt = len(tri)
     e = np.concatenate([tri[:, [0, 1]], tri[:, [1, 2]], tri[:, [2, 0]]])
+    et = np.concatenate([np.arange(nt)] * 3)
     mx = 0.5 * (P[e[:, 0], 0] + P[e[:, 1], 0])
     my = 0.5 * (P[e[:, 0], 1] + P[e[:, 1], 1])
     own = (mx >= x0) & (mx < x1) & (my >= y0) & (my < y1)
-    g = idx[e[own]].astype(np.int64)
+    e, et = e[own], et[own]
+    g = idx[e].astype(np.int64)
     g.sort(axis=1)
-    return g
+    key = g[:, 0] * len(pts) + g[:, 1]
+    o = np.argsort(key, kind="stable")
+    key = key[o]
+    start = np.flatnonzero(np.r_[True, key[1:] != key[:-1]])
+    if not len(key):
+        return np.empty((0, 2), np.int64)
+    if not certify:
+        return g[o[start]]
+    ok = np.maximum.reduceat(inside[et[o]], start)
+    # an edge of one local triangle (on the local hull): every circle through
+    # its ends whose centre lies beyond the triangle's circumcentre, away from
+    # the triangle, is empty of included points too; try such circles with
+    # centres ever farther out (their part inside the box shrinks toward the
+    # edge before it stretches along the box's border)
+    one = np.diff(np.r_[start, len(key)]) == 1
+    todo = np.flatnonzero(one & ~ok)
+    if len(todo):
+        k = o[start[todo]]
+        pa, pb = P[e[k, 0]], P[e[k, 1]]
+        pr = P[tri[et[k]].sum(1) - e[k, 0] - e[k, 1]]
+        mid = 0.5 * (pa + pb)
+        hl = 0.5 * np.hypot(*(pb - pa).T)
+        nrm = np.stack([-(pb - pa)[:, 1], (pb - pa)[:, 0]], 1) / (2.0 * hl[:, None])
+        nrm *= -np.sign(((pr - mid) * nrm).sum(1))[:, None]      # pointing away from the triangle
+        tt = et[k]
+        s1 = np.nan_to_num((np.stack([ux[tt], uy[tt]], 1) - mid) * nrm, nan=np.inf).sum(1)
+        base = np.maximum(s1, 0.0)
+        good = np.zeros(len(todo), bool)
+        for step in range(41):
+            sv = base + 2.0 * hl * (2.0 ** step - 1.0)
+            live = ~good & np.isfinite(sv) & (sv < 1e12)
+            if not live.any():
+                break
+            cx_ = mid[:, 0] + sv * nrm[:, 0]
+            cy_ = mid[:, 1] + sv * nrm[:, 1]
+            rr = np.hypot(hl, sv) * (1 + 1e-9) + 1e-6
+            fits = live.copy()
+            for rx0, rx1, ry0, ry1 in strips:
+                dx = np.maximum(np.maximum(rx0 - cx_, cx_ - rx1), 0.0)
+                dy = np.maximum(np.maximum(ry0 - cy_, cy_ - ry1), 0.0)
+                fits &= dx * dx + dy * dy > rr * rr
+            good |= fits
+        ok[todo] = good
+    if not ok.all():
+        return None
+    return g[o[start]]
 
 
-def tiled_delaunay_edges(pts, box, tiles, margin, progress=None, workers=16):
+def tiled_delaunay_edges(pts, box, tiles, margin=500.0, wide_margin=4000.0, progress=None, workers=16):
     """Delaunay edges of a large point set, triangulated tile by tile (tiles x
-    tiles squares of the box, each with `margin` metres of neighbouring points)
-    in a process pool.  Away from tile borders the triangulation is the global
-    one; an edge is taken from the tile that holds its midpoint."""
+    tiles squares of the box) in a process pool.  An inner tile takes `margin`
+    metres of neighbouring points and keeps its edges if all are certified
+    (see _tile_edges).  Tiles on the box's border (whose hull edges along a
+    clipped, nearly straight border cannot be certified from a neighbourhood)
+    and inner tiles that fail take `wide_margin` metres, and their owned edges
+    are kept as they come."""
     import concurrent.futures as cf
     import multiprocessing as mp
+    import os
     w = box / tiles
     tx = np.minimum((pts[:, 0] / w).astype(np.int64), tiles - 1)
     ty = np.minimum((pts[:, 1] / w).astype(np.int64), tiles - 1)
-    jobs = []
-    for i in range(tiles):
-        for j in range(tiles):
-            x0, x1 = i * w, (i + 1) * w if i < tiles - 1 else np.inf
-            y0, y1 = j * w, (j + 1) * w if j < tiles - 1 else np.inf
-            lo = (np.abs(tx - i) <= 1) & (np.abs(ty - j) <= 1)
-            cand = np.nonzero(lo)[0]
-            P = pts[cand]
-            keep = ((P[:, 0] >= x0 - margin) & (P[:, 0] < x1 + margin) &
-                    (P[:, 1] >= y0 - margin) & (P[:, 1] < y1 + margin))
-            jobs.append((cand[keep], P[keep], x0 if i else -np.inf, x1, y0 if j else -np.inf, y1))
-    out = []
-    with cf.ProcessPoolExecutor(max_workers=workers, mp_context=mp.get_context("fork")) as ex:
-        for k, e in enumerate(ex.map(_tile_edges, jobs)):
-            out.append(e)
-            if progress and (k + 1) % tiles == 0:
-                progress(f"delaunay tiles {k + 1}/{len(jobs)}")
-    return np.concatenate(out)
+    tid = tx * tiles + ty
+    order = np.argsort(tid, kind="stable")
+    bounds = np.searchsorted(tid[order], np.arange(tiles * tiles + 1))
+    _TILES.update(pts=pts, order=order, bounds=bounds, tiles=tiles, w=w, box=box)
+    workers = max(1, min(workers, os.cpu_count() or 1))
+    out = {}
+    inner = [(i, j) for i in range(1, tiles - 1) for j in range(1, tiles - 1)]
+    wide = [(i, j) for i in range(tiles) for j in range(tiles) if (i, j) not in set(inner)]
+    try:
+        with cf.ProcessPoolExecutor(max_workers=workers, mp_context=mp.get_context("fork")) as ex:
+            jobs = [(i, j, margin, True) for i, j in inner] + [(i, j, wide_margin, False) for i, j in wide]
+            # the wide border tiles first: they take longest
+            jobs = jobs[len(inner):] + jobs[:len(inner)]
+            for (i, j, _, _), e in zip(jobs, ex.map(_tile_edges, jobs)):
+                if e is None:
+                    wide.append((i, j))
+                else:
+                    out[(i, j)] = e
+            failed = [t for t in inner if t not in out]
+            if progress:
+                progress(f"delaunay tiles: {len(inner) - len(failed)} of {len(inner)} inner tiles certified with a "
+                         f"{margin:.0f} m margin; {tiles * tiles - len(inner)} border tiles with {wide_margin:.0f} m")
+            for (i, j), e in zip(failed, ex.map(_tile_edges, [(i, j, wide_margin, False) for i, j in failed])):
+                out[(i, j)] = e
+    finally:
+        _TILES.clear()
+    return np.concatenate([out[(i, j)] for i in range(tiles) for j in range(tiles)])
 
 
 def road_graph(n_points, seed, box=60000.0, clusters=200, sigma=2000.0, out_degree=2.8,
@@ -240,8 +379,6 @@ def road_graph(n_points, seed, box=60000.0, clusters=200, sigma=2000.0, out_degr
     of millions of points); ``progress`` receives a line after each stage."""
     say = progress or (lambda msg: None)
     from scipy.spatial import Delaunay
-    from scipy.sparse import coo_matrix
-    from scipy.sparse.csgraph import minimum_spanning_tree
 
     rng = np.random.default_rng(seed)
     centers = rng.uniform(0, box, (clusters, 2))
@@ -257,27 +394,30 @@ def road_graph(n_points, seed, box=60000.0, clusters=200, sigma=2000.0, out_degr
     # distinct points in lexicographic (x, y) order, through integer keys
     # (millimetre coordinates, at most 2^26 in the box) rather than row sorting
     kxy = (np.rint(pts[:, 0] * 1000).astype(np.int64) << 32) | np.rint(pts[:, 1] * 1000).astype(np.int64)
-    _, first = np.unique(kxy, return_index=True)
+    first = np.empty(len(kxy), np.int64)
+    first = first[:_gen().nwgen_unique_first(_ptr(kxy), len(kxy), _ptr(first))]
     pts = pts[first]
     pts = pts[rng.permutation(pts.shape[0])]
     n0 = pts.shape[0]
     say(f"points: {n0}")
     if tiles:
-        e = tiled_delaunay_edges(pts, box, tiles, margin=2.0 * sigma, progress=say)
+        e = tiled_delaunay_edges(pts, box, tiles, progress=say)
     else:
         tri = Delaunay(pts).simplices
         e = np.concatenate([tri[:, [0, 1]], tri[:, [1, 2]], tri[:, [2, 0]]])
         e.sort(axis=1)
-    ke = np.unique(e[:, 0].astype(np.int64) * n0 + e[:, 1])   # distinct edges, lexicographic
+    ke = np.ascontiguousarray(e[:, 0].astype(np.int64) * n0 + e[:, 1])
+    ke = ke[:_gen().nwgen_unique_sorted(_ptr(ke), len(ke))]    # distinct edges, lexicographic
     e = np.stack([ke // n0, ke % n0], 1).astype(np.int32)
     say(f"delaunay edges: {len(e)}")
-    ln = np.hypot(*(pts[e[:, 0]] - pts[e[:, 1]]).T)
-    mst = minimum_spanning_tree(coo_matrix((ln + 1e-9, (e[:, 0], e[:, 1])), shape=(n0, n0))).tocoo()
+    ea, eb = np.ascontiguousarray(e[:, 0]), np.ascontiguousarray(e[:, 1])
+    px, py = np.ascontiguousarray(pts[:, 0]), np.ascontiguousarray(pts[:, 1])
+    in_tree = np.empty(len(e), np.uint8)
+    _gen().nwgen_mst(n0, _ptr(ea), _ptr(eb), len(e), _ptr(px), _ptr(py), _ptr(in_tree))
+    in_tree = in_tree.view(bool)
     say("spanning tree done")
-    mst_e = np.stack([np.minimum(mst.row, mst.col), np.maximum(mst.row, mst.col)], 1)
-    key_all = e[:, 0].astype(np.int64) * n0 + e[:, 1]
-    key_mst = mst_e[:, 0].astype(np.int64) * n0 + mst_e[:, 1]
-    rest = e[~np.isin(key_all, key_mst)]
+    mst_e = e[in_tree]
+    rest = e[~in_tree]
     # target undirected edge count: arcs = E * (2 - oneway) = out_degree * n
     target = int(round(out_degree * n0 / (2.0 - oneway)))
     extra = max(0, min(rest.shape[0], target - mst_e.shape[0]))
@@ -294,25 +434,28 @@ def road_graph(n_points, seed, box=60000.0, clusters=200, sigma=2000.0, out_degr
     u, v = newid[u[sel]], newid[v[sel]]
     pts = pts[keepv]
     n = int(keepv.sum())
-    off, tgt, perm = _csr_from_pairs(n, u, v)
-    tails = np.repeat(np.arange(n), np.diff(off))
+    off, tgt = _csr_from_pairs(n, u, v)
+    say("largest strongly connected component and CSR done")
+    tails = np.repeat(np.arange(n, dtype=np.int32), np.diff(off))
     x, y = pts[:, 0].copy(), pts[:, 1].copy()
+    ln = np.hypot(x[tgt] - x[tails], y[tgt] - y[tails])
     hf = sample_field(fractal_height(rng, box, hmax), box, x, y)
-    dz = vertical_scale(x, y, hf, tails, tgt, alpha, slope, neg_share)
+    dz = vertical_scale(x, y, hf, tails, tgt, alpha, slope, neg_share, ln=ln)
     h = np.rint(hf * dz).astype(np.int32)
-    w = _energy(x, y, h, tails, tgt.astype(np.int64), alpha, slope)
+    w = _energy(x, y, h, tails, tgt, alpha, slope, ln=ln)
     g = RoadGraph(n, off, tgt, w, x, y, h, name or f"road{n_points}_s{seed}")
     g.extra.update(dict(box=box, clusters=clusters, sigma=sigma, hmax=hmax, dz_scale=dz))
     return g
 
 
-def vertical_scale(x, y, hf, tails, heads, alpha, slope, neg_share=0.10):
+def vertical_scale(x, y, hf, tails, heads, alpha, slope, neg_share=0.10, ln=None):
     """Vertical exaggeration of a height field so that about ``neg_share`` of the arcs
     come out negative.  This is the reference's rule for its ``ev`` metric on road
     graphs (python/dimacs.py:155-160, python/bigcsr.py:155-161: the 80th percentile
     of downhill energy slopes is scaled to 1).  A literal 0-800 m field over a
     60 km box gives grades far below alpha/slope = 1.6 and no negative arc at all."""
-    ln = np.hypot(x[heads] - x[tails], y[heads] - y[tails])
+    if ln is None:
+        ln = np.hypot(x[heads] - x[tails], y[heads] - y[tails])
     drop = hf[tails] - hf[heads]
     sel = (drop > 0) & (ln > 0)
     if not sel.any():
@@ -330,16 +473,24 @@ def metric_family(g: RoadGraph, K: int, seed: int, hmax=1200, slope_range=(5, 40
     :func:`vertical_scale`)."""
     rng = np.random.default_rng(seed)
     box = box or g.extra.get("box", float(max(g.x.max(), g.y.max())))
-    tails = g.tails().astype(np.int64)
-    heads = g.tgt.astype(np.int64)
+    tails = g.tails()
+    heads = g.tgt
+    ln = np.hypot(g.x[heads] - g.x[tails], g.y[heads] - g.y[tails])
     W = np.empty((K, g.m), np.int32)
     slopes = rng.integers(slope_range[0], slope_range[1] + 1, K)
-    dzs = np.empty(K)
-    for k in range(K):
-        hf = sample_field(fractal_height(rng, box, hmax), box, g.x, g.y)
-        dzs[k] = vertical_scale(g.x, g.y, hf, tails, heads, alpha, int(slopes[k]), neg_share)
-        h = np.rint(hf * dzs[k]).astype(np.int32)
-        W[k] = _energy(g.x, g.y, h, tails, heads, alpha, int(slopes[k]))
+    fields = [fractal_height(rng, box, hmax) for _ in range(K)]   # the draws in metric order
+
+    def one(k):
+        hf = sample_field(fields[k], box, g.x, g.y)
+        dz = vertical_scale(g.x, g.y, hf, tails, heads, alpha, int(slopes[k]), neg_share, ln=ln)
+        h = np.rint(hf * dz).astype(np.int32)
+        W[k] = _energy(g.x, g.y, h, tails, heads, alpha, int(slopes[k]), ln=ln)
+
+    # metrics on threads (numpy's array loops release the GIL); at most 8 at
+    # once bounds the temporaries
+    import concurrent.futures as cf
+    with cf.ThreadPoolExecutor(max(1, min(8, K))) as ex:
+        list(ex.map(one, range(K)))
     return W, slopes
 
 
