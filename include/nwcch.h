@@ -259,6 +259,21 @@ int nwcch_update(nwcch_dev* h, int32_t* w, int32_t* ud, const int64_t* edge_idx,
                  int64_t k, int32_t mode, int32_t* changed, int64_t changed_cap, int64_t* result,
                  void* stream);
 
+/* nwcch_update without any allocation.  No counterpart in the reference: it
+ * is for callers that cannot allocate device or pinned memory mid-run (tight
+ * serving loops, code under HIP graph capture, harnesses that forbid it).
+ * Same arguments, semantics and results as nwcch_update, except that the k
+ * changes are staged in a static device array of the library (one per GPU,
+ * held until the process ends) instead of the library's growable pinned and
+ * device slots: k is at most 65536 (NWCCH_ERR_CAPACITY beyond), and the host
+ * arrays edge_idx and new_w must stay unchanged until the stream has reached
+ * the update.  All calls on one GPU share that array: calls that overlap on
+ * one GPU (several streams or threads) race.  nwcch_update_prepare must have
+ * been called (it also locates the array on the handle's GPU). */
+int nwcch_update_noalloc(nwcch_dev* h, int32_t* w, int32_t* ud, const int64_t* edge_idx, const int32_t* new_w,
+                         int64_t k, int32_t mode, int32_t* changed, int64_t changed_cap, int64_t* result,
+                         void* stream);
+
 /* Copy the first `cap` changed arcs of the last update into out (device). */
 int nwcch_update_changed(nwcch_dev* h, int32_t* out, int64_t cap, void* stream);
 

@@ -19,6 +19,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -274,6 +275,8 @@ UpdArgs make_args(DevTopo& d, int32_t* w, int32_t* ud, int32_t* chg, i64 chg_cap
 
 }  // namespace
 
+int nwcch_update_noalloc_setup(int device);
+
 // Builds the update index of a topology on the host (once) and uploads it.
 extern "C" int nwcch_update_prepare(nwcch_dev* h, nwcch_topo* topo) {
   if (!h || !topo) { set_error("null argument"); return NWCCH_ERR_ARG; }
@@ -346,6 +349,8 @@ extern "C" int nwcch_update_prepare(nwcch_dev* h, nwcch_topo* topo) {
       HIP_OK(hipEventRecord(sg.ev, 0));
     }
     HIP_OK(hipDeviceSynchronize());
+    // the static staging of nwcch_update_noalloc, located once per device
+    if (nwcch_update_noalloc_setup(d.device) != NWCCH_OK) throw std::runtime_error(nwcch_last_error());
     d.has_update = true;
   } catch (const std::exception& e) {
     set_error(e.what());
@@ -353,6 +358,67 @@ extern "C" int nwcch_update_prepare(nwcch_dev* h, nwcch_topo* topo) {
   }
   return NWCCH_OK;
 }
+
+namespace {
+
+// Host validation of k changes; *L0 = the lowest ET height that can hold a
+// queued arc (d.levels if none).  Returns NWCCH_OK or NWCCH_ERR_ARG with the
+// message set (no exception on this path).
+int check_changes(const DevTopo& d, const int64_t* edge_idx, const int32_t* new_w, int64_t k, int32_t* L0) {
+  int32_t lo = d.levels;
+  for (int64_t i = 0; i < k; i++) {
+    const int64_t e = edge_idx[i];
+    if (e < 0 || e >= d.m) { set_error("edge index out of range"); return NWCCH_ERR_ARG; }
+    if (new_w[i] == (int32_t)0x80000000) { set_error("INT32_MIN is not a valid weight"); return NWCCH_ERR_ARG; }
+    const int32_t a = d.e_arc_host[e];
+    if (a >= 0) lo = std::min(lo, d.arc_lvl_host[a]);
+  }
+  *L0 = lo;
+  return NWCCH_OK;
+}
+
+// The update's launches once the k changes sit in device memory (e_dev,
+// w_dev): reset, seed, the frontier by height (one persistent workgroup or
+// one launch per height) and the result.  Allocates nothing.
+void launch_update(DevTopo& d, int32_t* w, int32_t* ud, const i64* e_dev, const int32_t* w_dev, int64_t k,
+                   int32_t L0, int32_t mode, int32_t* changed, int64_t changed_cap, int64_t* result,
+                   hipStream_t s, bool profile) {
+  d.epoch++;
+  UpdArgs A = make_args(d, w, ud, changed, changed_cap);
+  hipLaunchKernelGGL(k_upd_reset, dim3(std::max(1, std::min(64, (d.levels + 255) / 256))), dim3(256), 0, s, A,
+                     d.levels);
+  if (k > 0)
+    hipLaunchKernelGGL(k_upd_seed, dim3((unsigned)((k + 255) / 256)), dim3(256), 0, s, A, e_dev, w_dev, (i64)k,
+                       d.m);
+  // mode: 0 = auto, 1 = one persistent workgroup, 2 = one launch per height.
+  // One or two changed arcs usually leave a frontier that one workgroup
+  // handles fastest; more spread over thousands of arcs.
+  const bool persistent = mode == 1 || (mode == 0 && k <= d.upd_persistent_max);
+  if (L0 < d.levels) {
+    if (persistent) {
+      if (profile) {
+        ProfScope ps(d, s, kKUpdate);
+        hipLaunchKernelGGL(k_upd_persistent, dim3(1), dim3(1024), 0, s, A, L0, d.levels);
+      } else {
+        hipLaunchKernelGGL(k_upd_persistent, dim3(1), dim3(1024), 0, s, A, L0, d.levels);
+      }
+    } else {
+      for (int32_t L = L0; L < d.levels; L++) {
+        const i64 cap = d.lv[L].end - d.lv[L].begin;
+        const int blocks = (int)std::max<i64>(1, std::min<i64>((cap + 3) / 4, d.upd_max_blocks));
+        if (profile) {
+          ProfScope ps(d, s, kKUpdate);
+          hipLaunchKernelGGL(k_upd_level, dim3(blocks), dim3(256), 0, s, A, L);
+        } else {
+          hipLaunchKernelGGL(k_upd_level, dim3(blocks), dim3(256), 0, s, A, L);
+        }
+      }
+    }
+  }
+  hipLaunchKernelGGL(k_upd_finish, dim3(1), dim3(256), 0, s, A, d.levels);
+}
+
+}  // namespace
 
 extern "C" int nwcch_update(nwcch_dev* h, int32_t* w, int32_t* ud, const int64_t* edge_idx,
                             const int32_t* new_w, int64_t k, int32_t mode, int32_t* changed,
@@ -367,13 +433,7 @@ extern "C" int nwcch_update(nwcch_dev* h, int32_t* w, int32_t* ud, const int64_t
     hipStream_t s = (hipStream_t)stream;
     // validate on the host and find the lowest height that can hold a queued arc
     int32_t L0 = d.levels;
-    for (int64_t i = 0; i < k; i++) {
-      const int64_t e = edge_idx[i];
-      if (e < 0 || e >= d.m) throw std::invalid_argument("edge index out of range");
-      if (new_w[i] == (int32_t)0x80000000) throw std::invalid_argument("INT32_MIN is not a valid weight");
-      const int32_t a = d.e_arc_host[e];
-      if (a >= 0) L0 = std::min(L0, d.arc_lvl_host[a]);
-    }
+    if (const int rc = check_changes(d, edge_idx, new_w, k, &L0)) return rc;
     // the changes travel through pinned host staging, so the caller's arrays
     // may be released as soon as this call returns.  Slots rotate; a slot is
     // reused only after its previous copies have completed, so a new update
@@ -409,38 +469,93 @@ extern "C" int nwcch_update(nwcch_dev* h, int32_t* w, int32_t* ud, const int64_t
       HIP_OK(hipMemcpyAsync(sg.w, sg.hw, k * sizeof(int32_t), hipMemcpyHostToDevice, s));
       HIP_OK(hipEventRecord(sg.ev, s));
     }
-    d.epoch++;
-    UpdArgs A = make_args(d, w, ud, changed, changed_cap);
-    hipLaunchKernelGGL(k_upd_reset, dim3(std::max(1, std::min(64, (d.levels + 255) / 256))), dim3(256), 0, s,
-                       A, d.levels);
-    if (k > 0)
-      hipLaunchKernelGGL(k_upd_seed, dim3((unsigned)((k + 255) / 256)), dim3(256), 0, s, A, sg.e, sg.w, (i64)k,
-                         d.m);
-    // mode: 0 = auto, 1 = one persistent workgroup, 2 = one launch per height.
-    // One or two changed arcs usually leave a frontier that one workgroup
-    // handles fastest; more spread over thousands of arcs.
-    const bool persistent = mode == 1 || (mode == 0 && k <= d.upd_persistent_max);
-    if (L0 < d.levels) {
-      if (persistent) {
-        ProfScope ps(d, s, kKUpdate);
-        hipLaunchKernelGGL(k_upd_persistent, dim3(1), dim3(1024), 0, s, A, L0, d.levels);
-      } else {
-        for (int32_t L = L0; L < d.levels; L++) {
-          const i64 cap = d.lv[L].end - d.lv[L].begin;
-          const int blocks = (int)std::max<i64>(1, std::min<i64>((cap + 3) / 4, d.upd_max_blocks));
-          ProfScope ps(d, s, kKUpdate);
-          hipLaunchKernelGGL(k_upd_level, dim3(blocks), dim3(256), 0, s, A, L);
-        }
-      }
-    }
-    hipLaunchKernelGGL(k_upd_finish, dim3(1), dim3(256), 0, s, A, d.levels);
+    launch_update(d, w, ud, sg.e, sg.w, k, L0, mode, changed, changed_cap, result, s, true);
     HIP_OK(hipMemcpyAsync(result, d.upd_res, 4 * sizeof(long long), hipMemcpyDeviceToDevice, s));
     HIP_OK(hipGetLastError());
-  } catch (const std::invalid_argument& e) {
-    set_error(e.what());
-    return NWCCH_ERR_ARG;
   } catch (const std::exception& e) {
     set_error(e.what());
+    return NWCCH_ERR_RUNTIME;
+  }
+  return NWCCH_OK;
+}
+
+// ---------------------------------------------------- update without allocation
+
+namespace {
+
+// Staging of nwcch_update_noalloc: the edge ids of up to kNoAllocMaxChanges
+// changes, then their new weights.  Every GPU holds its own copy of this array
+// for the life of the process; host code reaches it only through the address
+// hipGetSymbolAddress gives for the current device (the name itself is a host
+// stand-in, not a device address).
+constexpr int64_t kNoAllocMaxChanges = 65536;
+static __device__ __align__(256) unsigned char scratch[kNoAllocMaxChanges * (sizeof(int64_t) + sizeof(int32_t))];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on GPU `device` (looked up once per device,
+// with that device current).  nullptr with the message set on failure.
+void* scratch_on(int device) {
+  if (device < 0 || device >= kMaxDevices) { set_error("device id out of range"); return nullptr; }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[device]) return g_scratch_ptr[device];
+  int prev = 0;
+  if (hipGetDevice(&prev) != hipSuccess) { set_error("hipGetDevice failed"); return nullptr; }
+  if (hipSetDevice(device) != hipSuccess) { set_error("hipSetDevice failed"); return nullptr; }
+  void* ptr = nullptr;
+  const hipError_t rc = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  (void)hipSetDevice(prev);
+  if (rc != hipSuccess || !ptr) {
+    set_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(rc));
+    return nullptr;
+  }
+  g_scratch_ptr[device] = ptr;
+  return ptr;
+}
+
+}  // namespace
+
+int nwcch_update_noalloc_setup(int device) { return scratch_on(device) ? NWCCH_OK : NWCCH_ERR_RUNTIME; }
+
+extern "C" int nwcch_update_noalloc(nwcch_dev* h, int32_t* w, int32_t* ud, const int64_t* edge_idx,
+                                    const int32_t* new_w, int64_t k, int32_t mode, int32_t* changed,
+                                    int64_t changed_cap, int64_t* result, void* stream) {
+  if (!h || !w || !ud || (k > 0 && (!edge_idx || !new_w)) || k < 0 || !result || changed_cap < 0) {
+    set_error("null argument");
+    return NWCCH_ERR_ARG;
+  }
+  if (k > kNoAllocMaxChanges) {
+    set_error("nwcch_update_noalloc: at most 65536 changes per call");
+    return NWCCH_ERR_CAPACITY;
+  }
+  DevTopo& d = h->d;
+  if (!d.has_update) { set_error("nwcch_update_prepare was not called"); return NWCCH_ERR_ARG; }
+  int32_t L0 = d.levels;
+  if (const int rc = check_changes(d, edge_idx, new_w, k, &L0)) return rc;
+  unsigned char* base = static_cast<unsigned char*>(scratch_on(d.device));
+  if (!base) return NWCCH_ERR_RUNTIME;
+  i64* e_dev = reinterpret_cast<i64*>(base);
+  int32_t* w_dev = reinterpret_cast<int32_t*>(base + kNoAllocMaxChanges * sizeof(int64_t));
+  hipStream_t s = (hipStream_t)stream;
+  // the changes are copied straight from the caller's host arrays into the
+  // static staging, in stream order
+  if (k > 0) {
+    if (hipMemcpyAsync(e_dev, edge_idx, k * sizeof(int64_t), hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(w_dev, new_w, k * sizeof(int32_t), hipMemcpyHostToDevice, s) != hipSuccess) {
+      set_error("nwcch_update_noalloc: copying the changes failed");
+      return NWCCH_ERR_RUNTIME;
+    }
+  }
+  launch_update(d, w, ud, e_dev, w_dev, k, L0, mode, changed, changed_cap, result, s, false);
+  if (hipMemcpyAsync(result, d.upd_res, 4 * sizeof(long long), hipMemcpyDeviceToDevice, s) != hipSuccess) {
+    set_error("nwcch_update_noalloc: copying the result failed");
+    return NWCCH_ERR_RUNTIME;
+  }
+  const hipError_t le = hipGetLastError();
+  if (le != hipSuccess) {
+    set_error(std::string("nwcch_update_noalloc: ") + hipGetErrorString(le));
     return NWCCH_ERR_RUNTIME;
   }
   return NWCCH_OK;

@@ -59,6 +59,7 @@ def lib():
             "nwcch_update_prepare": ([vp, vp], C.c_int),
             "nwcch_update": ([vp, p, p, p, p, i64, i32, p, i64, p, vp], C.c_int),
             "nwcch_update_changed": ([vp, p, i64, vp], C.c_int),
+            "nwcch_update_noalloc": ([vp, p, p, p, p, i64, i32, p, i64, p, vp], C.c_int),
             "nwcch_profile_end": ([vp, p, p, i64, p], C.c_int),
             "nwcch_query_strided": ([vp, p, i64, p, p, i64, p, p, p, vp], C.c_int),
             "nwcch_query_multi": ([vp, p, i32, p, p, p, i64, p, p, vp], C.c_int),
@@ -114,7 +115,7 @@ def exported_symbols():
             "nwcch_customize_batch", "nwcch_negcycle_scan_batch", "nwcch_batch_slab_width",
             "nwcch_set_tuning", "nwcch_get_tuning", "nwcch_query_multi", "nwcch_potential",
             "nwcch_query_path", "nwcch_topo_create_ex", "nwcch_dev_export_triangles",
-            "nwcch_dev_export_sides", "nwcch_topo_from_csr_ex"]
+            "nwcch_dev_export_sides", "nwcch_topo_from_csr_ex", "nwcch_update_noalloc"]
 
 
 # ------------------------------------------------------------ host topology
