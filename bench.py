@@ -1,31 +1,27 @@
 """Benchmark of the metric-dependent CCH path on one MI355X.
 
-Headline workload (`value`): the road-like graph of 264,000 clustered points
-(seed 1; Delaunay thinned to mean out-degree 2.8, 5% one-way, largest SCC),
-int32 energy weights, inertial-flow order (host, untimed).  One step = full
-customization of one metric + the negative-cycle scan (the plain one-metric
-form), inputs resident in HBM; value = lower triangles / step time.
+Headline workload (`value`, the default run): config 4, the full-scale
+road-like graph -- 23.9 M clustered points (seed 4), Delaunay thinned to a mean
+out-degree of 2.44, 5% one-way, largest SCC, continental fractal heights
+0-3,000 m, int32 energy weights, inertial-flow order (host, untimed).  One step
+= K = 8 metrics customized in one level sweep plus the per-metric
+negative-cycle scans, inputs resident in HBM, replayed from one HIP graph;
+value = K x lower triangles / step time.
 
-The same run also reports, as extra fields:
-  * ET queries on that metric: 100,000 random s-t pairs in one launch (with
-    their algorithmic bytes against the HBM peak), and single-query launches
-    (p50/p99 latency);
-  * partial updates of 1, 10 and 1,000 arcs, the CCH potential, 2,000 paths
-    with their vertices, and the metric-independent index built on the host
-    and on the GPU;
-  * the serving loop on the 1.1M-point graph (16 resident metrics, Poisson
-    queries at 0.2-2 M/s interleaved with updates), p50/p99;
-  * the batched form: 64 metrics (slopes 5-40, own height fields, one with an
-    injected 3-arc cycle of total -1) customized in one level sweep on the
-    321,000-point graph (seed 2), against 64 plain single-metric customizations;
-  * parity of every GPU result checked bit-for-bit against the CPU oracle;
-  * the CPU baseline: the reference's own C++ customization and ET queries
-    (cpp/cch.cpp, compiled from its sources) on the host, or the oracle port
-    where the reference was not built.
+The same run also reports, as extra fields: the plain one-metric
+customization and scan; 1 M random ET queries in one launch (three kernel
+forms) and single-query launches (p50/p99); per-launch roofline figures of
+the level kernels; the CPU baseline -- the reference's own C++ (cpp/cch.cpp,
+compiled from its sources) on the host: serial and OpenMP customization,
+Query::run on one core and on all cores; and parity: every arc of one metric
+against the reference's customization, every batched metric against its
+single-metric run, 100 queries against Johnson + Dijkstra, a query sample
+against the reference's Query::run.
 
-Other modes: `--full-scale` runs the 23.4 M-vertex road graph (customization,
-batched sweep, queries, checks against the reference's C++); `--prep` times
-everything after the order for `--config` on the host and on the GPU.
+Other modes: `--config 1` (plain one-metric workload on the 264,000-point
+graph, with updates, potential, paths, and the batched 64-metric run on
+config 2 and the serving run on config 3 after it); `--prep` times everything
+after the order for `--config` on the host and on the GPU.
 
 Prints ONE JSON line.
 """
@@ -65,6 +61,26 @@ def level_bytes(mp, T):
     pair read and its final pair written (16 B).  Schedule records and per-
     triangle arc ids are layout overhead and not counted."""
     return 8 * T + 16 * mp
+
+
+def cust_bytes_8d(n, m, mp, T, K):
+    """Algorithmic bytes of one customization of K metrics: a 4-byte c-index
+    per lower triangle, first and head, per metric every (up, down) pair read
+    and written once and the weight vector, the arc map (4 + 1 bytes per
+    original arc)."""
+    return 4 * T + 4 * (n + 1) + 4 * mp + K * (16 * mp + 4 * m) + 5 * m
+
+
+def level_bytes_8d(n, mp, T, K):
+    """The part of cust_bytes_8d the level launches move: the c-index stream,
+    first/head and the K (up, down) pairs of every arc read and written."""
+    return 4 * T + 4 * (n + 1) + 4 * mp + 16 * K * mp
+
+
+def layout_level_bytes(mp, T, K):
+    """The same in this build's layout: an (a, b) operand pair (8 B) per
+    triangle and a 16-byte schedule record per arc instead of the c-index."""
+    return 8 * T + 16 * mp + 16 * K * mp
 
 
 def read_traffic(kernel, algorithmic):
@@ -689,203 +705,331 @@ def serving(args, out):
     del srv, dt
 
 
-def full_scale(args, out):
-    """Config 4: the 23.9M-point road graph (seed 4, continental fractal heights
-    0-3,000 m).  One metric and K = 8 metrics customized on the GPU, the
-    negative-cycle scans, 1M random ET queries; every shortcut value checked
-    against the reference's own C++ customization (cpp/cch.cpp, compiled from
-    its sources) and a query sample against Johnson + Dijkstra; the reference's
-    serial and OpenMP customization timed on the host beside it."""
+def config4_inputs(args, progress):
+    """The config-4 graph, its order and the K metrics.  While the host orders
+    the graph (native, threads), the K weight vectors are drawn and, as the
+    checker of the query sample, Johnson's potentials and Dijkstra's answers
+    are computed on the original graph (neither needs the order).  With
+    --cache FILE everything is read from / written to one .npz."""
     import concurrent.futures as cf
 
     import numpy as np
+
+    from nwcch import _lib, graphs
+    if args.cache and Path(args.cache).exists():
+        t0 = time.perf_counter()
+        z = np.load(args.cache)
+        g = graphs.RoadGraph(int(z["n"]), z["off"], z["tgt"], z["w"], z["x"], z["y"], None, "c4")
+        g.extra["box"] = float(z["box"])
+        log(f"config {args.graph}: graph, order and metrics read from {args.cache} in {time.perf_counter() - t0:.1f}s")
+        chk = (z["chk_s"], z["chk_t"], z["chk_d"]) if "chk_d" in z else None
+        return g, z["rank"], z["W"], chk, {"cached": True}
+    host = {}
+    t0 = time.perf_counter()
+    g = graphs.make_config(args.graph, progress=progress)
+    host["gen_s"] = time.perf_counter() - t0
+    log(f"config {args.graph}: graph n={g.n} m={g.m} generated in {host['gen_s']:.1f}s")
+    qs_h, qt_h = graphs.random_pairs(g.n, args.full_check, seed=43)
+
+    def checker():
+        # Johnson's virtual-source potentials, then one Dijkstra per pair
+        from oracle import oracle as O
+        import queue
+        t1 = time.perf_counter()
+        p, neg = O.johnson(g.n, g.off, g.tgt, g.w)
+        free = queue.Queue()        # one searcher (own scratch arrays) per running task
+        for _ in range(4):
+            free.put(O.Dijkstra(g.n, g.off, g.tgt, g.w, p))
+
+        def run(i):
+            dj = free.get()
+            try:
+                return dj(int(qs_h[i]), int(qt_h[i]))
+            finally:
+                free.put(dj)
+        with cf.ThreadPoolExecutor(4) as ex:
+            want = np.array(list(ex.map(run, range(len(qs_h)))), np.int64)
+        want = np.where(want >= O.INF, O.INF, want)
+        log(f"config {args.graph}: Johnson + Dijkstra answers for {len(qs_h)} pairs in {time.perf_counter() - t1:.1f}s")
+        return want, bool(neg)
+
+    with cf.ThreadPoolExecutor(2) as side:
+        fut_w = side.submit(graphs.metric_family, g, args.metrics, 4, graphs.CONFIGS[args.graph].get("hmax", 3000))
+        fut_c = side.submit(checker) if not args.no_check else None
+        t0 = time.perf_counter()
+        rank = _lib.order(g.n, g.off, g.tgt, g.x, g.y, 64)
+        host["order_s"] = time.perf_counter() - t0
+        log(f"config {args.graph}: nested dissection {host['order_s']:.1f}s")
+        t0 = time.perf_counter()
+        W, _ = fut_w.result()
+        chk = None
+        if fut_c is not None:
+            want, _neg = fut_c.result()
+            chk = (qs_h, qt_h, want)
+        host["side_wait_s"] = time.perf_counter() - t0
+    if args.cache:
+        extra = {} if chk is None else {"chk_s": chk[0], "chk_t": chk[1], "chk_d": chk[2]}
+        np.savez(args.cache, n=g.n, off=g.off, tgt=g.tgt, w=g.w, x=g.x, y=g.y, box=g.extra["box"], rank=rank,
+                 W=W, **extra)
+        log(f"config {args.graph}: inputs written to {args.cache}")
+    return g, rank, W, chk, host
+
+
+def headline(args, out):
+    """Config 4, the full-scale road graph (23.9 M points, seed 4): K = 8
+    metrics customized in one level sweep plus their negative-cycle scans per
+    step (the timed work), the plain one-metric customization beside it,
+    1 M random ET queries, the reference's own C++ on the host as the CPU
+    baseline, and parity: every arc of one metric against the reference's
+    customization, every batched metric against its single-metric run, and
+    100 queries against Johnson + Dijkstra."""
+    import numpy as np
     import torch
 
-    from nwcch import graphs
-    from nwcch.device import DeviceTopology
+    from nwcch import _lib, graphs
+    from nwcch.device import DeviceTopology, StepGraph
 
     os.environ["NWCCH_PROGRESS"] = "1"   # a line per big nested-dissection cell and preprocessing phase
-    # the triangle index is counted and filled on the GPU
-    g, rank, topo, host = prepare(4, progress=lambda msg: log(f"config 4: {msg}"), device_index=0)
+    g, rank, W, chk, host = config4_inputs(args, lambda msg: log(f"config {args.graph}: {msg}"))
+    K = W.shape[0]
+    t0 = time.perf_counter()
+    topo = _lib.Topology.from_graph(g.n, g.off, g.tgt, rank, device_index=0, device_symbolic=0)
+    host["topology_s"] = time.perf_counter() - t0
     t0 = time.perf_counter()
     dt = DeviceTopology(topo, 0)
     torch.cuda.synchronize()
     host["dev_create_s"] = time.perf_counter() - t0
-    log(f"config 4: device topology created in {host['dev_create_s']:.1f} s (triangle lists filled on the GPU)")
     free_b, total_b = torch.cuda.mem_get_info()
-    log(f"device topology uploaded ({(total_b - free_b) / 1e9:.1f} GB of HBM in use)")
-    w = dt.weights(g.w)
+    log(f"config {args.graph}: n={g.n} m={g.m} m+={topo.mp} T={topo.triangles} depth={topo.depth} levels={topo.levels}; "
+        f"topology {host['topology_s']:.1f}s, device {host['dev_create_s']:.1f}s, "
+        f"{(total_b - free_b) / 1e9:.1f} GB of HBM in use")
+
+    # ---- the timed step: K metrics in one sweep + per-metric scans, one HIP graph
+    Wd = dt.weights_batch(W)
+    UD = torch.empty(dt.batch_shape(K), dtype=torch.int32, device=dt.dev)
+    sc = torch.empty((2, K), dtype=torch.int64, device=dt.dev)
+
+    def step():
+        dt.customize_batch(Wd, out=UD)
+        dt.negcycle_scan_batch(UD, K, out=sc)
+
+    step()
+    dt.check_status()
+    graph = StepGraph(step)
+    for _ in range(args.warmup):
+        graph.replay()
+    dt.check_status()
+    t_step = timed(graph.replay, args.steps)
+    t_eager = timed(step, 2)
+    dt.check_status()
+    flagged = [int(k) for k in np.nonzero(sc[0].cpu().numpy() > 0)[0]]
+    # device time of the batched customization alone (graph replay, HIP events)
+    cgraph = StepGraph(lambda: dt.customize_batch(Wd, out=UD))
+    cgraph.replay()
+    bms = statistics.median(event_ms(cgraph.replay, max(3, args.steps // 4)))
+    del cgraph
+    bscan_ms = statistics.median(event_ms(lambda: dt.negcycle_scan_batch(UD, K, out=sc), 5))
+    # per-launch durations: HIP events the library records around each launch
+    dt.profile_begin(topo.levels + 8)
+    dt.customize_batch(Wd, out=UD)
+    kid, kms = dt.profile_end()
+    n_lv = int((kid == 4).sum())
+    bbase_ms = float(kms[kid == 5].astype(np.float64).sum())
+    lv_avg_us = 1000.0 * (bms - bbase_ms) / n_lv
+    B8d_lv = level_bytes_8d(g.n, topo.mp, topo.triangles, K)
+    B8d = cust_bytes_8d(g.n, g.m, topo.mp, topo.triangles, K)
+    achieved = (B8d_lv / n_lv) / (lv_avg_us * 1e-6) / 1e9
+    traffic = read_traffic("k_customize_batch_level_road23.4m", B8d_lv / n_lv)
+    log(f"config {args.graph} K={K}: step {1e3 * t_step:.2f} ms ({K * topo.triangles / t_step:.3g} tri/s); customization "
+        f"{bms:.2f} ms = base {bbase_ms:.2f} ms + {n_lv} level launches of {lv_avg_us:.1f} us; "
+        f"{achieved:.0f} GB/s by the algorithmic bytes; flagged {flagged}")
+
+    # ---- the plain one-metric form beside it
+    w0 = dt.weights(W[0])
     ud = torch.empty((topo.mp, 2), dtype=torch.int32, device=dt.dev)
     scan = torch.empty(2, dtype=torch.int64, device=dt.dev)
-    dt.customize(w, out=ud)
-    dt.check_status()
-    cust_ms = event_ms(lambda: dt.customize(w, out=ud), args.steps)
-    scan_ms = event_ms(lambda: dt.negcycle_scan(ud, out=scan), args.steps)
-    dt.profile_begin(topo.levels + 2)
-    dt.customize(w, out=ud)
-    kid, kms = dt.profile_end()
-    n_lv = int((kid == 0).sum())
-    base_ms = float(np.median(kms[kid == 5]))
-    cmed = statistics.median(cust_ms)
-    B_lv = level_bytes(topo.mp, topo.triangles)
-    lv_avg_us = 1000.0 * (cmed - base_ms) / n_lv
-    achieved = (B_lv / n_lv) / (lv_avg_us * 1e-6) / 1e9
-    log(f"config 4 customize {cmed:.1f} ms ({topo.triangles / (cmed / 1e3):.3g} tri/s), level launches "
-        f"{n_lv}, {achieved:.0f} GB/s algorithmic; scan {statistics.median(scan_ms):.2f} ms")
-    # K = 8 metrics in one sweep
-    K = 8
-    W, _ = graphs.metric_family(g, K, seed=4, hmax=3000)
-    Wd = dt.weights_batch(W)
-    del W
-    UD = torch.empty(dt.batch_shape(K, 0), dtype=torch.int32, device=dt.dev)
-    sc = torch.empty((2, K), dtype=torch.int64, device=dt.dev)
-    dt.customize_batch(Wd, out=UD)
-    dt.check_status()
-    bms = statistics.median(event_ms(lambda: dt.customize_batch(Wd, out=UD), max(2, args.steps // 2)))
-    dt.negcycle_scan_batch(UD, K, out=sc)
-    flagged = [int(k) for k in np.nonzero(sc[0].cpu().numpy() > 0)[0]]
-    log(f"config 4 K={K}: {bms:.1f} ms ({K * topo.triangles / (bms / 1e3):.3g} tri/s), flagged {flagged}")
-    del UD, Wd
-    # 1M random ET queries on metric 0
-    nq = args.full_queries
-    qs_h, qt_h = graphs.random_pairs(g.n, nq, seed=41)
-    qs = torch.as_tensor(rank[qs_h]).to(dt.dev)
-    qt = torch.as_tensor(rank[qt_h]).to(dt.dev)
-    dist = torch.empty(nq, dtype=torch.int32, device=dt.dev)
-    order = dt.chain_order(qs, qt)
-    # the query kernels on this very deep hierarchy: the throughput kernel for
-    # the big launch, then each latency kernel for the big launch and alone
-    forms = {}
-    for name, chunk, deep in (("chain_walk_throughput", 0, 0), ("chunked_latency", 1, 1 << 30),
-                              ("chain_walk_latency", 0, 1 << 30)):
-        dt.set_tuning("query_chunk", chunk)
-        dt.set_tuning("query_deep_max", deep)
-        dt.query(ud, qs, qt, want_scanned=False, dist=dist, order=order)
-        bms_q = statistics.median(event_ms(lambda: dt.query(ud, qs, qt, want_scanned=False, dist=dist, order=order),
-                                           2))
-        one = [event_ms(lambda i=i: dt.query(ud, qs[i:i + 1], qt[i:i + 1], want_scanned=False), 1)[0]
-               for i in range(30)]
-        forms[name] = {"batch_ms": bms_q, "queries_per_s": nq / (bms_q / 1e3),
-                       "single_p50_ms": float(np.percentile(one, 50))}
-        log(f"config 4 queries, {name}: {nq} in {bms_q:.1f} ms; single p50 {np.percentile(one, 50):.2f} ms")
-    best_b = min(forms, key=lambda k: forms[k]["batch_ms"])
-    best_s = min(forms, key=lambda k: forms[k]["single_p50_ms"])
-    dt.set_tuning("query_chunk", 1 if best_b == "chunked_latency" else 0)
-    dt.set_tuning("query_deep_max", 0 if best_b == "chain_walk_throughput" else 1 << 30)
-    dt.query(ud, qs, qt, want_scanned=False, dist=dist, order=order)
-    qms = forms[best_b]["batch_ms"]
-    dt.set_tuning("query_chunk", 1 if best_s == "chunked_latency" else 0)
-    dt.set_tuning("query_deep_max", 0 if best_s == "chain_walk_throughput" else 1 << 30)
-    single = [event_ms(lambda i=i: dt.query(ud, qs[i:i + 1], qt[i:i + 1], want_scanned=False), 1)[0]
-              for i in range(50)]
-    dt.check_status()
-    log(f"config 4 queries: {nq} in {qms:.1f} ms ({nq / (qms / 1e3):.3g} q/s); single p50 "
-        f"{np.percentile(single, 50):.2f} ms")
-    res = {
-        "workload": f"config 4: n={g.n} m={g.m} m+={topo.mp} T={topo.triangles} ET depth={topo.depth}",
-        "n": g.n, "m": g.m, "m_plus": topo.mp, "triangles": topo.triangles, "depth": topo.depth,
-        "levels": topo.levels, "negative_arc_fraction": round(g.negative_fraction(), 4), "host": host,
-        "customize_ms": cmed, "triangles_per_s": topo.triangles / (cmed / 1e3),
-        "roofline": {"kernel": "level sweep", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "algorithmic_bytes_per_launch": B_lv / n_lv,
-                     "avg_launch_us": lv_avg_us, "level_launches": n_lv},
-        "scan_ms": statistics.median(scan_ms),
-        "batched": {"metrics": K, "ms": bms, "triangles_per_s": K * topo.triangles / (bms / 1e3),
-                    "flagged_metrics": flagged},
-        "queries": {"count": nq, "ms": qms, "queries_per_s": nq / (qms / 1e3), "batch_kernel": best_b,
-                    "single_kernel": best_s, "single_p50_ms": float(np.percentile(single, 50)),
-                    "single_p99_ms": float(np.percentile(single, 99)), "forms": forms},
-    }
-    if not args.no_check or not args.no_cpu:
-        from oracle import oracle as O
-        got = ud.cpu().numpy()
-        d_gpu = dist[:args.full_check].cpu().numpy()
-        del ud
-        torch.cuda.empty_cache()
-        if O.have_ref():
-            t0 = time.perf_counter()
-            ref = O.Reference(g.n, g.off, g.tgt, rank)
-            t_build = time.perf_counter() - t0
-            t0 = time.perf_counter()
-            up, dn = ref.customize(g.w)
-            t_ser = time.perf_counter() - t0
-            ok = True
-            for col, r in ((0, up), (1, dn)):
-                r = np.where(np.isinf(r), O.INF, r).astype(np.int64)
-                ok &= bool(np.array_equal(got[:, col].astype(np.int64), r))
-            del up, dn
-            thr = os.cpu_count() or 1
-            t0 = time.perf_counter()
-            ref.customize(g.w, threads=thr)
-            t_par = time.perf_counter() - t0
-            res["parity_vs_reference_customization"] = ok
-            res["cpu_baseline"] = [
-                {"value": topo.triangles / t_ser, "unit": "triangles/s", "cores": 1, "kind": "reference",
-                 "sample": "cpp/cch.cpp CCH::customize (serial, float64), one full customization", "ms": 1e3 * t_ser},
-                {"value": topo.triangles / t_par, "unit": "triangles/s", "cores": thr, "kind": "reference",
-                 "sample": "cpp/cch.cpp CCH::customize_parallel (OpenMP, level buckets), one full customization",
-                 "ms": 1e3 * t_par}]
-            res["reference_build_s"] = t_build
-            log(f"config 4 reference: build {t_build:.1f}s, serial {t_ser:.1f}s, {thr} threads {t_par:.1f}s; "
-                f"parity {ok}")
-            del ref
-        if not args.no_check:
-            t0 = time.perf_counter()
-            p, neg = O.johnson(g.n, g.off, g.tgt, g.w)
-            t_j = time.perf_counter() - t0
-            nchk = args.full_check
-            workers = min(8, os.cpu_count() or 1)
-            import queue
-            free = queue.Queue()   # one searcher (with its own scratch arrays) per running task
-            for _ in range(workers):
-                free.put(O.Dijkstra(g.n, g.off, g.tgt, g.w, p))
 
-            def run(i):
-                dj = free.get()
-                try:
-                    return dj(int(qs_h[i]), int(qt_h[i]))
-                finally:
-                    free.put(dj)
-            t0 = time.perf_counter()
-            with cf.ThreadPoolExecutor(workers) as ex:
-                want = np.array(list(ex.map(run, range(nchk), chunksize=1)), np.int64)
-            t_d = time.perf_counter() - t0
-            want = np.where(want >= O.INF, O.INF, want)
-            okq = bool(np.array_equal(d_gpu.astype(np.int64), want))
-            if not okq:
-                bad = np.nonzero(d_gpu.astype(np.int64) != want)[0]
-                log(f"config 4 query mismatches at {bad[:10].tolist()}: gpu {d_gpu[bad[:5]].tolist()} "
-                    f"dijkstra {want[bad[:5]].tolist()}")
-            res["query_check"] = {"pairs": nchk, "agree_with_johnson_dijkstra": okq, "johnson_s": t_j,
-                                  "dijkstra_s": t_d, "negative_cycle": neg}
-            log(f"config 4 queries vs Johnson+Dijkstra on {nchk}: {okq} (Johnson {t_j:.1f}s, Dijkstra {t_d:.1f}s)")
-    # the metric-independent index: the side lists on the GPU, then the same
-    # topology built with the host's triangle index for the time beside it and
-    # an entry-for-entry comparison of the triangle lists
-    t0 = time.perf_counter()
-    dt.prepare_update()
-    torch.cuda.synchronize()
-    t_side = time.perf_counter() - t0
-    prep = {"device_index": {"topology_s": host["topology_s"], "dev_create_s": host["dev_create_s"],
-                             "update_index_s": t_side}}
-    log(f"config 4: update side lists on the GPU in {t_side:.1f} s")
+    def step1():
+        dt.customize(w0, out=ud)
+        dt.negcycle_scan(ud, out=scan)
+
+    step1()
+    g1 = StepGraph(step1)
+    g1.replay()
+    t_plain = timed(g1.replay, max(3, args.steps // 2))
+    cust1_ms = statistics.median(event_ms(lambda: dt.customize(w0, out=ud), max(3, args.steps // 4)))
+    scan1_ms = statistics.median(event_ms(lambda: dt.negcycle_scan(ud, out=scan), 5))
+    dt.profile_begin(topo.levels + 8)
+    dt.customize(w0, out=ud)
+    kid1, kms1 = dt.profile_end()
+    n_lv1 = int((kid1 == 0).sum())
+    base1_ms = float(kms1[kid1 == 5].astype(np.float64).sum())
+    lv1_us = 1000.0 * (cust1_ms - base1_ms) / n_lv1
+    B8d_lv1 = level_bytes_8d(g.n, topo.mp, topo.triangles, 1)
+    ach1 = (B8d_lv1 / n_lv1) / (lv1_us * 1e-6) / 1e9
+    dt.check_status()
+    log(f"config {args.graph} one metric: step {1e3 * t_plain:.2f} ms, customization {cust1_ms:.2f} ms "
+        f"({topo.triangles / (cust1_ms / 1e3):.3g} tri/s), scan {scan1_ms:.3f} ms")
+
+    # ---- ET queries on metric 0: a batch of 1 M in one launch, and one query per launch
+    queries = None
+    if args.full_queries:
+        nq = args.full_queries
+        qs_h, qt_h = graphs.random_pairs(g.n, nq, seed=41)
+        qs = torch.as_tensor(rank[qs_h]).to(dt.dev)
+        qt = torch.as_tensor(rank[qt_h]).to(dt.dev)
+        dist = torch.empty(nq, dtype=torch.int32, device=dt.dev)
+        order = dt.chain_order(qs, qt)
+        forms = {}
+        for name, chunk, deep in (("chain_walk_throughput", 0, 0), ("chunked_latency", 1, 1 << 30),
+                                  ("chain_walk_latency", 0, 1 << 30)):
+            dt.set_tuning("query_chunk", chunk)
+            dt.set_tuning("query_deep_max", deep)
+            dt.query(ud, qs, qt, want_scanned=False, dist=dist, order=order)
+            ms = statistics.median(event_ms(lambda: dt.query(ud, qs, qt, want_scanned=False, dist=dist,
+                                                             order=order), 2))
+            one = [event_ms(lambda i=i: dt.query(ud, qs[i:i + 1], qt[i:i + 1], want_scanned=False), 1)[0]
+                   for i in range(30)]
+            forms[name] = {"batch_ms": ms, "queries_per_s": nq / (ms / 1e3), "single_p50_ms": float(np.median(one))}
+            log(f"config {args.graph} queries, {name}: {nq} in {ms:.1f} ms; single p50 {np.median(one):.2f} ms")
+        best_b = min(forms, key=lambda k: forms[k]["batch_ms"])
+        best_s = min(forms, key=lambda k: forms[k]["single_p50_ms"])
+
+        def use(name):
+            dt.set_tuning("query_chunk", 1 if name == "chunked_latency" else 0)
+            dt.set_tuning("query_deep_max", 0 if name == "chain_walk_throughput" else 1 << 30)
+        use(best_b)
+        dist_full, scanned = dt.query(ud, qs, qt, order=order)
+        qms = forms[best_b]["batch_ms"]
+        vdep = topo.export()["depth"].astype(np.int64)
+        chain = vdep[rank[qs_h]] + vdep[rank[qt_h]] + 2
+        qbytes = 8 * int(scanned.to(torch.int64).sum().item()) + 12 * int(chain.sum()) + 12 * nq
+        use(best_s)
+        single = [event_ms(lambda i=i: dt.query(ud, qs[i:i + 1], qt[i:i + 1], want_scanned=False), 1)[0]
+                  for i in range(args.single_full)]
+        dt.check_status()
+        queries = {"count": nq, "batch_ms": qms, "queries_per_s": nq / (qms / 1e3), "batch_kernel": best_b,
+                   "roofline": {"bound": "hbm", "achieved": qbytes / (qms / 1e3) / 1e9, "peak": HBM_PEAK_GBS,
+                                "unit": "GB/s", "frac": qbytes / (qms / 1e3) / 1e9 / HBM_PEAK_GBS,
+                                "bytes_per_query": qbytes / nq,
+                                "what": "8 B per scanned arc + 12 B per chain vertex + 12 B per query"},
+                   "single_kernel": best_s, "single_launches": len(single),
+                   "single_p50_ms": float(np.percentile(single, 50)),
+                   "single_p99_ms": float(np.percentile(single, 99)), "forms": forms}
+        log(f"config {args.graph} queries: {nq} in {qms:.1f} ms ({nq / (qms / 1e3):.3g} q/s); single p50 "
+            f"{np.percentile(single, 50):.2f} ms p99 {np.percentile(single, 99):.2f} ms")
+
+    # ---- parity (the checker runs outside every timed region)
+    parity = {}
     if not args.no_check:
-        from nwcch import _lib
+        # every batched metric equals its single-metric customization
+        same_b = True
+        for k in range(K):
+            dt.customize(dt.weights(W[k]), out=ud)
+            same_b &= bool(torch.equal(dt.metric_view(UD, k), ud))
+        parity["batched_equals_single"] = same_b
+        if chk is not None:
+            # the checker's answers are on the graph's own metric
+            cs, ct, want = chk
+            dt.customize(dt.weights(g.w), out=ud)
+            d, _ = dt.query(ud, rank[cs], rank[ct], want_scanned=False)
+            parity["queries_vs_johnson_dijkstra"] = {
+                "pairs": len(cs), "equal": bool(np.array_equal(d.cpu().numpy().astype(np.int64), want))}
+        dt.customize(w0, out=ud)
+        dt.check_status()
+    got = ud.cpu().numpy()
+    cpu = None
+    from oracle import oracle as O
+    if (not args.no_cpu or not args.no_check) and O.have_ref():
         t0 = time.perf_counter()
-        th = _lib.Topology.from_graph(g.n, g.off, g.tgt, rank)
-        t_host = time.perf_counter() - t0
-        sh, oh, tab = th.export_triangles()
-        sd, od, _ = topo.export_triangles(with_list=False)
-        same = bool(np.array_equal(sh, sd) and np.array_equal(oh, od))
-        del sh, oh, sd, od
-        th.close()
-        same &= bool(np.array_equal(dt.export_triangles(), tab))
-        del tab
-        prep["host_index"] = {"topology_s": t_host}
-        prep["identical"] = same
-        log(f"config 4: host-built index {t_host:.1f} s; device-built lists identical: {same}")
-    res["preprocessing"] = prep
-    out["full_scale"] = res
+        ref = O.Reference(g.n, g.off, g.tgt, rank)
+        t_build = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        up, dn = ref.customize(W[0])
+        t_ser = time.perf_counter() - t0
+        ok = True
+        for col, r in ((0, up), (1, dn)):
+            ok &= bool(np.array_equal(got[:, col].astype(np.int64), O.float_to_i32(r).astype(np.int64)))
+        parity["customization_vs_reference_cpp"] = {"arcs": topo.mp, "equal": ok}
+        log(f"config {args.graph} reference C++: structure {t_build:.1f}s, serial customization {t_ser:.2f}s; "
+            f"all {topo.mp} arcs equal: {ok}")
+        if not args.no_cpu:
+            thr = min(16, os.cpu_count() or 1)
+            t0 = time.perf_counter()
+            ref.customize(W[0], threads=thr)
+            t_par = time.perf_counter() - t0
+            nqc = min(args.cpu_queries, args.full_queries) if args.full_queries else 0
+            qres = None
+            if nqc:
+                ref.query(up, dn, qs_h[:50], qt_h[:50])   # warm-up
+                t0 = time.perf_counter()
+                dq = ref.query(up, dn, qs_h[:nqc], qt_h[:nqc])
+                t_q1 = time.perf_counter() - t0
+                t0 = time.perf_counter()
+                dqp = ref.query_par(up, dn, qs_h[:nqc * 4], qt_h[:nqc * 4], thr)
+                t_qp = time.perf_counter() - t0
+                eq = bool(np.array_equal(O.float_to_i32(dq), dist_full[:nqc].cpu().numpy()))
+                eqp = bool(np.array_equal(O.float_to_i32(dqp), dist_full[:nqc * 4].cpu().numpy()))
+                parity["queries_vs_reference_cpp"] = {"pairs": nqc * 4, "equal": eq and eqp}
+                qres = {"serial": {"value": nqc / t_q1, "unit": "queries/s", "cores": 1, "kind": "reference",
+                                   "sample": f"cpp/cch.cpp Query::run, {nqc} random pairs of the GPU batch"},
+                        "parallel": {"value": 4 * nqc / t_qp, "unit": "queries/s", "cores": thr,
+                                     "kind": "reference",
+                                     "sample": f"Query::run on {thr} OpenMP threads (one Query each), "
+                                               f"{4 * nqc} pairs"}}
+                log(f"config {args.graph} reference queries: {nqc / t_q1:.0f} q/s on 1 core, {4 * nqc / t_qp:.0f} q/s on "
+                    f"{thr}; equal to the GPU's: {eq and eqp}")
+            cpu = {"value": topo.triangles / t_ser, "unit": "triangles/s", "cores": 1, "kind": "reference",
+                   "sample": f"cpp/cch.cpp CCH::customize (serial, float64) of metric 0 of the same graph and "
+                             f"order, one full customization ({t_ser:.2f} s)",
+                   "ms": 1e3 * t_ser, "cpu_model": cpu_model(),
+                   "parallel": {"value": topo.triangles / t_par, "cores": thr, "ms": 1e3 * t_par,
+                                "sample": "cpp/cch.cpp CCH::customize_parallel (OpenMP, level buckets), one "
+                                          "full customization"},
+                   "queries": qres, "reference_structure_s": t_build}
+            log(f"config {args.graph} reference C++ on {thr} threads: {t_par:.2f}s")
+        del ref, up, dn
+    if parity:
+        log(f"config {args.graph} parity: {parity}")
+
+    out.update({
+        "value": K * topo.triangles / t_step,
+        "ms_per_step": 1000.0 * t_step,
+        "config": {"workload": f"road-like graph, config {args.graph}: {graphs.CONFIGS[args.graph].get('n_points', 0):,} "
+                               f"clustered points, seed {graphs.CONFIGS[args.graph].get('seed')}, n={g.n} m={g.m} "
+                               f"m+={topo.mp} T={topo.triangles} ET depth={topo.depth}; {K} metrics customized "
+                               f"in one level sweep + per-metric negative-cycle scans per step",
+                   "n": g.n, "m": g.m, "m_plus": topo.mp, "triangles": topo.triangles, "depth": topo.depth,
+                   "levels": topo.levels, "metrics": K, "negative_arc_fraction": round(g.negative_fraction(), 4)},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic["bytes_per_launch"] if traffic else None,
+                     "traffic_vs_algorithmic": traffic["vs_algorithmic"] if traffic else None,
+                     "traffic_source": traffic["source"] if traffic else None,
+                     "kernel": "k_customize_batch_level", "level_launches": n_lv,
+                     "algorithmic_bytes_per_launch": B8d_lv / n_lv, "avg_launch_us": lv_avg_us,
+                     "what": "algorithmic bytes 4 T + 4 (n+1) + 4 m+ + 16 K m+ over the level launches "
+                             "(c-index stream, first/head, every (up, down) pair read and written once); "
+                             "average launch = (customization - base pass) / level launches, HIP events",
+                     "layout_bytes_per_launch": layout_level_bytes(topo.mp, topo.triangles, K) / n_lv,
+                     "customization_ms": bms, "base_pass_ms": bbase_ms,
+                     "customization_algorithmic_bytes": B8d,
+                     "customization_frac": B8d / (bms / 1e3) / 1e9 / HBM_PEAK_GBS},
+        "cpu_baseline": cpu,
+        "speedup_vs_cpu_baseline": (K * topo.triangles / t_step) / cpu["value"] if cpu else None,
+        "step_ms": {"hip_graph": 1e3 * t_step, "eager": 1e3 * t_eager, "batched_scan_ms": bscan_ms},
+        "flagged_metrics": flagged,
+        "plain_one_metric": {"step_ms": 1e3 * t_plain, "customize_ms": cust1_ms, "scan_ms": scan1_ms,
+                             "triangles_per_s": topo.triangles / (cust1_ms / 1e3),
+                             "batched_speedup_per_metric": (K * cust1_ms) / bms,
+                             "roofline": {"kernel": "k_customize_flat", "achieved": ach1, "frac": ach1 / HBM_PEAK_GBS,
+                                          "avg_launch_us": lv1_us, "level_launches": n_lv1,
+                                          "base_pass_ms": base1_ms}},
+        "queries": queries,
+        "parity": parity,
+        "host": dict(host, nproc=os.cpu_count()),
+    })
+    del dt
 
 
 def main():
@@ -893,16 +1037,19 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", type=int, default=1)
+    ap.add_argument("--config", type=int, default=4,
+                    help="4 (default): the full-scale headline; 1-3: the plain, batched and serving workloads")
+    ap.add_argument("--cache", default=None, help="config 4: read/write graph, order and metrics to this .npz")
+    ap.add_argument("--graph", type=int, default=4, help="config 4 mode on another config's graph (a quick run)")
+    ap.add_argument("--single-full", type=int, default=200, help="config 4: single-query launches for p50/p99")
+    ap.add_argument("--cpu-queries", type=int, default=2000, help="config 4: reference queries timed on 1 core")
     ap.add_argument("--queries", type=int, default=100_000)
     ap.add_argument("--single", type=int, default=500, help="single-query launches for latency")
-    ap.add_argument("--metrics", type=int, default=64)
+    ap.add_argument("--metrics", type=int, default=None, help="metrics per batch (8 at config 4, 64 at config 2)")
     ap.add_argument("--batch-steps", type=int, default=5)
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="also run one device copy of this many MiB (scales HBM byte counters)")
     ap.add_argument("--no-graph", action="store_true", help="skip the HIP graph replay of the step")
-    ap.add_argument("--full-scale", action="store_true",
-                    help="run the 23.9M-point road graph (config 4) instead of the default workloads")
     ap.add_argument("--prep", action="store_true",
                     help="time everything after the order for --config on the host and on the GPU")
     ap.add_argument("--full-queries", type=int, default=1_000_000)
@@ -928,12 +1075,14 @@ def main():
         "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "int32", "data": "synthetic",
     }
-    if args.full_scale:
-        full_scale(args, out)
-        print(json.dumps(out), flush=True)
-        return
+    if args.metrics is None:
+        args.metrics = 8 if args.config == 4 else 64
     if args.prep:
         prep(args, out)
+        print(json.dumps(out), flush=True)
+        return
+    if args.config == 4:
+        headline(args, out)
         print(json.dumps(out), flush=True)
         return
     plain(args, out)

@@ -154,6 +154,17 @@ int nwcch_customize(nwcch_dev* h, const int32_t* w, int32_t* ud, void* stream);
  * 256 streams per handle keep their own slot; a further stream takes the
  * oldest slot over after a device synchronisation. */
 
+/* The values python/cch.py's customize(g, stop_on_negative_cycle=True) leaves
+ * when it stops (python/cch.py:259-278): from the complete customization
+ * ud_full (device) of weights w (device), ud_out (device, not ud_full) gets the
+ * final values of every arc whose lower end has rank <= x_stop, and for every
+ * other arc its base value folded with the lower triangles whose lower vertex
+ * is <= x_stop.  The reference stops at the row of its first witness arc, so
+ * x_stop = that arc's lower end; x_stop = -1 gives the base values it leaves
+ * for a negative self-loop (python/cch.py:259-263). */
+int nwcch_customize_stop_state(nwcch_dev* h, const int32_t* w, const int32_t* ud_full, int32_t x_stop,
+                               int32_t* ud_out, void* stream);
+
 /* Negative-cycle scan of ud[mp][2] (any metric, no handle needed): out2
  * (device int64[2]) = {number of arcs with up + down < 0 (INF never counts),
  * smallest such arc or -1}.  Replaces cch_nb.negative_cycle_arcs

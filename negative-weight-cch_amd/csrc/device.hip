@@ -132,6 +132,59 @@ __global__ void __launch_bounds__(256) k_base_all(i64 mp, const int32_t* __restr
   if (st) atomicOr(status, st);
 }
 
+// The values the reference leaves behind when customize(g,
+// stop_on_negative_cycle=True) stops (python/cch.py:259-278): it applies the
+// lower triangles x ascending and checks the arcs of every vertex once the
+// triangles pass its row, so when it stops at the row of vertex x_stop every
+// triangle whose lower vertex is at most x_stop has been applied and no other.
+// From the complete customization `full`: arcs whose tail is at most x_stop
+// keep their (final) values; every other arc gets its base value folded with
+// the triangles of lower vertex <= x_stop, whose two sides are final.  x_stop
+// = -1 leaves the base values (a negative self-loop, python/cch.py:259-263).
+// One wavefront per schedule position.
+__global__ void __launch_bounds__(256) k_stop_state(i64 mp, const PosRec* __restrict__ prec,
+                                                    const int2* __restrict__ tri_ab,
+                                                    const int32_t* __restrict__ tail,
+                                                    const int32_t* __restrict__ ae_off,
+                                                    const int32_t* __restrict__ ae_idx,
+                                                    const int32_t* __restrict__ w, const int2* __restrict__ full,
+                                                    int32_t x_stop, int2* out, uint32_t* status) {
+  const int lane = threadIdx.x & 63;
+  const i64 wstride = (i64)gridDim.x * (blockDim.x >> 6);
+  uint32_t st = 0;
+  for (i64 p = (i64)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); p < mp; p += wstride) {
+    const PosRec r = prec[p];
+    const int32_t c = r.arc;
+    if (tail[c] <= x_stop) {
+      if (lane == 0) out[c] = full[c];
+      continue;
+    }
+    int32_t up = kInf, dn = kInf;
+    for (int32_t j = ae_off[c] + lane; j < ae_off[c + 1]; j += 64) {
+      const int32_t e = ae_idx[j];
+      const int32_t wk = w[e >> 1];
+      if (wk == (int32_t)0x80000000) st |= kStatusBadWeight;
+      if (e & 1) up = min(up, wk);
+      else dn = min(dn, wk);
+    }
+    const i64 t1 = r.tri_begin + r.tri_count;
+    for (i64 q = r.tri_begin + lane; q < t1; q += 64) {
+      const int2 ab = tri_ab[q];
+      if (tail[ab.x] > x_stop) continue;  // lower vertex past the stop
+      const int2 A = full[ab.x], B = full[ab.y];
+      up = min(up, inf_add(A.y, B.x, st));
+      dn = min(dn, inf_add(B.y, A.x, st));
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      up = min(up, __shfl_xor(up, o, 64));
+      dn = min(dn, __shfl_xor(dn, o, 64));
+    }
+    if (lane == 0) out[c] = make_int2(up, dn);
+  }
+  if (st) atomicOr(status, st);
+}
+
 // The buffers of one call into its argument slot (a kernel, so the values
 // travel with the launch in stream order).
 __global__ void k_set_slot(SlotPtrs* P, const void* in, void* out) {
@@ -1513,6 +1566,29 @@ extern "C" int nwcch_customize(nwcch_dev* h, const int32_t* w, int32_t* ud, void
     DevTopo& d = h->d;
     run_cached(d, d.cust, (hipStream_t)stream, w, ud,
                [&](const SlotPtrs* P, hipStream_t s) { launch_customize(d, P, s); });
+  } catch (const std::exception& e) {
+    return fail(e);
+  }
+  return NWCCH_OK;
+}
+
+extern "C" int nwcch_customize_stop_state(nwcch_dev* h, const int32_t* w, const int32_t* ud_full, int32_t x_stop,
+                                          int32_t* ud_out, void* stream) {
+  if (!h || !w || !ud_full || !ud_out) { set_error("null argument"); return NWCCH_ERR_ARG; }
+  if (ud_out == ud_full) { set_error("ud_out must not be ud_full"); return NWCCH_ERR_ARG; }
+  try {
+    DevTopo& d = h->d;
+    if (x_stop < -1 || x_stop >= d.n) throw std::invalid_argument("x_stop out of range");
+    if (d.mp > 0) {
+      const i64 blocks = std::min<i64>((d.mp + 3) / 4, (i64)1 << 16);
+      hipLaunchKernelGGL(k_stop_state, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, d.mp, d.prec,
+                         d.tri_ab, d.tail, d.ae_off, d.ae_idx, w, reinterpret_cast<const int2*>(ud_full), x_stop,
+                         reinterpret_cast<int2*>(ud_out), d.status);
+    }
+    HIP_OK(hipGetLastError());
+  } catch (const std::invalid_argument& e) {
+    set_error(e.what());
+    return NWCCH_ERR_ARG;
   } catch (const std::exception& e) {
     return fail(e);
   }

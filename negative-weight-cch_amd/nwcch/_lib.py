@@ -51,6 +51,7 @@ def lib():
             "nwcch_dev_export_triangles": ([vp, p], C.c_int),
             "nwcch_dev_export_sides": ([vp, p, p], C.c_int),
             "nwcch_customize": ([vp, p, p, vp], C.c_int),
+            "nwcch_customize_stop_state": ([vp, p, p, i32, p, vp], C.c_int),
             "nwcch_negcycle_scan": ([p, i64, p, vp], C.c_int),
             "nwcch_negcycle_list": ([p, i64, p, i64, p, vp], C.c_int),
             "nwcch_query": ([vp, p, p, p, i64, p, p, vp], C.c_int),
@@ -115,7 +116,8 @@ def exported_symbols():
             "nwcch_customize_batch", "nwcch_negcycle_scan_batch", "nwcch_batch_slab_width",
             "nwcch_set_tuning", "nwcch_get_tuning", "nwcch_query_multi", "nwcch_potential",
             "nwcch_query_path", "nwcch_topo_create_ex", "nwcch_dev_export_triangles",
-            "nwcch_dev_export_sides", "nwcch_topo_from_csr_ex", "nwcch_update_noalloc"]
+            "nwcch_dev_export_sides", "nwcch_topo_from_csr_ex", "nwcch_update_noalloc",
+            "nwcch_customize_stop_state"]
 
 
 # ------------------------------------------------------------ host topology

@@ -150,6 +150,8 @@ class CCH:
         self.ud = self.dev.customize(self.edge_w, out=self.ud)
         self._last = None
         if (self._loop_w < 0).any():
+            # the reference returns before any triangle, leaving the base values
+            self.ud = self.dev.stop_state(self.edge_w, self.ud, -1)
             self.dev.check_status(negative_cycle=True)
             self.cycle_arc = -1
             return [-1]
@@ -165,6 +167,9 @@ class CCH:
                 a_max = int((self.first[rows + 1] - 2).max())
                 x = int(np.searchsorted(self.first, lst[0], side="right") - 1)
                 if self.first[x + 1] <= a_max:
+                    # the values the reference leaves when it stops at row x
+                    self.ud = self.dev.stop_state(self.edge_w, self.ud, x)
+                    self.dev.check_status(negative_cycle=True)
                     self.cycle_arc = lst[0]
                     return [lst[0]]
         return lst

@@ -106,6 +106,18 @@ class DeviceTopology:
         check(lib().nwcch_customize(self._h, ptr(w), ptr(out), _stream(stream)), "nwcch_customize")
         return out
 
+    def stop_state(self, w: torch.Tensor, ud_full: torch.Tensor, x_stop: int, out: torch.Tensor | None = None,
+                   stream=None) -> torch.Tensor:
+        """The values python/cch.py's early-stopping customize leaves when it
+        stops at the row of rank x_stop (-1: base values), from the complete
+        customization ud_full of the device weights w."""
+        if out is None:
+            out = torch.empty_like(ud_full)
+        assert out.shape == (self.mp, 2) and ud_full.shape == (self.mp, 2) and out.data_ptr() != ud_full.data_ptr()
+        check(lib().nwcch_customize_stop_state(self._h, ptr(w), ptr(ud_full), int(x_stop), ptr(out),
+                                               _stream(stream)), "nwcch_customize_stop_state")
+        return out
+
     def negcycle_scan(self, ud: torch.Tensor, out: torch.Tensor | None = None, stream=None):
         """Device int64[2] = (count, first witness or -1); not synchronised."""
         out = self._scan_out if out is None else out

@@ -88,6 +88,7 @@ def rlib():
         L.ref_negcycle.argtypes = [p, p, p]
         L.ref_negcycle.restype = i64
         L.ref_query.argtypes = [p, p, p, i64, p, p, p]
+        L.ref_query_par.argtypes = [p, p, p, i64, p, p, p, C.c_int]
         L.ref_levels.argtypes = [p, p]
         L.ref_load_dir.argtypes = [C.c_char_p, C.c_char_p]
         L.ref_load_dir.restype = C.c_void_p
@@ -144,13 +145,15 @@ class Structure:
                                    _p(self.e_up), _p(w), _p(up), _p(dn))
         return up, dn, bool(ovf)
 
-    def customize_stop(self, w):
-        """Early-stop witness of python/cch.py:270-278, or -1."""
+    def customize_stop(self, w, with_state=False):
+        """Early-stop witness of python/cch.py:270-278, or -1; with_state also
+        returns the (up, dn) values at the stop."""
         w = np.ascontiguousarray(w, np.int32)
         up = np.empty(self.mp, np.int32)
         dn = np.empty(self.mp, np.int32)
-        return int(olib().orc_customize_stop(self.n, _p(self.first), _p(self.head), self.m, _p(self.e_arc),
-                                             _p(self.e_up), _p(w), _p(up), _p(dn)))
+        wit = int(olib().orc_customize_stop(self.n, _p(self.first), _p(self.head), self.m, _p(self.e_arc),
+                                            _p(self.e_up), _p(w), _p(up), _p(dn)))
+        return (wit, up, dn) if with_state else wit
 
     def customize_par(self, w, threads=8):
         w = np.ascontiguousarray(w, np.int32)
@@ -364,6 +367,13 @@ class Reference:
         t = np.ascontiguousarray(t, np.int32)
         d = np.empty(len(s), np.float64)
         rlib().ref_query(self.h, _p(up), _p(dn), len(s), _p(s), _p(t), _p(d))
+        return d
+
+    def query_par(self, up, dn, s, t, threads):
+        s = np.ascontiguousarray(s, np.int32)
+        t = np.ascontiguousarray(t, np.int32)
+        d = np.empty(len(s), np.float64)
+        rlib().ref_query_par(self.h, _p(up), _p(dn), len(s), _p(s), _p(t), _p(d), int(threads))
         return d
 
     def levels(self):

@@ -156,6 +156,22 @@ void ref_query(void* p, const double* up, const double* dn, int64_t nq, const in
   for (int64_t i = 0; i < nq; i++) dist[i] = q.run(s[i], t[i], u, d);
 }
 
+// The same queries spread over `threads` OpenMP threads, each with its own
+// Query object (its own labels and marks): the reference's query run on
+// independent queries in parallel.
+void ref_query_par(void* p, const double* up, const double* dn, int64_t nq, const int32_t* s, const int32_t* t,
+                   double* dist, int threads) {
+  RefHandle* h = (RefHandle*)p;
+  const size_t mp = h->c.head.size();
+  vector<double> u(up, up + mp), d(dn, dn + mp);
+#pragma omp parallel num_threads(threads)
+  {
+    Query q(h->c);
+#pragma omp for schedule(dynamic, 64)
+    for (int64_t i = 0; i < nq; i++) dist[i] = q.run(s[i], t[i], u, d);
+  }
+}
+
 // Level buckets of CCH::levels: level[x] for every rank x
 void ref_levels(void* p, int32_t* level) {
   RefHandle* h = (RefHandle*)p;

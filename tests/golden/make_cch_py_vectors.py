@@ -7,10 +7,12 @@ for the outputs that no closed form or other reference test fixes:
 * ``CCH.update_edges`` (python/cch.py:349-402): the number of arcs evaluated
   and ``last_changed`` (the changed arcs in pop order);
 * ``CCH.customize(g, stop_on_negative_cycle=True)`` (python/cch.py:264-278):
-  the witness arc;
+  the witness arc and the values it leaves behind when it stops;
 * ``CCH.customize(g)``: the full witness list (python/cch.py:287);
 * ``CCH.query_path`` (python/cch.py:406-507): the vertex sequence of the
-  unpacked shortest path, which depends on the order of ties.
+  unpacked shortest path, which depends on the order of ties;
+* ``CCH.query`` (python/cch.py:675-732): the distance and the number of arcs
+  scanned, which no other output fixes.
 
 Only data is written: ``cch_py_vectors.npz`` with the inputs (graph CSR, order,
 weights, update batches) and the reference's outputs.  The graphs come from
@@ -73,6 +75,17 @@ def main():
         out[f"{name}/off"], out[f"{name}/tgt"] = g.off, g.tgt
         out[f"{name}/w"], out[f"{name}/rank"] = g.w, rank
         out[f"{name}/up0"], out[f"{name}/dn0"] = as_i32(c.up), as_i32(c.down)
+        # ET queries on the customized metric: distance and arcs scanned
+        qr2 = np.random.default_rng(29)
+        qs2, qt2 = qr2.integers(0, g.n, 300), qr2.integers(0, g.n, 300)
+        qd, qsc = [], []
+        for a, b in zip(qs2.tolist(), qt2.tolist()):
+            dd, scn = c.query(a, b)
+            qd.append(INF32 if dd == float("inf") else int(dd))
+            qsc.append(int(scn))
+        out[f"{name}/q_s"], out[f"{name}/q_t"] = qs2.astype(np.int32), qt2.astype(np.int32)
+        out[f"{name}/q_dist"] = np.array(qd, np.int64)
+        out[f"{name}/q_scanned"] = np.array(qsc, np.int64)
         # unpacked shortest paths on the customized metric
         qr = np.random.default_rng(23)
         qs_, qt_ = qr.integers(0, g.n, 30), qr.integers(0, g.n, 30)
@@ -129,6 +142,7 @@ def main():
         out[f"{name}/neg_witnesses"] = np.asarray(cn.customize(rgn), np.int64)
         cs = refcch.CCH(rgn, order=rank.tolist(), log=lambda *_: None)
         out[f"{name}/neg_stop_witness"] = np.asarray(cs.customize(rgn, stop_on_negative_cycle=True), np.int64)
+        out[f"{name}/neg_stop_up"], out[f"{name}/neg_stop_dn"] = as_i32(cs.up), as_i32(cs.down)
         print(name, g.n, g.m, "mp", c.m, "witnesses", len(out[f"{name}/neg_witnesses"]),
               "stop", out[f"{name}/neg_stop_witness"].tolist(), flush=True)
     out["cases"] = np.array([n for n, _ in cases])

@@ -126,3 +126,27 @@ def test_path_limit_on_negative_cycle():
         else:
             assert inv[path[i, :plen[i]]].tolist() == pw if plen[i] >= 0 else pw == [], i
     assert stopped > 0 and dt.status() & 16
+
+
+@pytest.mark.parametrize("form", ["chain_walk_throughput", "chain_walk_latency", "chunked_latency"])
+@pytest.mark.parametrize("name", ["grid", "small0", "small1", "small2"])
+def test_query_scanned_equals_reference_python(name, form):
+    """Every query kernel form against the distances and scanned-arc counts the
+    reference's python/cch.py CCH.query returned (python/cch.py:675-732)."""
+    from pathlib import Path
+    G = np.load(Path(__file__).resolve().parent / "golden" / "cch_py_vectors.npz")
+    print(f"case: golden queries {name} {form}", flush=True)
+    off, tgt, w, rank = (G[f"{name}/{k}"] for k in ("off", "tgt", "w", "rank"))
+    n = len(rank)
+    topo = _lib.Topology.from_graph(n, off, tgt, rank.astype(np.int32))
+    dt = DeviceTopology(topo)
+    ud = dt.customize(w.astype(np.int32))
+    chunk, deep = {"chain_walk_throughput": (0, 0), "chain_walk_latency": (0, 1 << 30),
+                   "chunked_latency": (1, 1 << 30)}[form]
+    dt.set_tuning("query_chunk", chunk)
+    dt.set_tuning("query_deep_max", deep)
+    qs, qt = rank[G[f"{name}/q_s"]], rank[G[f"{name}/q_t"]]
+    dist, sc = dt.query(ud, qs, qt)
+    assert np.array_equal(dist.cpu().numpy().astype(np.int64), G[f"{name}/q_dist"])
+    assert np.array_equal(sc.cpu().numpy().astype(np.int64), G[f"{name}/q_scanned"])
+    dt.check_status()

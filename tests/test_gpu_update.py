@@ -170,6 +170,9 @@ def test_dropin_matches_reference_python_vectors(name, gpu_index):
     c2 = CCH(gn, order=rank, log=lambda *_: None, gpu_index=gpu_index)
     assert c2.customize(gn) == G[f"{name}/neg_witnesses"].tolist()
     assert c2.customize(gn, stop_on_negative_cycle=True) == G[f"{name}/neg_stop_witness"].tolist()
+    # and the values the reference leaves when it stops
+    ud = c2.ud.cpu().numpy()
+    assert np.array_equal(ud[:, 0], G[f"{name}/neg_stop_up"]) and np.array_equal(ud[:, 1], G[f"{name}/neg_stop_dn"])
 
 
 @pytest.mark.parametrize("mode", [1, 2])
@@ -198,3 +201,51 @@ def test_updates_queued_back_to_back(c0, mode):
     got = ud.cpu().numpy()
     assert np.array_equal(got[:, 0], upd.up) and np.array_equal(got[:, 1], upd.dn)
     assert dt.status() == 0
+
+
+@pytest.mark.parametrize("idx", [0, 4, 8, 12])
+def test_stop_state_equals_oracle(idx):
+    """customize(g, stop_on_negative_cycle=True) through the drop-in leaves the
+    oracle's restatement of the reference's stopped values (python/cch.py:
+    270-278) on graphs with an injected negative cycle; a negative self-loop
+    leaves the base values (python/cch.py:259-263)."""
+    g = small_graphs()[idx] if idx < 12 else graphs.make_config(0)
+    rank = order_of(g) if idx < 12 else _lib.order(g.n, g.off, g.tgt, g.x, g.y, 64)
+    print(f"case: stop state, graph {idx}", flush=True)
+    tri = graphs.find_directed_triangle(g, np.random.default_rng(idx))
+    if tri is None:
+        pytest.skip("no directed triangle")
+    wn = graphs.inject_negative_cycle(g.w, tri, total=-3)
+    gn = g.with_weights(wn)
+    st = O.Structure(g.n, g.off, g.tgt, rank)
+    wit, up, dn = st.customize_stop(wn, with_state=True)
+    c = CCH(gn, order=rank, log=lambda *_: None)
+    got = c.customize(gn, stop_on_negative_cycle=True)
+    fu, fd, _ = st.customize(wn)
+    full = np.nonzero((fu != INF) & (fd != INF) & (fu.astype(np.int64) + fd < 0))[0].tolist()
+    if wit >= 0:
+        assert got == [wit]
+        ud = c.ud.cpu().numpy()
+        assert np.array_equal(ud[:, 0], up) and np.array_equal(ud[:, 1], dn)
+    else:
+        assert got == full
+
+
+def test_negative_self_loop_leaves_base_values():
+    g0 = graphs.shifted_random(80, seed=5)
+    arcs = [(u, int(v), int(w)) for u in range(g0.n) for v, w in zip(g0.tgt[g0.off[u]:g0.off[u + 1]],
+                                                                     g0.w[g0.off[u]:g0.off[u + 1]])]
+    arcs.append((7, 7, -1))
+    g = graphs.from_arcs(g0.n, arcs, xy=list(zip(g0.x, g0.y)))
+    rank = order_of(g)
+    c = CCH(g, order=rank, log=lambda *_: None)
+    assert c.customize(g, stop_on_negative_cycle=True) == [-1]
+    st = O.Structure(g.n, g.off, g.tgt, rank)
+    base_up = np.full(st.mp, INF, np.int64)
+    base_dn = np.full(st.mp, INF, np.int64)
+    m = st.e_arc >= 0
+    np.minimum.at(base_up, st.e_arc[m & (st.e_up == 1)], g.w[m & (st.e_up == 1)])
+    np.minimum.at(base_dn, st.e_arc[m & (st.e_up == 0)], g.w[m & (st.e_up == 0)])
+    ud = c.ud.cpu().numpy()
+    assert np.array_equal(ud[:, 0], base_up) and np.array_equal(ud[:, 1], base_dn)
+    assert c.customize(g) == [-1]
