@@ -104,7 +104,7 @@ struct CellFlow {
   void build(const std::vector<i32>& cell, const Member& loc,
              const std::vector<i64>& noff, const std::vector<i32>& nbr,
              const std::vector<int8_t>& role, const std::vector<char>& active,
-             const std::vector<int>& nid, const std::vector<double>& proj) {
+             const std::vector<int>& nid) {
     s = (int)cell.size();
     S = 2 * s;
     T = 2 * s + 1;
@@ -134,84 +134,86 @@ struct CellFlow {
       to[ia] = b; cap[ia] = c; rev[ia] = ib;
       to[ib] = a; cap[ib] = 0; rev[ib] = ia;
     };
-    // out(l)'s arcs: T first for a sink, then the neighbours by projection
-    // descending, so the depth-first search below heads for the sink side
-    std::vector<std::pair<double, int>> nb;
     for (int l = 0; l < s; l++) {
       if (!active[l]) continue;
       i32 v = cell[l];
       const int nl = nid[l];
       // sources and sinks cannot be cut (infinite vertex capacity)
       add(2 * nl, 2 * nl + 1, role[l] ? kInfCap : 1);
-      if (role[l] == 2) add(2 * nl + 1, T, kInfCap);
-      nb.clear();
       for (i64 j = noff[v]; j < noff[v + 1]; j++) {
         i32 u = nbr[j];
-        if (loc[u] >= 0 && active[loc[u]]) nb.emplace_back(-proj[loc[u]], nid[loc[u]]);
+        if (loc[u] >= 0 && active[loc[u]]) add(2 * nl + 1, 2 * nid[loc[u]], kInfCap);
       }
-      std::sort(nb.begin(), nb.end());
-      for (const auto& q : nb) add(2 * nl + 1, 2 * q.second, kInfCap);
       if (role[l] == 1) add(S, 2 * nl, kInfCap);
+      if (role[l] == 2) add(2 * nl + 1, T, kInfCap);
     }
     level.assign(N, -1);
     it.assign(N, 0);
     queue.assign(N, 0);
   }
 
-  // Maximum flow in rounds of one depth-first search each, from S with a
-  // round stamp per node: the search follows each node's arcs in order (toward
-  // the sink side of the projection); on reaching T it augments the path on its
-  // stack and starts again from S, keeping the stamps, so the paths of a round
-  // are node-disjoint and every node is entered at most once per round.  A
-  // round that finds no path ends the flow: T is then unreachable.  Any maximum
-  // flow gives the same residual reachability from S, hence the same cut.
-  long maxflow_dfs(long limit, const std::atomic<long>* bound) {
-    std::vector<int>& seen = level;
-    std::vector<int>& cur = it;
+  // Maximum flow in rounds of one breadth-first search each: the search tree
+  // from S is traced back from every node with a residual arc into T, and each
+  // traced path that shares no node with a path already taken this round is
+  // augmented.  On these networks one search yields many disjoint paths (a
+  // blocking-flow phase restricted to shortest paths yields about one, since
+  // path lengths across a clustered point set all differ).  Any maximum flow
+  // gives the same residual reachability from S, hence the same cut.
+  // Stops early, returning a value above `*bound`, once the flow exceeds
+  // *bound (the smallest cut another direction has already completed): such
+  // a cut can no longer be the chosen one.
+  long maxflow_rounds(long limit, const std::atomic<long>* bound) {
+    std::vector<int>& seen = level;  // round stamp of the search
+    std::vector<int>& par = it;      // tree arc into each reached node
+    std::vector<int> used(N, 0);     // round stamp of nodes on augmented paths
     std::fill(seen.begin(), seen.end(), 0);
     std::vector<int>& path = stack_;
     long flow = 0;
     for (int round = 1; flow < limit; round++) {
-      for (int i = 0; i < N; i++) cur[i] = off[i];
-      long got = 0;
+      int qh = 0, qt = 0;
+      queue[qt++] = S;
       seen[S] = round;
-      path.clear();
-      int x = S;
-      while (true) {
-        if (x == T) {
-          int f = kInfCap;
-          for (int a : path) f = std::min(f, cap[a]);
-          for (int a : path) {
-            cap[a] -= f;
-            cap[rev[a]] += f;
-          }
-          flow += f;
-          got += f;
-          if (flow >= limit || flow > bound->load(std::memory_order_relaxed)) break;
-          path.clear();
-          x = S;
-          continue;
-        }
-        int e = cur[x];
-        const int ee = off[x + 1];
-        while (e < ee && (cap[e] <= 0 || (to[e] != T && seen[to[e]] == round))) e++;
-        if (e < ee) {
-          cur[x] = e + 1;
+      while (qh < qt) {
+        const int x = queue[qh++];
+        for (int e = off[x]; e < off[x + 1]; e++) {
           const int y = to[e];
-          if (y != T) seen[y] = round;
-          path.push_back(e);
-          x = y;
-        } else {
-          cur[x] = ee;
-          if (path.empty()) break;  // S has no way on: the round is over
-          x = to[rev[path.back()]];
-          path.pop_back();
+          if (cap[e] > 0 && seen[y] != round) {
+            seen[y] = round;
+            par[y] = e;
+            if (y != T) queue[qt++] = y;
+          }
         }
       }
-      if (got == 0 || flow >= limit || flow > bound->load(std::memory_order_relaxed)) break;
+      if (seen[T] != round) break;
+      long got = 0;
+      for (int e = off[T]; e < off[T + 1] && flow < limit; e++) {
+        const int u = to[e];  // arc u -> T is rev[e]
+        if (seen[u] != round || used[u] == round || cap[rev[e]] <= 0) continue;
+        path.clear();
+        path.push_back(rev[e]);
+        bool ok = true;
+        for (int y = u; y != S; y = to[rev[par[y]]]) {
+          if (used[y] == round) { ok = false; break; }
+          path.push_back(par[y]);
+        }
+        if (!ok) continue;
+        int f = kInfCap;
+        for (int a : path) f = std::min(f, cap[a]);
+        for (int a : path) {
+          cap[a] -= f;
+          cap[rev[a]] += f;
+          used[to[a]] = round;
+        }
+        used[T] = 0;
+        flow += f;
+        got += f;
+      }
+      if (got == 0) break;  // cannot happen while T is reachable; guards the loop
+      if (flow > bound->load(std::memory_order_relaxed)) break;
     }
     return flow;
   }
+
   // residual reachability from S (after the maximum flow)
   void reach(std::vector<char>& seen) {
     seen.assign(N, 0);
@@ -328,11 +330,9 @@ Split direction_cut(int d, const std::vector<i32>& cell, const Member& loc,
     }
     if (inner) active[l] = 0;
   }
-  std::vector<double> proj(s);
-  for (int l = 0; l < s; l++) proj[l] = key(l);
   CellFlow net;
-  net.build(cell, loc, noff, nbr, role, active, nid, proj);
-  const long flow = net.maxflow_dfs((long)s + 1, bound);
+  net.build(cell, loc, noff, nbr, role, active, nid);
+  const long flow = net.maxflow_rounds((long)s + 1, bound);
   long b = bound->load(std::memory_order_relaxed);
   if (flow > b) return cur;  // loses to a finished direction
   while (flow < b && !bound->compare_exchange_weak(b, flow, std::memory_order_relaxed)) {
