@@ -95,18 +95,27 @@ __global__ void __launch_bounds__(256) k_base_batch(i64 mp, int32_t K, int32_t s
   if (st) atomicOr(status, st);
 }
 
-// Launch geometry of one level of the batched sweep: wave-team class j gives
-// WPA = 16 >> j wavefronts to each arc of positions [cls[j], cls[j+1]) (the
-// level's arcs are sorted by triangle count, descending) and owns blocks
-// [blk[j], blk[j+1]); every block is 16 wavefronts.
+// Launch geometry of one level of the batched sweep.  Wave classes j = 0..4
+// give WPA = 16 >> j wavefronts to each arc of positions [cls[j], cls[j+1])
+// (the level's arcs are sorted by triangle count, descending).  Sub-wave
+// classes j = 5, 6, 7 put 2, 4, 8 arcs in one wavefront (a team of S >> (j - 4)
+// triangle sub-lanes x KL metrics each), for the light arcs that dominate the
+// lowest levels.  Arcs without lower triangles keep their base value and are
+// not launched.  Class j owns blocks [blk[j], blk[j+1]); every block is 16
+// wavefronts.
 constexpr int kWaveClasses = 5;
+constexpr int kMaxClasses = 8;  // with up to 3 sub-wave classes (S >= 8)
 constexpr int kUnroll = 4;      // triangles per lane with their gathers in flight at once (8 measured 15 % slower)
 constexpr int kSlabWidth = 64;  // default metrics per slab
 struct BatchLaunch {
-  i64 cls[kWaveClasses + 1];
-  int32_t blk[kWaveClasses + 1];
-  int32_t xcd;  // 1: a class's blocks (a multiple of 8) renumbered so that XCD x takes the x-th slice
+  i64 cls[kMaxClasses + 1];
+  int32_t blk[kMaxClasses + 1];
+  int32_t ncls;  // classes in use: kWaveClasses + the sub-wave classes S allows
+  int32_t xcd;   // 1: a class's blocks (a multiple of 8) renumbered so that XCD x takes the x-th slice
 };
+
+// Arcs per wavefront of class j: 1 for the wave classes, 2, 4, 8 below them.
+__host__ __device__ inline int class_apw_log2(int j) { return j < kWaveClasses ? 0 : j - kWaveClasses + 1; }
 
 // One level of the batched sweep.  A wavefront holds KL metrics x (64 / KL)
 // triangle sub-lanes; WPA wavefronts of one block split an arc's triangles,
@@ -127,9 +136,11 @@ __global__ void __launch_bounds__(1024) k_customize_batch_level(BatchLaunch L, i
   __shared__ int2 red[16][64];
   const int32_t b = blockIdx.x;
   int j = 0;
-  while (j < kWaveClasses - 1 && b >= L.blk[j + 1]) j++;
-  const int wlog = 4 - j;
-  const int WPA = 1 << wlog;
+  while (j < L.ncls - 1 && b >= L.blk[j + 1]) j++;
+  const int wlog = j < kWaveClasses ? 4 - j : 0;
+  const int WPA = 1 << wlog;          // wavefronts per arc (wave classes)
+  const int alog = class_apw_log2(j);  // arcs per wavefront = 1 << alog (sub-wave classes)
+  const int SP = S >> alog;           // triangle sub-lanes per arc
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   int32_t local = b - L.blk[j];
   if (L.xcd) {
@@ -138,9 +149,10 @@ __global__ void __launch_bounds__(1024) k_customize_batch_level(BatchLaunch L, i
   }
   const int32_t slice = local % slices;
   const i64 grp = local / slices;
-  const i64 p = L.cls[j] + grp * (16 >> wlog) + (wave >> wlog);
-  const int chunk = wave & (WPA - 1);
   const int km = lane % KL, ts = lane / KL;
+  const int team = ts / SP, tsl = ts % SP;  // sub-wave classes: arc of the lane's team, sub-lane in it
+  const i64 p = L.cls[j] + grp * ((16 >> wlog) << alog) + ((wave >> wlog) << alog) + team;
+  const int chunk = wave & (WPA - 1);
   const int32_t metric = slice * KL + km;
   const bool live = p < L.cls[j + 1] && metric < K;
   int2* slab = UD + (i64)slice * mp * KL + km;  // element of arc a: slab[a * KL]
@@ -150,14 +162,14 @@ __global__ void __launch_bounds__(1024) k_customize_batch_level(BatchLaunch L, i
   if (live) {
     const PosRec r = prec[bpos ? (i64)bpos[p] : p];
     c = r.arc;
-    if (chunk == 0 && ts == 0) {
+    if (chunk == 0 && tsl == 0) {
       const int2 base = slab[(i64)c * KL];
       up = base.x;
       dn = base.y;
     }
-    const int32_t stride = WPA * S;
+    const int32_t stride = WPA * SP;
     const i64 t1 = r.tri_begin + r.tri_count;
-    i64 q = r.tri_begin + chunk * S + ts;
+    i64 q = r.tri_begin + chunk * SP + tsl;
     for (; q + (kUnroll - 1) * stride < t1; q += kUnroll * stride) {
       int2 ab[kUnroll], A[kUnroll], B[kUnroll];
 #pragma unroll
@@ -181,10 +193,13 @@ __global__ void __launch_bounds__(1024) k_customize_batch_level(BatchLaunch L, i
       dn = min(dn, inf_add_b(B.y, A.x, st));
     }
   }
+  // the sub-lanes of one team are lanes KL apart inside a run of SP * KL lanes
 #pragma unroll
   for (int o = KL; o < 64; o <<= 1) {
-    up = min(up, __shfl_xor(up, o, 64));
-    dn = min(dn, __shfl_xor(dn, o, 64));
+    if (o < SP * KL) {  // uniform over the block
+      up = min(up, __shfl_xor(up, o, 64));
+      dn = min(dn, __shfl_xor(dn, o, 64));
+    }
   }
   if (WPA > 1) {  // uniform over the block
     red[wave][lane] = make_int2(up, dn);
@@ -197,7 +212,7 @@ __global__ void __launch_bounds__(1024) k_customize_batch_level(BatchLaunch L, i
       }
     }
   }
-  if (live && chunk == 0 && ts == 0) slab[(i64)c * KL] = make_int2(up, dn);
+  if (live && chunk == 0 && tsl == 0) slab[(i64)c * KL] = make_int2(up, dn);
   if (st) atomicOr(status, st);
 }
 
@@ -231,40 +246,52 @@ __global__ void k_batch_scan_finish(long long* out, int32_t K) {
     if (out[K + k] == 0x7fffffffffffffffll) out[K + k] = -1;
 }
 
-// Class boundaries of one level for S triangle sub-lanes per metric: the
-// smallest WPA (1..16) giving each lane at most R = batch_tri_per_lane
-// triangles.
+// Class boundaries of one level for S triangle sub-lanes per metric and R =
+// batch_tri_per_lane triangles per lane: wave class j (WPA = 16 >> j) holds the
+// arcs with (WPA/2) * S * R < count <= WPA * S * R (class 0 unbounded above);
+// sub-wave class j >= 5 (S >> (j - 4) sub-lanes per arc) those with half of
+// its upper bound < count <= (S >> (j - 4)) * R, the last class down to 1.
 BatchLaunch batch_launch(const DevTopo& d, const LevelRange& r, int S, int32_t slices) {
   BatchLaunch L;
   L.xcd = d.batch_xcd;
+  int sub = 0;
+  while (sub < kMaxClasses - kWaveClasses && (S >> (sub + 1)) >= 1) sub++;
+  L.ncls = kWaveClasses + sub;
   const i64 R = d.batch_tri_per_lane;
-  // positions are sorted by count descending: class j holds counts in
-  // (limit(j+1), limit(j)] where limit(j) = (16 >> j) * S * R
-  i64 p = r.begin;
-  for (int j = 0; j < kWaveClasses; j++) {
-    L.cls[j] = p;
-    if (j == kWaveClasses - 1) { p = r.end; continue; }
-    const i64 lim = (i64)(16 >> (j + 1)) * S * R;  // upper bound of class j+1
-    // advance while count > lim (binary search over the descending counts)
-    i64 lo = p, hi = r.end;
+  // positions are sorted by count descending; the arcs with no triangle end the level
+  auto first_at_most = [&](i64 lo, i64 lim) {  // first position in [lo, r.end) with count <= lim
+    i64 hi = r.end;
     while (lo < hi) {
       const i64 mid = (lo + hi) / 2;
       if (d.pos_cnt[mid] > lim) lo = mid + 1;
       else hi = mid;
     }
-    p = lo;
+    return lo;
+  };
+  const i64 end = first_at_most(r.begin, 0);
+  i64 p = r.begin;
+  for (int j = 0; j < L.ncls; j++) {
+    L.cls[j] = p;
+    if (j == L.ncls - 1) { p = end; continue; }
+    // upper bound of class j+1: lanes per arc of class j+1 x R
+    const i64 lim = j + 1 < kWaveClasses ? (i64)(16 >> (j + 1)) * S * R : (i64)(S >> (j + 1 - kWaveClasses + 1)) * R;
+    p = std::max(p, std::min(end, first_at_most(p, lim)));
   }
-  L.cls[kWaveClasses] = r.end;
+  L.cls[L.ncls] = end;
   int32_t b = 0;
-  for (int j = 0; j <= kWaveClasses; j++) {
+  for (int j = 0; j <= L.ncls; j++) {
     L.blk[j] = b;
-    if (j < kWaveClasses) {
+    if (j < L.ncls) {
       const i64 arcs = L.cls[j + 1] - L.cls[j];
-      const i64 per = 16 >> (4 - j);
+      const i64 per = j < kWaveClasses ? (i64)(16 >> (4 - j)) : (i64)16 << class_apw_log2(j);  // arcs per block
       i64 nb = ((arcs + per - 1) / per) * slices;
       if (L.xcd) nb = (nb + 7) / 8 * 8;  // padding blocks find no arc and only join the reduction
       b += (int32_t)nb;
     }
+  }
+  for (int j = L.ncls + 1; j <= kMaxClasses; j++) {
+    L.cls[j] = end;
+    L.blk[j] = b;
   }
   return L;
 }
@@ -277,8 +304,8 @@ void ensure_bpos(DevTopo& d, int S) {
   std::vector<int32_t> pos(d.mp);
   for (const LevelRange& r : d.lv) {
     const BatchLaunch L = batch_launch(d, r, S, 1);
-    for (int j = 0; j < kWaveClasses; j++) {
-      for (i64 p = L.cls[j]; p < L.cls[j + 1]; p++) pos[p] = (int32_t)p;
+    for (i64 p = r.begin; p < r.end; p++) pos[p] = (int32_t)p;
+    for (int j = 0; j < L.ncls; j++) {
       std::sort(pos.begin() + L.cls[j], pos.begin() + L.cls[j + 1],
                 [&](int32_t a, int32_t b) { return d.sched_host[a] < d.sched_host[b]; });
     }
@@ -300,8 +327,8 @@ void ensure_bpos(DevTopo& d, int S) {
 template <int KL>
 void launch_level(const DevTopo& d, const LevelRange& r, int32_t slices, int32_t K, int2* UD, hipStream_t s) {
   const BatchLaunch L = batch_launch(d, r, 64 / KL, slices);
-  if (L.blk[kWaveClasses] == 0) return;
-  hipLaunchKernelGGL(k_customize_batch_level<KL>, dim3(L.blk[kWaveClasses]), dim3(1024), 0, s, L, slices,
+  if (L.blk[L.ncls] == 0) return;
+  hipLaunchKernelGGL(k_customize_batch_level<KL>, dim3(L.blk[L.ncls]), dim3(1024), 0, s, L, slices,
                      d.prec, d.batch_order ? d.bpos : nullptr, d.tri_ab, K, d.mp, UD, d.status);
 }
 

@@ -1647,6 +1647,16 @@ int launch_queries(nwcch_dev* h, const int32_t* ud, int64_t ud_stride, const int
     const size_t lds = (size_t)2 * maxlen * sizeof(int32_t);
     if (lds > 160 * 1024) throw std::runtime_error("elimination tree too deep for the LDS query kernel");
     ProfScope ps(d, s, kKQuery);
+    if (d.query_blk && query_blk_lds_bytes(maxlen) <= 64 * 1024) {
+      // blocks of 32 ancestors, one query per workgroup; more wavefronts per
+      // query for small launches (latency), fewer for large ones (throughput)
+      int nws = d.query_blk_waves;
+      if (nws == 0) nws = nq <= 256 ? 8 : (nq <= 4096 ? 4 : 2);
+      launch_query_blk(d, nws, d.query_blk_unroll, reinterpret_cast<const int2*>(ud), ud_stride,
+                       (const long long*)ud_table, qmetric, n_metrics, s_rank, t_rank, nq, order, dist, scanned, s);
+      HIP_OK(hipGetLastError());
+      return NWCCH_OK;
+    }
     // wavefronts (queries) per workgroup: as many as asked for whose labels
     // fit 64 KB of LDS per workgroup
     int W = d.query_wpb;
@@ -1795,6 +1805,24 @@ extern "C" int nwcch_set_tuning(nwcch_dev* h, const char* key, int32_t value) {
     (k == "batch_order" ? h->d.batch_order : h->d.batch_xcd) = value;
     return NWCCH_OK;
   }
+  if (k == "query_blk") {
+    if (value < 0 || value > 1) { set_error("query_blk is 0 or 1"); return NWCCH_ERR_ARG; }
+    h->d.query_blk = value;
+    return NWCCH_OK;
+  }
+  if (k == "query_blk_waves") {
+    if (value != 0 && value != 1 && value != 2 && value != 4 && value != 8) {
+      set_error("query_blk_waves is 0 (by batch size), 1, 2, 4 or 8");
+      return NWCCH_ERR_ARG;
+    }
+    h->d.query_blk_waves = value;
+    return NWCCH_OK;
+  }
+  if (k == "query_blk_unroll") {
+    if (value != 4 && value != 8 && value != 16) { set_error("query_blk_unroll is 4, 8 or 16"); return NWCCH_ERR_ARG; }
+    h->d.query_blk_unroll = value;
+    return NWCCH_OK;
+  }
   if (k == "query_wpb") {
     if (value != 1 && value != 2 && value != 4) { set_error("query_wpb is 1, 2 or 4"); return NWCCH_ERR_ARG; }
     h->d.query_wpb = value;
@@ -1827,6 +1855,9 @@ extern "C" int nwcch_get_tuning(const nwcch_dev* h, const char* key, int32_t* va
   else if (k == "flat_min_tri") *value = h->d.flat_min_tri;
   else if (k == "flat_xcd") *value = h->d.flat_xcd;
   else if (k == "query_wpb") *value = h->d.query_wpb;
+  else if (k == "query_blk") *value = h->d.query_blk;
+  else if (k == "query_blk_waves") *value = h->d.query_blk_waves;
+  else if (k == "query_blk_unroll") *value = h->d.query_blk_unroll;
   else if (k == "customize_graph") *value = h->d.customize_graph;
   else if (k == "flat_store") *value = h->d.flat_store;
   else if (k == "query_deep_max") *value = h->d.query_deep_max;

@@ -87,7 +87,7 @@ struct DevTopo {
   std::vector<i64> lv_tri0;          // [levels+1] host: first triangle of each level
   // tunables (nwcch_set_tuning) and the launch geometry derived from them
   int32_t tri_per_lane = kTriPerLane;        // single-metric level kernel
-  int32_t batch_tri_per_lane = 8;            // batched level kernel
+  int32_t batch_tri_per_lane = 16;           // batched level kernel
   int32_t batch_order = 0;                   // 1: arcs of a wave class in id order (bpos)
   int32_t batch_xcd = 0;                     // 1: each XCD takes a contiguous slice of a class
   int32_t upd_max_blocks = 1024;             // workgroups of a per-height update launch, at most
@@ -96,6 +96,9 @@ struct DevTopo {
   int32_t query_chunk = 0;                   // queries read ancestor chains 32 at a time
   int32_t query_wpb = 1;                     // queries (wavefronts) per workgroup: 1, 2 or 4
                                              // (set at creation for deep hierarchies)
+  int32_t query_blk = 1;                     // blocked query kernel (blocks of 32 ancestors)
+  int32_t query_blk_waves = 0;               // its wavefronts per search side (0: by launch size)
+  int32_t query_blk_unroll = 8;              // its 64-arc chunks in flight per wavefront: 4, 8, 16
   int32_t query_deep_max = 32768;            // query launches up to this size prefetch 4 x 32
                                              // arcs of the next row, larger ones 32
   int32_t flat_min_tri = 0;                  // levels whose heaviest arc has >= this many
@@ -203,6 +206,15 @@ __device__ __forceinline__ i64 row_find(const int32_t* __restrict__ head, i64 lo
   }
   return (lo < end && head[lo] == key) ? lo : -1;
 }
+
+// The blocked ET query (query_blk.hip): LDS bytes per query, and the launch
+// of nq queries with nws wavefronts per search side (1, 2, 4 or 8) and
+// `unroll` (4, 8 or 16) 64-arc chunks in flight per wavefront.
+size_t query_blk_lds_bytes(int32_t maxlen);
+void launch_query_blk(const DevTopo& d, int nws, int unroll, const int2* ud, i64 ud_stride,
+                      const long long* ud_table, const int32_t* qmetric, int32_t n_metrics, const int32_t* s_rank,
+                      const int32_t* t_rank, i64 nq, const int32_t* order, int32_t* dist, int32_t* scanned,
+                      hipStream_t s);
 
 // Down lists of the upward CSR: for every vertex y the arcs (x, y) with x < y,
 // in ascending arc id (so ascending x).

@@ -1,0 +1,323 @@
+// Elimination-tree query in blocks of 32 ancestors (python/cch.py:675-732,
+// python/cch_nb.py:577-615, cpp/cch.cpp:549-575).
+//
+// One query per workgroup: the first half of the workgroup runs the upward
+// search from s, the second half the downward search from t.  Both searches
+// walk the vertex's ancestor chain bottom-up; every arc out of a chain vertex
+// ends at a higher chain vertex, so labels live in LDS indexed by ET depth.
+//
+// The chain is cut into blocks of 32 consecutive ancestors.  Within a block
+// the labels become final in chain order, which is a sequential chain of
+// dependencies; everything else is independent of the labels:
+//   1. the arcs that stay inside the block (a prefix of each row, since rows
+//      are sorted by head rank, i.e. nearest ancestor first) are scattered into
+//      a 32 x 32 (min,+) matrix in LDS;
+//   2. one wavefront settles the block: lane t holds the label of block vertex
+//      t and column t of the matrix in registers, and for j = 0..31 takes the
+//      settled label of vertex j with a lane read and relaxes its own label --
+//      a few instructions per ancestor instead of a memory round trip;
+//   3. all wavefronts of the side relax the arcs of the settled block that
+//      leave it upward, flattened over the rows with a finite label, into the
+//      LDS labels with LDS atomic minima.
+// The next block's chain ids and vertex records are loaded while a block is
+// processed.  A label is relaxed from with its final value, every finite row
+// relaxes all its arcs, and min is order independent, so every distance, the
+// overflow status and `scanned` (the arcs of the rows with a finite label) are
+// those of the vertex-by-vertex sweep.
+#include <hip/hip_runtime.h>
+
+#include "nwcch_dev.h"
+#include "../../include/nwcch.h"
+
+namespace nwcch {
+namespace {
+
+__device__ __forceinline__ int32_t qadd(int32_t a, int32_t b, uint32_t& st) {
+  if (a == kInf || b == kInf) return kInf;
+  const long long s = (long long)a + (long long)b;
+  if (s >= (long long)kInf || s <= -(long long)kInf) {
+    st |= kStatusOverflow;
+    return s > 0 ? kInf - 1 : -kInf + 1;
+  }
+  return (int32_t)s;
+}
+
+constexpr int kBlk = 32;          // ancestors per block
+constexpr int kMStride = kBlk + 1;  // padded matrix rows: lane t reads row t without bank conflicts
+
+// LDS words of one query: labels of both sides, two block matrices, per side
+// the block rows' first arc, degree, settled label and flattened end, and two
+// words for the lowest common ancestor and the scan counts.
+__host__ __device__ inline size_t blk_lds_words(int32_t maxlen) {
+  return (size_t)2 * maxlen + 2 * kBlk * kMStride + 2 * 4 * kBlk + 4;
+}
+
+// NWS wavefronts per search side; U 64-arc chunks per wavefront with their
+// loads in flight at once in step 3.
+template <int NWS, int U>
+__global__ void __launch_bounds__(128 * NWS) k_query_blk(const VInfo* __restrict__ vinfo,
+                                                       const int2* __restrict__ euler,
+                                                       const int32_t* __restrict__ adep,
+                                                       const int32_t* __restrict__ anc,
+                                                       const int2* __restrict__ ud_one, i64 ud_stride,
+                                                       const long long* __restrict__ ud_table,
+                                                       const int32_t* __restrict__ qmetric, int32_t n_metrics,
+                                                       const int32_t* __restrict__ qs,
+                                                       const int32_t* __restrict__ qt, i64 nq,
+                                                       const int32_t* __restrict__ order, int32_t n,
+                                                       int32_t maxlen, int32_t* __restrict__ dist,
+                                                       int32_t* __restrict__ scanned, uint32_t* status) {
+  extern __shared__ int32_t lds[];
+  constexpr int SL = 64 * NWS;  // lanes per side
+  const int tid = threadIdx.x;
+  const int side = tid >= SL ? 1 : 0;  // 0: s, upward arcs; 1: t, downward arcs
+  const int stid = tid - side * SL;
+  const int swave = stid >> 6;
+  const int lane = tid & 63;
+  const i64 q = order ? (i64)order[blockIdx.x] : (i64)blockIdx.x;
+  if (q < 0 || q >= nq) return;
+  const int32_t rs = qs[q], rt = qt[q];
+  const int32_t mq = ud_table ? qmetric[q] : 0;
+  const int2* __restrict__ ud =
+      ud_table ? ((mq >= 0 && mq < n_metrics) ? reinterpret_cast<const int2*>(ud_table[mq]) : nullptr) : ud_one;
+  if (rs < 0 || rs >= n || rt < 0 || rt >= n || !ud) {  // uniform over the workgroup
+    if (tid == 0) {
+      dist[q] = kInf;
+      if (scanned) scanned[q] = 0;
+      atomicOr(status, (uint32_t)kStatusBadQuery);
+    }
+    return;
+  }
+  int32_t* lab = lds + side * maxlen;
+  int32_t* M = lds + 2 * maxlen + side * (kBlk * kMStride);
+  int32_t* rows = lds + 2 * maxlen + 2 * kBlk * kMStride + side * 4 * kBlk;
+  int32_t* rfirst = rows;
+  int32_t* rdeg = rows + kBlk;
+  int32_t* rlab = rows + 2 * kBlk;
+  int32_t* rend = rows + 3 * kBlk;
+  int32_t* misc = lds + 2 * maxlen + 2 * kBlk * kMStride + 2 * 4 * kBlk;
+
+  const int32_t x0 = side ? rt : rs;
+  const VInfo v0 = vinfo[x0];
+  const int32_t D = v0.depth;
+  const int32_t Dmax = max(vinfo[rs].depth, vinfo[rt].depth);
+  const int32_t nb = (D + kBlk) / kBlk;  // blocks of this side: ceil((D + 1) / 32)
+  const int32_t nbmax = (Dmax + kBlk) / kBlk;
+  const int32_t tin_s = euler[rs].x;
+  for (int32_t i = stid; i <= D; i += SL) lab[i] = kInf;
+  uint32_t st = 0;
+  // chain of the current block on lanes 0..31 of the side's first wavefront
+  const bool lead = swave == 0 && lane < kBlk;
+  int32_t cid = -1;
+  VInfo cv{0, 0, -1, 0};
+  bool cin = false;
+  if (lead) {
+    cid = lane == 0 ? x0 : anc[(i64)x0 * 32 + lane - 1];
+    if (cid >= 0) {
+      cv = lane == 0 ? v0 : vinfo[cid];
+      if (side) {
+        const int2 e = euler[cid];
+        cin = e.x <= tin_s && tin_s <= e.y;
+      }
+    }
+  }
+  int32_t lca = -1;  // t side: depth of the lowest common ancestor, once found
+  int32_t nscan = 0;
+  __syncthreads();
+  if (stid == 0) lab[D] = 0;
+
+  for (int32_t b = 0; b < nbmax; b++) {
+    const bool act = b < nb;  // uniform per side
+    const int32_t dlo = D - kBlk * b;             // depth of block position 0
+    const int32_t dtop = max(dlo - (kBlk - 1), 0);  // depth of the block's last position
+    const int32_t bsz = act ? dlo - dtop + 1 : 0;
+    int32_t nid = -1;
+    if (swave == 0 && act) {
+      const int32_t last = __shfl(cid, kBlk - 1, 64);
+      if (lane < kBlk && last >= 0) nid = anc[(i64)last * 32 + lane];
+      if (lead) {
+        rfirst[lane] = cv.first;
+        rdeg[lane] = (lane < bsz && cid >= 0) ? cv.deg : 0;
+      }
+      if (side && lca < 0) {  // the first chain vertex whose preorder interval holds s
+        const unsigned long long m = __ballot(lead && lane < bsz && cin);
+        if (m) lca = dlo - (__ffsll((long long)m) - 1);
+      }
+    }
+    for (int i = stid; i < kBlk * kMStride; i += SL) M[i] = kInf;
+    __syncthreads();
+    // 1. arcs inside the block: among the first bsz - 1 - j arcs of row j; a
+    //    wavefront takes two rows per pass (one per half), all loads first
+    if (act) {
+      constexpr int RPW = kBlk / (2 * NWS);  // row pairs per wavefront
+      const int half = lane >> 5, hl = lane & 31;
+      int32_t ad1[RPW], w1[RPW];
+#pragma unroll
+      for (int r = 0; r < RPW; r++) {
+        const int j = (r * NWS + swave) * 2 + half;
+        ad1[r] = -1;
+        w1[r] = kInf;
+        if (j < bsz && hl < min(rdeg[j], bsz - 1 - j)) {
+          const int32_t a = rfirst[j] + hl;
+          ad1[r] = adep[a];
+          const int2 u = ud[(i64)a * ud_stride];
+          w1[r] = side ? u.y : u.x;
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < RPW; r++) {
+        const int j = (r * NWS + swave) * 2 + half;
+        if (ad1[r] >= dtop) M[(dlo - ad1[r]) * kMStride + j] = w1[r];
+      }
+    }
+    // the next block's vertex records, consumed after this block
+    VInfo nv{0, 0, -1, 0};
+    bool nin = false;
+    if (lead && nid >= 0) {
+      nv = vinfo[nid];
+      if (side) {
+        const int2 e = euler[nid];
+        nin = e.x <= tin_s && tin_s <= e.y;
+      }
+    }
+    __syncthreads();
+    // 2. settle the block in chain order
+    if (swave == 0 && act) {
+      const int t = lane & (kBlk - 1);
+      int32_t L = lane < bsz ? lab[dlo - lane] : kInf;
+      int32_t mc[kBlk];
+#pragma unroll
+      for (int j = 0; j < kBlk; j++) mc[j] = M[t * kMStride + j];
+#pragma unroll
+      for (int j = 0; j < kBlk; j++) {
+        const int32_t lj = __builtin_amdgcn_readlane(L, j);
+        if (j < bsz && lj != kInf) L = min(L, qadd(lj, mc[j], st));
+      }
+      int32_t cnt = (lane < bsz && L != kInf) ? cv.deg : 0;
+      nscan += cnt;
+#pragma unroll
+      for (int o = 1; o < kBlk; o <<= 1) {
+        const int32_t v = __shfl_up(cnt, o, 64);
+        if (t >= o) cnt += v;
+      }
+      if (lane < bsz) lab[dlo - lane] = L;
+      if (lead) {
+        rlab[lane] = L;
+        rend[lane] = cnt;  // inclusive end of row lane in the flattened arcs of finite rows
+      }
+    }
+    __syncthreads();
+    // 3. arcs leaving the block upward, flattened over the finite rows
+    if (act) {
+      const int32_t tot = rend[kBlk - 1];
+      const int32_t e_l = rend[lane & (kBlk - 1)];  // lane r < 32 holds the end of row r
+      for (int32_t c0 = swave * 64; c0 < tot; c0 += SL * U) {
+        int32_t aa[U], lj[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+          const int32_t c = c0 + u * SL;  // uniform chunk start
+          const int32_t idx = c + lane;
+          aa[u] = -1;
+          lj[u] = kInf;
+          if (c < tot) {
+            // rows ending at or before c precede the chunk; rows ending inside it split it
+            const unsigned long long before = __ballot(lane < kBlk && e_l <= c);
+            unsigned long long inside = __ballot(lane < kBlk && e_l > c && e_l < c + 64);
+            int32_t j = __popcll(before);
+            while (inside) {
+              const int r = __ffsll((long long)inside) - 1;
+              inside &= inside - 1;
+              j += idx >= __builtin_amdgcn_readlane(e_l, r) ? 1 : 0;
+            }
+            if (idx < tot) {
+              const int32_t start = j > 0 ? rend[j - 1] : 0;
+              aa[u] = rfirst[j] + (idx - start);
+              lj[u] = rlab[j];
+            }
+          }
+        }
+        int32_t ad[U], w[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+          ad[u] = 0;
+          w[u] = kInf;
+          if (aa[u] >= 0) {
+            ad[u] = adep[aa[u]];
+            const int2 uv = ud[(i64)aa[u] * ud_stride];
+            w[u] = side ? uv.y : uv.x;
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+          if (aa[u] >= 0 && ad[u] < dtop) {
+            const int32_t v = qadd(lj[u], w[u], st);
+            if (v != kInf) atomicMin(&lab[ad[u]], v);
+          }
+        }
+      }
+    }
+    __syncthreads();
+    if (lead) {
+      cid = nid;
+      cv = nv;
+      cin = nin;
+    }
+  }
+  // meet at the common ancestors: depths 0..depth(lca)
+  if (side && stid == 0) misc[0] = lca;
+  if (swave == 0) {
+    int32_t s = nscan;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if (lane == 0) misc[1 + side] = s;
+  }
+  __syncthreads();
+  const int32_t lcad = misc[0];
+  int32_t best = kInf;
+  for (int32_t d = tid; d <= lcad; d += 2 * SL) best = min(best, qadd(lds[d], lds[maxlen + d], st));
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) best = min(best, __shfl_xor(best, o, 64));
+  __syncthreads();
+  if (lane == 0) rows[swave] = best;  // the block matrices and row tables are free now
+  if (st) atomicOr(status, st);
+  __syncthreads();
+  if (tid == 0) {
+    int32_t r = kInf;
+    for (int i = 0; i < 2 * NWS; i++) r = min(r, lds[2 * maxlen + 2 * kBlk * kMStride + (i / NWS) * 4 * kBlk + (i % NWS)]);
+    dist[q] = r;
+    if (scanned) scanned[q] = misc[1] + misc[2];
+  }
+}
+
+}  // namespace
+
+size_t query_blk_lds_bytes(int32_t maxlen) { return blk_lds_words(maxlen) * sizeof(int32_t); }
+
+void launch_query_blk(const DevTopo& d, int nws, int unroll, const int2* ud, i64 ud_stride,
+                      const long long* ud_table, const int32_t* qmetric, int32_t n_metrics, const int32_t* s_rank,
+                      const int32_t* t_rank, i64 nq, const int32_t* order, int32_t* dist, int32_t* scanned,
+                      hipStream_t s) {
+  const int32_t maxlen = d.depth + 1;
+  const size_t lds = query_blk_lds_bytes(maxlen);
+  const dim3 grid((unsigned)nq);
+#define NWCCH_BLK_LAUNCH(NW, U)                                                                                \
+  hipLaunchKernelGGL((k_query_blk<NW, U>), grid, dim3(128 * NW), lds, s, d.vinfo, d.euler, d.adep, d.anc, ud,  \
+                     ud_stride, ud_table, qmetric, n_metrics, s_rank, t_rank, nq, order, (int32_t)d.n, maxlen, \
+                     dist, scanned, d.status)
+#define NWCCH_BLK_UNROLL(NW)                          \
+  switch (unroll) {                                   \
+    case 4: NWCCH_BLK_LAUNCH(NW, 4); break;           \
+    case 16: NWCCH_BLK_LAUNCH(NW, 16); break;         \
+    default: NWCCH_BLK_LAUNCH(NW, 8); break;          \
+  }
+  switch (nws) {
+    case 1: NWCCH_BLK_UNROLL(1); break;
+    case 2: NWCCH_BLK_UNROLL(2); break;
+    case 4: NWCCH_BLK_UNROLL(4); break;
+    default: NWCCH_BLK_UNROLL(8); break;
+  }
+#undef NWCCH_BLK_UNROLL
+#undef NWCCH_BLK_LAUNCH
+}
+
+}  // namespace nwcch

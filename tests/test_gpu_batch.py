@@ -69,7 +69,7 @@ def test_batch_layouts_equal_default(order, xcd):
         for tpl in (4, 16):   # two wave-class shapes: the id order is rebuilt for each
             dt.set_tuning("batch_tri_per_lane", tpl)
             assert torch.equal(dt.customize_batch(W, slab=slab), ref), (slab, tpl)
-        dt.set_tuning("batch_tri_per_lane", 8)
+        dt.set_tuning("batch_tri_per_lane", 16)
         dt.set_tuning("batch_order", 0)
         dt.set_tuning("batch_xcd", 0)
     assert dt.status() == 0

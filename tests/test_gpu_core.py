@@ -101,8 +101,23 @@ def test_road_config1_bit_exact():
     d0, s0, _ = st.query(up, dn, s, t, threads=8)
     assert np.array_equal(dist.cpu().numpy(), d0)
     assert np.array_equal(sc.cpu().numpy().astype(np.int64), s0)
-    # every query kernel (chain walk with deep or shallow prefetch, chunked
-    # ancestors) with 1, 2 or 4 queries per workgroup gives the same answers
+    # every query kernel gives the same answers: the blocked form with 1, 2, 4
+    # or 8 wavefronts per side, in small and large launches, and the chain walk
+    # (deep or shallow prefetch, chunked ancestors) with 1, 2 or 4 queries per
+    # workgroup
+    for waves in (1, 2, 4, 8):
+        dt.set_tuning("query_blk_waves", waves)
+        dist1, sc1 = dt.query(ud, rank[s], rank[t])
+        assert torch.equal(dist1, dist) and torch.equal(sc1, sc), waves
+        d1, s1 = dt.query(ud, rank[s[:7]], rank[t[:7]])
+        assert torch.equal(d1, dist[:7]) and torch.equal(s1, sc[:7]), waves
+    for unroll in (4, 16):
+        dt.set_tuning("query_blk_unroll", unroll)
+        dist1, sc1 = dt.query(ud, rank[s], rank[t])
+        assert torch.equal(dist1, dist) and torch.equal(sc1, sc), unroll
+    dt.set_tuning("query_blk_unroll", 8)
+    dt.set_tuning("query_blk_waves", 0)
+    dt.set_tuning("query_blk", 0)
     chunk0 = dt.get_tuning("query_chunk")
     for chunk in (0, 1):
         for deep in (32768, 0):
@@ -115,6 +130,7 @@ def test_road_config1_bit_exact():
     dt.set_tuning("query_chunk", chunk0)
     dt.set_tuning("query_deep_max", 32768)
     dt.set_tuning("query_wpb", 1)
+    dt.set_tuning("query_blk", 1)
     p, neg = O.johnson(g.n, g.off, g.tgt, g.w)
     assert not neg
     dj = O.Dijkstra(g.n, g.off, g.tgt, g.w, p)

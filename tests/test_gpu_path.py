@@ -128,7 +128,8 @@ def test_path_limit_on_negative_cycle():
     assert stopped > 0 and dt.status() & 16
 
 
-@pytest.mark.parametrize("form", ["chain_walk_throughput", "chain_walk_latency", "chunked_latency"])
+@pytest.mark.parametrize("form", ["chain_walk_throughput", "chain_walk_latency", "chunked_latency",
+                                  "blocked_1", "blocked_2", "blocked_4", "blocked_8"])
 @pytest.mark.parametrize("name", ["grid", "small0", "small1", "small2"])
 def test_query_scanned_equals_reference_python(name, form):
     """Every query kernel form against the distances and scanned-arc counts the
@@ -141,10 +142,15 @@ def test_query_scanned_equals_reference_python(name, form):
     topo = _lib.Topology.from_graph(n, off, tgt, rank.astype(np.int32))
     dt = DeviceTopology(topo)
     ud = dt.customize(w.astype(np.int32))
-    chunk, deep = {"chain_walk_throughput": (0, 0), "chain_walk_latency": (0, 1 << 30),
-                   "chunked_latency": (1, 1 << 30)}[form]
-    dt.set_tuning("query_chunk", chunk)
-    dt.set_tuning("query_deep_max", deep)
+    if form.startswith("blocked"):
+        dt.set_tuning("query_blk", 1)
+        dt.set_tuning("query_blk_waves", int(form.split("_")[1]))
+    else:
+        chunk, deep = {"chain_walk_throughput": (0, 0), "chain_walk_latency": (0, 1 << 30),
+                       "chunked_latency": (1, 1 << 30)}[form]
+        dt.set_tuning("query_blk", 0)
+        dt.set_tuning("query_chunk", chunk)
+        dt.set_tuning("query_deep_max", deep)
     qs, qt = rank[G[f"{name}/q_s"]], rank[G[f"{name}/q_t"]]
     dist, sc = dt.query(ud, qs, qt)
     assert np.array_equal(dist.cpu().numpy().astype(np.int64), G[f"{name}/q_dist"])
