@@ -880,31 +880,47 @@ def headline(args, out):
         qt = torch.as_tensor(rank[qt_h]).to(dt.dev)
         dist = torch.empty(nq, dtype=torch.int32, device=dt.dev)
         order = dt.chain_order(qs, qt)
-        forms = {}
-        for name, chunk, deep in (("chain_walk_throughput", 0, 0), ("chunked_latency", 1, 1 << 30),
-                                  ("chain_walk_latency", 0, 1 << 30)):
+        # query kernel forms: (query_blk, wavefronts per side, query_chunk, query_deep_max)
+        kinds = {"blocked_4": (1, 4, 0, 1 << 30), "blocked_2": (1, 2, 0, 1 << 30),
+                 "chunked_latency": (0, 0, 1, 1 << 30), "chain_walk_throughput": (0, 0, 0, 0)}
+
+        def use(name):
+            blk, waves, chunk, deep = kinds[name]
+            dt.set_tuning("query_blk", blk)
+            dt.set_tuning("query_blk_waves", waves)
             dt.set_tuning("query_chunk", chunk)
             dt.set_tuning("query_deep_max", deep)
+        forms = {}
+        for name in kinds:
+            use(name)
             dt.query(ud, qs, qt, want_scanned=False, dist=dist, order=order)
             ms = statistics.median(event_ms(lambda: dt.query(ud, qs, qt, want_scanned=False, dist=dist,
                                                              order=order), 2))
+            forms[name] = {"batch_ms": ms, "queries_per_s": nq / (ms / 1e3)}
+            log(f"config {args.graph} queries, {name}: {nq} in {ms:.1f} ms")
+        # one query per launch: the blocked form with 8 wavefronts per side (its
+        # default for launches of up to 256 queries) and the older chunked form
+        singles = {}
+        for name, blk, chunk in (("blocked_8", 1, 0), ("chunked_latency", 0, 1)):
+            dt.set_tuning("query_blk", blk)
+            dt.set_tuning("query_blk_waves", 0)
+            dt.set_tuning("query_chunk", chunk)
+            dt.set_tuning("query_deep_max", 1 << 30)
             one = [event_ms(lambda i=i: dt.query(ud, qs[i:i + 1], qt[i:i + 1], want_scanned=False), 1)[0]
                    for i in range(30)]
-            forms[name] = {"batch_ms": ms, "queries_per_s": nq / (ms / 1e3), "single_p50_ms": float(np.median(one))}
-            log(f"config {args.graph} queries, {name}: {nq} in {ms:.1f} ms; single p50 {np.median(one):.2f} ms")
+            singles[name] = float(np.median(one))
+            log(f"config {args.graph} single query, {name}: p50 {np.median(one):.2f} ms")
         best_b = min(forms, key=lambda k: forms[k]["batch_ms"])
-        best_s = min(forms, key=lambda k: forms[k]["single_p50_ms"])
-
-        def use(name):
-            dt.set_tuning("query_chunk", 1 if name == "chunked_latency" else 0)
-            dt.set_tuning("query_deep_max", 0 if name == "chain_walk_throughput" else 1 << 30)
+        best_s = min(singles, key=lambda k: singles[k])
         use(best_b)
         dist_full, scanned = dt.query(ud, qs, qt, order=order)
         qms = forms[best_b]["batch_ms"]
         vdep = topo.export()["depth"].astype(np.int64)
         chain = vdep[rank[qs_h]] + vdep[rank[qt_h]] + 2
         qbytes = 8 * int(scanned.to(torch.int64).sum().item()) + 12 * int(chain.sum()) + 12 * nq
-        use(best_s)
+        dt.set_tuning("query_blk", 1 if best_s == "blocked_8" else 0)
+        dt.set_tuning("query_blk_waves", 0)
+        dt.set_tuning("query_chunk", 0 if best_s == "blocked_8" else 1)
         single = [event_ms(lambda i=i: dt.query(ud, qs[i:i + 1], qt[i:i + 1], want_scanned=False), 1)[0]
                   for i in range(args.single_full)]
         dt.check_status()
@@ -915,7 +931,8 @@ def headline(args, out):
                                 "what": "8 B per scanned arc + 12 B per chain vertex + 12 B per query"},
                    "single_kernel": best_s, "single_launches": len(single),
                    "single_p50_ms": float(np.percentile(single, 50)),
-                   "single_p99_ms": float(np.percentile(single, 99)), "forms": forms}
+                   "single_p99_ms": float(np.percentile(single, 99)), "forms": forms,
+                   "single_p50_by_form_ms": singles}
         log(f"config {args.graph} queries: {nq} in {qms:.1f} ms ({nq / (qms / 1e3):.3g} q/s); single p50 "
             f"{np.percentile(single, 50):.2f} ms p99 {np.percentile(single, 99):.2f} ms")
 

@@ -63,6 +63,64 @@ __global__ void __launch_bounds__(256) k_transpose_w(const int32_t* __restrict__
   }
 }
 
+// The same transpose for K <= 8 metrics: one thread per original arc reads its
+// K weights (each a coalesced stream over the arcs) and writes them as K
+// contiguous words.  The tiled transpose would leave 56 of 64 tile rows idle.
+__global__ void __launch_bounds__(256) k_transpose_w_small(const int32_t* __restrict__ W, i64 m, int32_t K,
+                                                           int32_t* __restrict__ WT) {
+  const i64 stride = (i64)gridDim.x * blockDim.x;
+  for (i64 e = (i64)blockIdx.x * blockDim.x + threadIdx.x; e < m; e += stride) {
+    int32_t v[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) v[k] = k < K ? W[(i64)k * m + e] : 0;
+#pragma unroll
+    for (int k = 0; k < 8; k++)
+      if (k < K) WT[e * K + k] = v[k];
+  }
+}
+
+// Base values for slab widths SW <= 8: one thread per arc and slab, the SW
+// values of an original arc read as one contiguous run of WT and the SW
+// (up, down) pairs written as one contiguous run of UD; a grid-stride loop, so
+// every thread keeps several arcs' dependent loads (offsets, arc list,
+// weights) going instead of one.
+template <int SW>
+__global__ void __launch_bounds__(256) k_base_batch_small(i64 mp, int32_t K, int32_t slabs,
+                                                          const int32_t* __restrict__ ae_off,
+                                                          const int32_t* __restrict__ ae_idx,
+                                                          const int32_t* __restrict__ WT, int2* UD,
+                                                          uint32_t* status) {
+  uint32_t st = 0;
+  const i64 stride = (i64)gridDim.x * blockDim.x;
+  for (i64 c = (i64)blockIdx.x * blockDim.x + threadIdx.x; c < mp; c += stride) {
+    const int32_t e0 = ae_off[c], e1 = ae_off[c + 1];
+    for (int32_t sl = 0; sl < slabs; sl++) {
+      int32_t up[SW], dn[SW];
+#pragma unroll
+      for (int km = 0; km < SW; km++) {
+        up[km] = kInf;
+        dn[km] = kInf;
+      }
+      for (int32_t j = e0; j < e1; j++) {
+        const int32_t e = ae_idx[j];
+        const int32_t* w = WT + (i64)(e >> 1) * K + sl * SW;
+#pragma unroll
+        for (int km = 0; km < SW; km++) {
+          if (sl * SW + km >= K) break;
+          const int32_t x = w[km];
+          if (x == (int32_t)0x80000000) st |= kStatusBadWeight;
+          if (e & 1) up[km] = min(up[km], x);
+          else dn[km] = min(dn[km], x);
+        }
+      }
+      int2* o = UD + ((i64)sl * mp + c) * SW;
+#pragma unroll
+      for (int km = 0; km < SW; km++) o[km] = make_int2(up[km], dn[km]);
+    }
+  }
+  if (st) atomicOr(status, st);
+}
+
 // Base values of every (arc, metric) in the slab layout: UD[slab][c][km] = min
 // over the original arcs of c, per direction (python/cch_nb.py:534-542 per
 // metric).  A block holds 256 / SW arcs x SW metrics, metric fastest, so reads
@@ -377,13 +435,34 @@ extern "C" int nwcch_customize_batch(nwcch_dev* h, const int32_t* W, int32_t K, 
     }
     {
       ProfScope ps(d, s, kKOther);
-      dim3 grid((unsigned)((d.m + 63) / 64), (unsigned)((K + 63) / 64));
-      if (d.m > 0) hipLaunchKernelGGL(k_transpose_w, grid, dim3(256), 0, s, W, d.m, K, d.wt);
-      int32_t sw_log2 = 0;
-      while ((1 << sw_log2) < KL) sw_log2++;
-      const i64 blocks = std::max<i64>((d.mp + (256 >> sw_log2) - 1) / (256 >> sw_log2), 1);
-      hipLaunchKernelGGL(k_base_batch, dim3((unsigned)blocks), dim3(256), 0, s, d.mp, K, sw_log2, slices, d.ae_off,
-                         d.ae_idx, d.wt, ud, d.status);
+      if (d.m > 0) {
+        if (K <= 8) {
+          const i64 blocks = std::min<i64>((d.m + 255) / 256, 65536);
+          hipLaunchKernelGGL(k_transpose_w_small, dim3((unsigned)blocks), dim3(256), 0, s, W, d.m, K, d.wt);
+        } else {
+          dim3 grid((unsigned)((d.m + 63) / 64), (unsigned)((K + 63) / 64));
+          hipLaunchKernelGGL(k_transpose_w, grid, dim3(256), 0, s, W, d.m, K, d.wt);
+        }
+      }
+      if (KL <= 8) {
+        const i64 blocks = std::max<i64>(std::min<i64>((d.mp + 255) / 256, 65536), 1);
+#define NWCCH_BASE_SMALL(SWV)                                                                                 \
+  hipLaunchKernelGGL(k_base_batch_small<SWV>, dim3((unsigned)blocks), dim3(256), 0, s, d.mp, K, slices, d.ae_off, \
+                     d.ae_idx, d.wt, ud, d.status)
+        switch (KL) {
+          case 1: NWCCH_BASE_SMALL(1); break;
+          case 2: NWCCH_BASE_SMALL(2); break;
+          case 4: NWCCH_BASE_SMALL(4); break;
+          default: NWCCH_BASE_SMALL(8); break;
+        }
+#undef NWCCH_BASE_SMALL
+      } else {
+        int32_t sw_log2 = 0;
+        while ((1 << sw_log2) < KL) sw_log2++;
+        const i64 blocks = std::max<i64>((d.mp + (256 >> sw_log2) - 1) / (256 >> sw_log2), 1);
+        hipLaunchKernelGGL(k_base_batch, dim3((unsigned)blocks), dim3(256), 0, s, d.mp, K, sw_log2, slices,
+                           d.ae_off, d.ae_idx, d.wt, ud, d.status);
+      }
     }
     if (d.batch_order) ensure_bpos(d, 64 / KL);
     for (const LevelRange& r : d.lv) {

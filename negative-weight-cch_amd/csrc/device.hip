@@ -1168,6 +1168,10 @@ void drop_cust_graphs(DevTopo& d);
 
 }  // namespace
 
+void* nwcch::dev_upload_bytes(DevTopo& d, const void* host, size_t bytes) {
+  return dev_upload(d, static_cast<const unsigned char*>(host), bytes);
+}
+
 void nwcch::down_csr(const Topology& t, std::vector<i64>& off, std::vector<int32_t>& arc) {
   off.assign(t.n + 1, 0);
   for (i64 a = 0; a < t.mp; a++) off[t.head[a] + 1]++;
@@ -1355,6 +1359,7 @@ extern "C" int nwcch_dev_create(const nwcch_topo* topo, int device, nwcch_dev** 
     std::vector<int32_t> adep(t.mp);
     for (i64 a = 0; a < t.mp; a++) adep[a] = t.vdepth[t.head[a]];
     d.adep = dev_upload(d, adep.data(), adep.size());
+    build_rows(d, t);
     d.arc_pos = dev_upload(d, t.arc_pos.data(), t.arc_pos.size());
     d.status = dev_alloc<uint32_t>(d, 1);
     HIP_OK(hipMemset(d.status, 0, sizeof(uint32_t)));
@@ -1445,6 +1450,10 @@ void launch_customize(DevTopo& d, const SlotPtrs* P, hipStream_t s) {
     const LevelRange& r = d.lv[i];
     if (r.end == r.begin || r.triangles == 0) continue;  // base values are final
     ProfScope ps(d, s, kKCustomize);
+    if (rows_level_enabled(d, i)) {
+      launch_rows_level(d, i, P, s);
+      continue;
+    }
     if (d.pos_cnt[r.begin] >= d.flat_min_tri) {  // positions of a level: heaviest first
       const i64 q0 = d.lv_tri0[i], nt = d.lv_tri0[i + 1] - q0;
       const i64 nb = (nt + 255) / 256;
@@ -1805,6 +1814,16 @@ extern "C" int nwcch_set_tuning(nwcch_dev* h, const char* key, int32_t value) {
     (k == "batch_order" ? h->d.batch_order : h->d.batch_xcd) = value;
     return NWCCH_OK;
   }
+  if (k == "rows_min_len") {
+    if (value < 0 || value > 65) { set_error("rows_min_len is 0..65 (65: never)"); return NWCCH_ERR_ARG; }
+    h->d.rows_min_len = value;
+    return NWCCH_OK;
+  }
+  if (k == "rows_min_rows" || k == "rows_max_rows") {
+    if (value < 0) { set_error(k + " must be >= 0"); return NWCCH_ERR_ARG; }
+    (k == "rows_min_rows" ? h->d.rows_min_rows : h->d.rows_max_rows) = value;
+    return NWCCH_OK;
+  }
   if (k == "query_blk") {
     if (value < 0 || value > 1) { set_error("query_blk is 0 or 1"); return NWCCH_ERR_ARG; }
     h->d.query_blk = value;
@@ -1856,6 +1875,9 @@ extern "C" int nwcch_get_tuning(const nwcch_dev* h, const char* key, int32_t* va
   else if (k == "flat_xcd") *value = h->d.flat_xcd;
   else if (k == "query_wpb") *value = h->d.query_wpb;
   else if (k == "query_blk") *value = h->d.query_blk;
+  else if (k == "rows_min_len") *value = h->d.rows_min_len;
+  else if (k == "rows_min_rows") *value = h->d.rows_min_rows;
+  else if (k == "rows_max_rows") *value = h->d.rows_max_rows;
   else if (k == "query_blk_waves") *value = h->d.query_blk_waves;
   else if (k == "query_blk_unroll") *value = h->d.query_blk_unroll;
   else if (k == "customize_graph") *value = h->d.customize_graph;

@@ -107,12 +107,25 @@ struct DevTopo {
                                              // store arcs that lie in one wavefront (2: non-temporal)
   int32_t flat_xcd = 0;                      // triangle-parallel levels: one contiguous
                                              // slice of the level per XCD (else interleaved)
+  int32_t rows_min_len = 16;                 // levels whose suffix pieces average at least this
+                                             // many arcs run in row-merge form (65: never) ...
+  int32_t rows_min_rows = 3;                 // ... if they have this many rows with triangles
+  int32_t rows_max_rows = 200;               // ... and at most this many
   std::vector<LevelLaunch> launches;         // per level, for tri_per_lane
   int32_t launches_for = -1;
   int32_t* wt = nullptr;             // batched weights transposed to [m][K] (scratch)
   int32_t* bpos = nullptr;           // batched sweep: schedule positions, id order inside each wave class
   long long bpos_key = -1;           // (sub-lanes, triangles per lane) bpos was built for
   i64 wt_cap = 0;
+  // row-merge form of the level sweep (rows.hip): work items {row, first
+  // piece, pieces, whole row} by level, suffix pieces {a, (k << 7) | len}
+  int4* rows_items = nullptr;
+  int2* rows_pieces = nullptr;
+  std::vector<i64> rows_it_off;      // [levels+1] items of each level
+  std::vector<i64> rows_pieces_lv;   // [levels] pieces of each level
+  std::vector<i64> rows_tri_lv;      // [levels] triangles of each level
+  std::vector<i64> rows_n_lv;        // [levels] rows with triangles in each level
+  size_t rows_lds = 0;               // LDS bytes of a row-merge workgroup
   // query
   VInfo* vinfo = nullptr;            // [n]
   int2* euler = nullptr;             // [n] preorder interval of the ET subtree
@@ -215,6 +228,15 @@ void launch_query_blk(const DevTopo& d, int nws, int unroll, const int2* ud, i64
                       const long long* ud_table, const int32_t* qmetric, int32_t n_metrics, const int32_t* s_rank,
                       const int32_t* t_rank, i64 nq, const int32_t* order, int32_t* dist, int32_t* scanned,
                       hipStream_t s);
+
+// A device copy of `bytes` host bytes owned by d.
+void* dev_upload_bytes(DevTopo& d, const void* host, size_t bytes);
+
+// The row-merge level sweep (rows.hip): its index built once per handle from
+// the host topology, whether level h runs in that form, and its launch.
+void build_rows(DevTopo& d, const Topology& t);
+bool rows_level_enabled(const DevTopo& d, size_t h);
+void launch_rows_level(const DevTopo& d, size_t h, const SlotPtrs* P, hipStream_t s);
 
 // Down lists of the upward CSR: for every vertex y the arcs (x, y) with x < y,
 // in ascending arc id (so ascending x).

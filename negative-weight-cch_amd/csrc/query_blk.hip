@@ -42,6 +42,31 @@ __device__ __forceinline__ int32_t qadd(int32_t a, int32_t b, uint32_t& st) {
   return (int32_t)s;
 }
 
+// The metric reached through the per-launch table is a plain pointer whose
+// address space the compiler cannot see; loads through it would be flat
+// loads, which share a counter with LDS traffic and force a wait after each.
+// Metric arrays always live in device global memory, so the device pass
+// loads them through a global-address-space pointer (the host pass, which
+// only checks the kernel body, sees the plain load).
+__device__ __forceinline__ int2 load_pair(const int2* p, i64 i) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  typedef const int2 __attribute__((address_space(1))) gint2;
+  return ((const gint2*)p)[i];
+#else
+  return p[i];
+#endif
+}
+
+// Workgroup barrier that orders LDS only.  __syncthreads() is a release fence
+// on all memory, which waits for every global load in flight -- the next
+// block's prefetches included.  Threads here share data through LDS alone
+// (global memory is only read), so the barrier waits for LDS operations only.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 constexpr int kBlk = 32;          // ancestors per block
 constexpr int kMStride = kBlk + 1;  // padded matrix rows: lane t reads row t without bank conflicts
 
@@ -78,9 +103,9 @@ __global__ void __launch_bounds__(128 * NWS) k_query_blk(const VInfo* __restrict
   if (q < 0 || q >= nq) return;
   const int32_t rs = qs[q], rt = qt[q];
   const int32_t mq = ud_table ? qmetric[q] : 0;
-  const int2* __restrict__ ud =
+  const int2* ud_p =
       ud_table ? ((mq >= 0 && mq < n_metrics) ? reinterpret_cast<const int2*>(ud_table[mq]) : nullptr) : ud_one;
-  if (rs < 0 || rs >= n || rt < 0 || rt >= n || !ud) {  // uniform over the workgroup
+  if (rs < 0 || rs >= n || rt < 0 || rt >= n || !ud_p) {  // uniform over the workgroup
     if (tid == 0) {
       dist[q] = kInf;
       if (scanned) scanned[q] = 0;
@@ -88,6 +113,7 @@ __global__ void __launch_bounds__(128 * NWS) k_query_blk(const VInfo* __restrict
     }
     return;
   }
+  const int2* ud = ud_p;
   int32_t* lab = lds + side * maxlen;
   int32_t* M = lds + 2 * maxlen + side * (kBlk * kMStride);
   int32_t* rows = lds + 2 * maxlen + 2 * kBlk * kMStride + side * 4 * kBlk;
@@ -145,7 +171,7 @@ __global__ void __launch_bounds__(128 * NWS) k_query_blk(const VInfo* __restrict
       }
     }
     for (int i = stid; i < kBlk * kMStride; i += SL) M[i] = kInf;
-    __syncthreads();
+    lds_barrier();
     // 1. arcs inside the block: among the first bsz - 1 - j arcs of row j; a
     //    wavefront takes two rows per pass (one per half), all loads first
     if (act) {
@@ -153,34 +179,29 @@ __global__ void __launch_bounds__(128 * NWS) k_query_blk(const VInfo* __restrict
       const int half = lane >> 5, hl = lane & 31;
       int32_t ad1[RPW], w1[RPW];
 #pragma unroll
-      for (int r = 0; r < RPW; r++) {
+      for (int r = 0; r < RPW; r++) {  // loads on a valid arc for every lane: no branch between them
         const int j = (r * NWS + swave) * 2 + half;
-        ad1[r] = -1;
-        w1[r] = kInf;
-        if (j < bsz && hl < min(rdeg[j], bsz - 1 - j)) {
-          const int32_t a = rfirst[j] + hl;
-          ad1[r] = adep[a];
-          const int2 u = ud[(i64)a * ud_stride];
-          w1[r] = side ? u.y : u.x;
-        }
+        const bool ok = j < bsz && hl < min(rdeg[j], bsz - 1 - j);
+        const int32_t a = ok ? rfirst[j] + hl : 0;
+        ad1[r] = adep[a];
+        const int2 u = load_pair(ud, (i64)a * ud_stride);
+        w1[r] = side ? u.y : u.x;
+        if (!ok) ad1[r] = -1;
       }
 #pragma unroll
-      for (int r = 0; r < RPW; r++) {
+      for (int r = 0; r < RPW; r++) {  // arcs leaving the block go to the unread pad word M[31][32]
         const int j = (r * NWS + swave) * 2 + half;
-        if (ad1[r] >= dtop) M[(dlo - ad1[r]) * kMStride + j] = w1[r];
+        M[ad1[r] >= dtop ? (dlo - ad1[r]) * kMStride + j : kBlk * kMStride - 1] = w1[r];
       }
     }
     // the next block's vertex records, consumed after this block
     VInfo nv{0, 0, -1, 0};
-    bool nin = false;
+    int2 ne = make_int2(1, 0);  // an empty interval
     if (lead && nid >= 0) {
       nv = vinfo[nid];
-      if (side) {
-        const int2 e = euler[nid];
-        nin = e.x <= tin_s && tin_s <= e.y;
-      }
+      if (side) ne = euler[nid];
     }
-    __syncthreads();
+    lds_barrier();
     // 2. settle the block in chain order
     if (swave == 0 && act) {
       const int t = lane & (kBlk - 1);
@@ -206,61 +227,63 @@ __global__ void __launch_bounds__(128 * NWS) k_query_blk(const VInfo* __restrict
         rend[lane] = cnt;  // inclusive end of row lane in the flattened arcs of finite rows
       }
     }
-    __syncthreads();
+    lds_barrier();
     // 3. arcs leaving the block upward, flattened over the finite rows
     if (act) {
       const int32_t tot = rend[kBlk - 1];
       const int32_t e_l = rend[lane & (kBlk - 1)];  // lane r < 32 holds the end of row r
       for (int32_t c0 = swave * 64; c0 < tot; c0 += SL * U) {
-        int32_t aa[U], lj[U];
+        // the row of each lane's arc in every chunk: rows ending at or before
+        // the chunk start precede it, rows ending inside it split it
+        int32_t jj[U];
 #pragma unroll
         for (int u = 0; u < U; u++) {
           const int32_t c = c0 + u * SL;  // uniform chunk start
-          const int32_t idx = c + lane;
-          aa[u] = -1;
-          lj[u] = kInf;
+          int32_t j = kBlk - 1;
           if (c < tot) {
-            // rows ending at or before c precede the chunk; rows ending inside it split it
             const unsigned long long before = __ballot(lane < kBlk && e_l <= c);
             unsigned long long inside = __ballot(lane < kBlk && e_l > c && e_l < c + 64);
-            int32_t j = __popcll(before);
+            j = __popcll(before);
             while (inside) {
               const int r = __ffsll((long long)inside) - 1;
               inside &= inside - 1;
-              j += idx >= __builtin_amdgcn_readlane(e_l, r) ? 1 : 0;
+              j += c + lane >= __builtin_amdgcn_readlane(e_l, r) ? 1 : 0;
             }
-            if (idx < tot) {
-              const int32_t start = j > 0 ? rend[j - 1] : 0;
-              aa[u] = rfirst[j] + (idx - start);
-              lj[u] = rlab[j];
-            }
+            j = min(j, kBlk - 1);
           }
+          jj[u] = j;
         }
+        int32_t aa[U], lj[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+          const int32_t idx = c0 + u * SL + lane;
+          const int32_t start = jj[u] > 0 ? rend[jj[u] - 1] : 0;
+          const int32_t f = rfirst[jj[u]];
+          lj[u] = rlab[jj[u]];
+          aa[u] = idx < tot ? f + (idx - start) : -1;
+        }
+        // all loads in flight at once, on a valid arc for every lane
         int32_t ad[U], w[U];
 #pragma unroll
         for (int u = 0; u < U; u++) {
-          ad[u] = 0;
-          w[u] = kInf;
-          if (aa[u] >= 0) {
-            ad[u] = adep[aa[u]];
-            const int2 uv = ud[(i64)aa[u] * ud_stride];
-            w[u] = side ? uv.y : uv.x;
-          }
+          const int32_t a = aa[u] >= 0 ? aa[u] : 0;
+          ad[u] = adep[a];
+          const int2 uv = load_pair(ud, (i64)a * ud_stride);
+          w[u] = side ? uv.y : uv.x;
         }
 #pragma unroll
-        for (int u = 0; u < U; u++) {
-          if (aa[u] >= 0 && ad[u] < dtop) {
-            const int32_t v = qadd(lj[u], w[u], st);
-            if (v != kInf) atomicMin(&lab[ad[u]], v);
-          }
+        for (int u = 0; u < U; u++) {  // INF + w is INF: lanes with nothing to relax add nothing
+          const bool ok = aa[u] >= 0 && ad[u] < dtop;
+          const int32_t v = qadd(ok ? lj[u] : kInf, w[u], st);
+          if (v != kInf) atomicMin(&lab[ad[u]], v);
         }
       }
     }
-    __syncthreads();
+    lds_barrier();
     if (lead) {
       cid = nid;
       cv = nv;
-      cin = nin;
+      cin = ne.x <= tin_s && tin_s <= ne.y;
     }
   }
   // meet at the common ancestors: depths 0..depth(lca)

@@ -25,7 +25,7 @@ HIP_FLAGS = [
 ]
 HOST_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-march=x86-64-v2"]
 
-SOURCES_HIP = ["device.hip", "update.hip", "batch.hip", "symbolic.hip", "query_blk.hip"]
+SOURCES_HIP = ["device.hip", "update.hip", "batch.hip", "symbolic.hip", "query_blk.hip", "rows.hip"]
 SOURCES_CPP = ["capi.cpp", "host_prep.cpp", "contract_host.cpp"]
 
 

@@ -249,3 +249,37 @@ def test_blocked_query_equals_chain_sweep():
                 d, s = _blocked_query(st.first, st.head, st.parent, st.vdepth, up, dn,
                                       int(rank[qs[i]]), int(rank[qt[i]]), B)
                 assert (d, s) == (int(d0[i]), int(s0[i])), (gi, B, i)
+
+
+def _row_merge_triangles(first, head):
+    """The triangles the row-merge sweep (rows.hip) forms: for every arc
+    a = (x, y), the arcs b after a in row x, in pieces of at most 64, each
+    improving the arc c = (y, head b) of row y.  Returns (c, a, b) sorted."""
+    first = np.asarray(first, np.int64)
+    head = np.asarray(head, np.int64)
+    n = len(first) - 1
+    tail = np.repeat(np.arange(n), np.diff(first))
+    key = {(int(tail[c]), int(head[c])): c for c in range(len(head))}
+    out = []
+    for a in range(len(head)):
+        x, y = int(tail[a]), int(head[a])
+        length = int(first[x + 1]) - a - 1
+        for k in range(0, length, 64):
+            for lane in range(min(64, length - k)):
+                b = a + 1 + k + lane
+                out.append((key[(y, int(head[b]))], a, b))
+    return sorted(out)
+
+
+def test_row_merge_enumeration_is_the_lower_triangle_set():
+    """Every lower triangle (x; y, z) is formed exactly once by the suffix
+    pieces of the row-merge sweep, with the same operand arcs as the stored
+    triangle list (python/cch_nb.py:543-562 enumerates x's pairs the same way)."""
+    for g in small_graphs()[::4]:
+        rank = order_of(g)
+        topo = _lib.Topology.from_graph(g.n, g.off, g.tgt, rank)
+        ex = topo.export()
+        sched, toff, tab = topo.export_triangles()
+        c = np.repeat(sched.astype(np.int64), np.diff(toff))
+        want = sorted(zip(c.tolist(), tab[:, 0].tolist(), tab[:, 1].tolist()))
+        assert _row_merge_triangles(ex["first"], ex["head"]) == want

@@ -286,3 +286,24 @@ def test_customize_graph_paths(c0):
     assert torch.equal(dt.customize(wd[1]), outs[1])
     dt.set_tuning("customize_graph", 1)
     assert dt.status() == 0
+
+
+@pytest.mark.parametrize("min_len", [0, 16, 40])
+def test_row_merge_levels(min_len):
+    """Levels in row-merge form (every level, or those whose suffix pieces
+    average at least 16 / 40 arcs) give the oracle's customization on config 1
+    and on the small family; so do plain calls replayed from the library's
+    captured launch sequence."""
+    graphs_ = [(graphs.make_config(1), 64)] + [(g, 8) for g in small_graphs()[::3]]
+    for g, leaf in graphs_:
+        print(f"case: row-merge levels, min_len {min_len}, {g.name}", flush=True)
+        rank = _lib.order(g.n, g.off, g.tgt, g.x, g.y, leaf)
+        dt, st = _setup(g, rank)
+        dt.set_tuning("rows_min_len", min_len)
+        dt.set_tuning("rows_min_rows", 0)
+        dt.set_tuning("rows_max_rows", 1 << 30)
+        for _ in range(2):
+            _check_customize(dt, st, g.w)
+        W, _ = graphs.metric_family(g, 2, seed=5, hmax=800) if g.name.startswith("c") else (None, None)
+        if W is not None:
+            _check_customize(dt, st, W[1])
