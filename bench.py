@@ -775,6 +775,61 @@ def config4_inputs(args, progress):
     return g, rank, W, chk, host
 
 
+def counter_run(args, out):
+    """A short config-4 run for hardware-counter passes (rocprofv3 --pmc): the
+    graph, its order and K = 8 metrics, then 2 batched customizations (and 2
+    more with the arcs of each wave class in id order and one slice per XCD), 2
+    single-metric ones, 2 launches of 20,000 queries on each query form, and
+    one device copy of --copy-mib MiB to scale the byte counters.  No CPU
+    baseline, no parity check, no timing loop."""
+    import torch
+
+    from nwcch import _lib, graphs
+    from nwcch.device import DeviceTopology
+
+    os.environ["NWCCH_PROGRESS"] = "1"
+    args.no_check = True
+    g, rank, W, _, host = config4_inputs(args, lambda msg: log(f"config {args.graph}: {msg}"))
+    K = W.shape[0]
+    topo = _lib.Topology.from_graph(g.n, g.off, g.tgt, rank, device_index=0, device_symbolic=0)
+    dt = DeviceTopology(topo, 0)
+    Wd = dt.weights_batch(W)
+    UD = torch.empty(dt.batch_shape(K), dtype=torch.int32, device=dt.dev)
+    for _ in range(2):
+        dt.customize_batch(Wd, out=UD)
+    # the same sweep with the arcs of each wave class in id order, one
+    # contiguous slice of each class per XCD (rows of one tail on one XCD)
+    dt.set_tuning("batch_order", 1)
+    dt.set_tuning("batch_xcd", 1)
+    for _ in range(2):
+        dt.customize_batch(Wd, out=UD)
+    dt.set_tuning("batch_order", 0)
+    dt.set_tuning("batch_xcd", 0)
+    w0 = dt.weights(W[0])
+    ud = torch.empty((topo.mp, 2), dtype=torch.int32, device=dt.dev)
+    for _ in range(2):
+        dt.customize(w0, out=ud)
+    nq = 20000
+    qs_h, qt_h = graphs.random_pairs(g.n, nq, seed=41)
+    qs = torch.as_tensor(rank[qs_h]).to(dt.dev)
+    qt = torch.as_tensor(rank[qt_h]).to(dt.dev)
+    up, dn = dt.split(ud)
+    dt.set_tuning("query_blk_waves", 2)
+    for _ in range(2):
+        dt.query(ud, qs, qt, want_scanned=False)
+        dt.query_split(up, dn, qs, qt, want_scanned=False)
+    dt.set_tuning("query_blk_waves", 0)
+    if args.copy_mib:
+        src = torch.ones(args.copy_mib << 18, dtype=torch.int32, device=dt.dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+    dt.check_status()
+    torch.cuda.synchronize()
+    out.update({"config": {"workload": "counter run: config 4, 2 batched (K=8) + 2 single customizations, "
+                                       "2 x 20,000 queries per form"}, "host": host})
+    del dt
+
+
 def headline(args, out):
     """Config 4, the full-scale road graph (23.9 M points, seed 4): K = 8
     metrics customized in one level sweep plus their negative-cycle scans per
@@ -898,6 +953,16 @@ def headline(args, out):
                                                              order=order), 2))
             forms[name] = {"batch_ms": ms, "queries_per_s": nq / (ms / 1e3)}
             log(f"config {args.graph} queries, {name}: {nq} in {ms:.1f} ms")
+        # the metric split once into the reference's separate up / down arrays
+        # (timed apart), then the blocked kernel on it
+        use("blocked_2")
+        up_s, dn_s = dt.split(ud)
+        split_ms = statistics.median(event_ms(lambda: dt.split(ud, out=(up_s, dn_s)), 3))
+        dt.query_split(up_s, dn_s, qs, qt, want_scanned=False, dist=dist, order=order)
+        ms = statistics.median(event_ms(lambda: dt.query_split(up_s, dn_s, qs, qt, want_scanned=False, dist=dist,
+                                                               order=order), 2))
+        forms["split_2"] = {"batch_ms": ms, "queries_per_s": nq / (ms / 1e3), "split_ms": split_ms}
+        log(f"config {args.graph} queries, split_2: {nq} in {ms:.1f} ms (metric split {split_ms:.3f} ms)")
         # one query per launch: the blocked form with 8 wavefronts per side (its
         # default for launches of up to 256 queries) and the older chunked form
         singles = {}
@@ -912,8 +977,11 @@ def headline(args, out):
             log(f"config {args.graph} single query, {name}: p50 {np.median(one):.2f} ms")
         best_b = min(forms, key=lambda k: forms[k]["batch_ms"])
         best_s = min(singles, key=lambda k: singles[k])
-        use(best_b)
-        dist_full, scanned = dt.query(ud, qs, qt, order=order)
+        use("blocked_2" if best_b == "split_2" else best_b)
+        if best_b == "split_2":
+            dist_full, scanned = dt.query_split(up_s, dn_s, qs, qt, order=order)
+        else:
+            dist_full, scanned = dt.query(ud, qs, qt, order=order)
         qms = forms[best_b]["batch_ms"]
         vdep = topo.export()["depth"].astype(np.int64)
         chain = vdep[rank[qs_h]] + vdep[rank[qt_h]] + 2
@@ -1067,6 +1135,8 @@ def main():
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="also run one device copy of this many MiB (scales HBM byte counters)")
     ap.add_argument("--no-graph", action="store_true", help="skip the HIP graph replay of the step")
+    ap.add_argument("--counter-run", action="store_true",
+                    help="config 4: a short run of the sweep and query kernels for rocprofv3 --pmc passes")
     ap.add_argument("--prep", action="store_true",
                     help="time everything after the order for --config on the host and on the GPU")
     ap.add_argument("--full-queries", type=int, default=1_000_000)
@@ -1096,6 +1166,10 @@ def main():
         args.metrics = 8 if args.config == 4 else 64
     if args.prep:
         prep(args, out)
+        print(json.dumps(out), flush=True)
+        return
+    if args.counter_run:
+        counter_run(args, out)
         print(json.dumps(out), flush=True)
         return
     if args.config == 4:

@@ -195,6 +195,24 @@ int nwcch_query_strided(nwcch_dev* h, const int32_t* ud, int64_t ud_stride, cons
                         const int32_t* t_rank, int64_t nq, const int32_t* order, int32_t* dist,
                         int32_t* scanned, void* stream);
 
+/* One metric's interleaved (up, down) pairs ud[mp][2] (device) split into the
+ * separate arrays up[mp] and dn[mp] (device), the layout in which the
+ * reference keeps a customization (python/cch_nb.py:529-563 returns upw and
+ * dnw; cpp/cch.cpp:437-463 keeps vectors up and dn).  Asynchronous on
+ * `stream`. */
+int nwcch_metric_split(const int32_t* ud, int64_t mp, int32_t* up, int32_t* dn, void* stream);
+
+/* ET queries on a metric given as separate up[mp] and dn[mp] arrays (device):
+ * the arguments of cch_nb.query(s_rank, t_rank, first, head, parent, upw, dnw,
+ * ...) python/cch_nb.py:577-615 and Query::run cpp/cch.cpp:549-575, for nq
+ * pairs at once; results as nwcch_query.  Each search then reads 6 bytes per
+ * scanned arc (its own weight and a 16-bit head depth) instead of a pair and a
+ * 32-bit depth.  Runs the blocked query kernel, so the elimination tree may be
+ * at most about 7,000 levels deep (NWCCH_ERR_CAPACITY beyond). */
+int nwcch_query_split(nwcch_dev* h, const int32_t* up, const int32_t* dn, const int32_t* s_rank,
+                      const int32_t* t_rank, int64_t nq, const int32_t* order, int32_t* dist,
+                      int32_t* scanned, void* stream);
+
 /* Slab width used for K metrics when `slab` (a power of two <= 64, 0 = the
  * default 64) is requested: min(slab, K rounded up to a power of two). */
 int nwcch_batch_slab_width(int32_t K, int32_t slab);

@@ -121,6 +121,55 @@ __global__ void __launch_bounds__(256) k_base_batch_small(i64 mp, int32_t K, int
   if (st) atomicOr(status, st);
 }
 
+// The transpose for exactly 8 metrics: one thread per original arc, its 8
+// weights stored as two 16-byte words.
+__global__ void __launch_bounds__(256) k_transpose_w8(const int32_t* __restrict__ W, i64 m, int32_t* __restrict__ WT) {
+  const i64 stride = (i64)gridDim.x * blockDim.x;
+  for (i64 e = (i64)blockIdx.x * blockDim.x + threadIdx.x; e < m; e += stride) {
+    int4 lo, hi;
+    lo.x = W[e]; lo.y = W[m + e]; lo.z = W[2 * m + e]; lo.w = W[3 * m + e];
+    hi.x = W[4 * m + e]; hi.y = W[5 * m + e]; hi.z = W[6 * m + e]; hi.w = W[7 * m + e];
+    int4* o = reinterpret_cast<int4*>(WT + e * 8);
+    o[0] = lo;
+    o[1] = hi;
+  }
+}
+
+// Base values when the K metrics fill one slab exactly (K = SW = 2, 4 or 8):
+// SW / 2 lanes per arc, each lane two metrics -- an 8-byte load of their two
+// weights per original arc and one 16-byte store of their two (up, down)
+// pairs, so the lanes of an arc read one contiguous run of WT and write one of
+// UD.
+template <int SW>
+__global__ void __launch_bounds__(256) k_base_batch_pairs(i64 mp, const int32_t* __restrict__ ae_off,
+                                                          const int32_t* __restrict__ ae_idx,
+                                                          const int32_t* __restrict__ WT, int2* UD,
+                                                          uint32_t* status) {
+  constexpr int LPA = SW / 2;  // lanes per arc
+  uint32_t st = 0;
+  const i64 stride = (i64)gridDim.x * blockDim.x;
+  for (i64 t = (i64)blockIdx.x * blockDim.x + threadIdx.x; t < mp * LPA; t += stride) {
+    const i64 c = t / LPA;
+    const int q = (int)(t % LPA);
+    const int32_t e0 = ae_off[c], e1 = ae_off[c + 1];
+    int4 v = make_int4(kInf, kInf, kInf, kInf);  // (up, down) of metrics 2q and 2q + 1
+    for (int32_t j = e0; j < e1; j++) {
+      const int32_t e = ae_idx[j];
+      const int2 w = *reinterpret_cast<const int2*>(WT + (i64)(e >> 1) * SW + 2 * q);
+      if (w.x == (int32_t)0x80000000 || w.y == (int32_t)0x80000000) st |= kStatusBadWeight;
+      if (e & 1) {
+        v.x = min(v.x, w.x);
+        v.z = min(v.z, w.y);
+      } else {
+        v.y = min(v.y, w.x);
+        v.w = min(v.w, w.y);
+      }
+    }
+    reinterpret_cast<int4*>(UD + c * SW)[q] = v;
+  }
+  if (st) atomicOr(status, st);
+}
+
 // Base values of every (arc, metric) in the slab layout: UD[slab][c][km] = min
 // over the original arcs of c, per direction (python/cch_nb.py:534-542 per
 // metric).  A block holds 256 / SW arcs x SW metrics, metric fastest, so reads
@@ -436,15 +485,28 @@ extern "C" int nwcch_customize_batch(nwcch_dev* h, const int32_t* W, int32_t K, 
     {
       ProfScope ps(d, s, kKOther);
       if (d.m > 0) {
-        if (K <= 8) {
-          const i64 blocks = std::min<i64>((d.m + 255) / 256, 65536);
-          hipLaunchKernelGGL(k_transpose_w_small, dim3((unsigned)blocks), dim3(256), 0, s, W, d.m, K, d.wt);
+        const i64 tblocks = std::min<i64>((d.m + 255) / 256, 65536);
+        if (K == 8) {
+          hipLaunchKernelGGL(k_transpose_w8, dim3((unsigned)tblocks), dim3(256), 0, s, W, d.m, d.wt);
+        } else if (K <= 8) {
+          hipLaunchKernelGGL(k_transpose_w_small, dim3((unsigned)tblocks), dim3(256), 0, s, W, d.m, K, d.wt);
         } else {
           dim3 grid((unsigned)((d.m + 63) / 64), (unsigned)((K + 63) / 64));
           hipLaunchKernelGGL(k_transpose_w, grid, dim3(256), 0, s, W, d.m, K, d.wt);
         }
       }
-      if (KL <= 8) {
+      if (K == KL && (KL == 2 || KL == 4 || KL == 8) && reinterpret_cast<uintptr_t>(UD) % 16 == 0) {
+        const i64 blocks = std::max<i64>(std::min<i64>((d.mp * (KL / 2) + 255) / 256, 1 << 20), 1);
+        if (KL == 8)
+          hipLaunchKernelGGL(k_base_batch_pairs<8>, dim3((unsigned)blocks), dim3(256), 0, s, d.mp, d.ae_off,
+                             d.ae_idx, d.wt, ud, d.status);
+        else if (KL == 4)
+          hipLaunchKernelGGL(k_base_batch_pairs<4>, dim3((unsigned)blocks), dim3(256), 0, s, d.mp, d.ae_off,
+                             d.ae_idx, d.wt, ud, d.status);
+        else
+          hipLaunchKernelGGL(k_base_batch_pairs<2>, dim3((unsigned)blocks), dim3(256), 0, s, d.mp, d.ae_off,
+                             d.ae_idx, d.wt, ud, d.status);
+      } else if (KL <= 8) {
         const i64 blocks = std::max<i64>(std::min<i64>((d.mp + 255) / 256, 65536), 1);
 #define NWCCH_BASE_SMALL(SWV)                                                                                 \
   hipLaunchKernelGGL(k_base_batch_small<SWV>, dim3((unsigned)blocks), dim3(256), 0, s, d.mp, K, slices, d.ae_off, \

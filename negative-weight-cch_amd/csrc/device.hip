@@ -1359,6 +1359,10 @@ extern "C" int nwcch_dev_create(const nwcch_topo* topo, int device, nwcch_dev** 
     std::vector<int32_t> adep(t.mp);
     for (i64 a = 0; a < t.mp; a++) adep[a] = t.vdepth[t.head[a]];
     d.adep = dev_upload(d, adep.data(), adep.size());
+    if (t.depth < 32768) {
+      const std::vector<int16_t> a16(adep.begin(), adep.end());
+      d.adep16 = dev_upload(d, a16.data(), a16.size());
+    }
     build_rows(d, t);
     d.arc_pos = dev_upload(d, t.arc_pos.data(), t.arc_pos.size());
     d.status = dev_alloc<uint32_t>(d, 1);
@@ -1662,7 +1666,8 @@ int launch_queries(nwcch_dev* h, const int32_t* ud, int64_t ud_stride, const int
       int nws = d.query_blk_waves;
       if (nws == 0) nws = nq <= 256 ? 8 : (nq <= 4096 ? 4 : 2);
       launch_query_blk(d, nws, d.query_blk_unroll, reinterpret_cast<const int2*>(ud), ud_stride,
-                       (const long long*)ud_table, qmetric, n_metrics, s_rank, t_rank, nq, order, dist, scanned, s);
+                       (const long long*)ud_table, qmetric, n_metrics, nullptr, nullptr, s_rank, t_rank, nq, order,
+                       dist, scanned, s);
       HIP_OK(hipGetLastError());
       return NWCCH_OK;
     }
@@ -1718,6 +1723,63 @@ extern "C" int nwcch_query_multi(nwcch_dev* h, const int64_t* ud_table, int32_t 
   }
   return launch_queries(h, nullptr, 1, ud_table, qmetric, n_metrics, s_rank, t_rank, nq, nullptr, dist, scanned,
                         stream);
+}
+
+namespace {
+__global__ void __launch_bounds__(256) k_metric_split(const int2* __restrict__ ud, i64 mp, int32_t* __restrict__ up,
+                                                      int32_t* __restrict__ dn) {
+  const i64 stride = (i64)gridDim.x * blockDim.x;
+  for (i64 a = (i64)blockIdx.x * blockDim.x + threadIdx.x; a < mp; a += stride) {
+    const int2 v = ud[a];
+    up[a] = v.x;
+    dn[a] = v.y;
+  }
+}
+}  // namespace
+
+extern "C" int nwcch_metric_split(const int32_t* ud, int64_t mp, int32_t* up, int32_t* dn, void* stream) {
+  if (!ud || !up || !dn || mp < 0) {
+    set_error("null argument");
+    return NWCCH_ERR_ARG;
+  }
+  if (mp == 0) return NWCCH_OK;
+  const i64 blocks = std::min<i64>((mp + 255) / 256, 65536);
+  hipLaunchKernelGGL(k_metric_split, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream,
+                     reinterpret_cast<const int2*>(ud), (i64)mp, up, dn);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    set_error(std::string("nwcch_metric_split: ") + hipGetErrorString(e));
+    return NWCCH_ERR_RUNTIME;
+  }
+  return NWCCH_OK;
+}
+
+extern "C" int nwcch_query_split(nwcch_dev* h, const int32_t* up, const int32_t* dn, const int32_t* s_rank,
+                                 const int32_t* t_rank, int64_t nq, const int32_t* order, int32_t* dist,
+                                 int32_t* scanned, void* stream) {
+  if (!h || !up || !dn || !s_rank || !t_rank || !dist || nq < 0) {
+    set_error("null argument");
+    return NWCCH_ERR_ARG;
+  }
+  if (nq == 0) return NWCCH_OK;
+  if (nq > 0x7fffffffll) { set_error("at most 2^31-1 queries per launch"); return NWCCH_ERR_CAPACITY; }
+  try {
+    DevTopo& d = h->d;
+    if (query_blk_lds_bytes(d.depth + 1) > 64 * 1024 || !d.adep16) {
+      set_error("nwcch_query_split: elimination tree deeper than the blocked query supports (7,000 levels)");
+      return NWCCH_ERR_CAPACITY;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    ProfScope ps(d, s, kKQuery);
+    int nws = d.query_blk_waves;
+    if (nws == 0) nws = nq <= 256 ? 8 : (nq <= 4096 ? 4 : 2);
+    launch_query_blk(d, nws, d.query_blk_unroll, nullptr, 1, nullptr, nullptr, 0, up, dn, s_rank, t_rank, nq, order,
+                     dist, scanned, s);
+    HIP_OK(hipGetLastError());
+  } catch (const std::exception& e) {
+    return fail(e);
+  }
+  return NWCCH_OK;
 }
 
 extern "C" int nwcch_query(nwcch_dev* h, const int32_t* ud, const int32_t* s_rank, const int32_t* t_rank,

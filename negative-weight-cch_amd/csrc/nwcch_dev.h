@@ -130,6 +130,7 @@ struct DevTopo {
   VInfo* vinfo = nullptr;            // [n]
   int2* euler = nullptr;             // [n] preorder interval of the ET subtree
   int32_t* adep = nullptr;           // [mp] ET depth of the arc's head
+  int16_t* adep16 = nullptr;         // [mp] the same in 16 bits (null for depths >= 32768)
   int32_t* anc = nullptr;            // [n][32] 1st..32nd ancestors, -1 above the root
   // potential sweeps
   int32_t* head = nullptr;           // [mp] upper end of each arc
@@ -224,10 +225,12 @@ __device__ __forceinline__ i64 row_find(const int32_t* __restrict__ head, i64 lo
 // of nq queries with nws wavefronts per search side (1, 2, 4 or 8) and
 // `unroll` (4, 8 or 16) 64-arc chunks in flight per wavefront.
 size_t query_blk_lds_bytes(int32_t maxlen);
+// The metric is ud (pairs, stride in pairs) or ud_table/qmetric, or -- when
+// `up` is not null -- the separate arrays up and dn.
 void launch_query_blk(const DevTopo& d, int nws, int unroll, const int2* ud, i64 ud_stride,
-                      const long long* ud_table, const int32_t* qmetric, int32_t n_metrics, const int32_t* s_rank,
-                      const int32_t* t_rank, i64 nq, const int32_t* order, int32_t* dist, int32_t* scanned,
-                      hipStream_t s);
+                      const long long* ud_table, const int32_t* qmetric, int32_t n_metrics, const int32_t* up,
+                      const int32_t* dn, const int32_t* s_rank, const int32_t* t_rank, i64 nq,
+                      const int32_t* order, int32_t* dist, int32_t* scanned, hipStream_t s);
 
 // A device copy of `bytes` host bytes owned by d.
 void* dev_upload_bytes(DevTopo& d, const void* host, size_t bytes);

@@ -26,6 +26,8 @@
 // those of the vertex-by-vertex sweep.
 #include <hip/hip_runtime.h>
 
+#include <stdexcept>
+
 #include "nwcch_dev.h"
 #include "../../include/nwcch.h"
 
@@ -67,6 +69,36 @@ __device__ __forceinline__ void lds_barrier() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+__device__ __forceinline__ int32_t load_word(const int32_t* p, i64 i) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  typedef const int32_t __attribute__((address_space(1))) gint;
+  return ((const gint*)p)[i];
+#else
+  return p[i];
+#endif
+}
+
+// The query's view of one metric: interleaved (up, down) pairs (stride in
+// pairs), or -- SPLIT -- separate up and down arrays, as the reference keeps
+// them (python/cch_nb.py:577-615 takes upw and dnw), with 16-bit head depths:
+// each search then reads 6 bytes per arc instead of 12.
+template <bool SPLIT>
+struct QMetric {
+  const int2* ud;
+  i64 stride;
+  const int32_t* up;
+  const int32_t* dn;
+  const int16_t* adep16;
+  __device__ __forceinline__ int32_t w(int side, int32_t a) const {
+    if constexpr (SPLIT) {
+      return load_word(side ? dn : up, a);
+    } else {
+      const int2 u = load_pair(ud, (i64)a * stride);
+      return side ? u.y : u.x;
+    }
+  }
+};
+
 constexpr int kBlk = 32;          // ancestors per block
 constexpr int kMStride = kBlk + 1;  // padded matrix rows: lane t reads row t without bank conflicts
 
@@ -78,12 +110,16 @@ __host__ __device__ inline size_t blk_lds_words(int32_t maxlen) {
 }
 
 // NWS wavefronts per search side; U 64-arc chunks per wavefront with their
-// loads in flight at once in step 3.
-template <int NWS, int U>
+// loads in flight at once in step 3; SPLIT: the metric as separate up and down
+// arrays (up, dn) with 16-bit head depths (adep16), else interleaved pairs.
+template <int NWS, int U, bool SPLIT>
 __global__ void __launch_bounds__(128 * NWS) k_query_blk(const VInfo* __restrict__ vinfo,
                                                        const int2* __restrict__ euler,
                                                        const int32_t* __restrict__ adep,
+                                                       const int16_t* __restrict__ adep16,
                                                        const int32_t* __restrict__ anc,
+                                                       const int32_t* __restrict__ up,
+                                                       const int32_t* __restrict__ dn,
                                                        const int2* __restrict__ ud_one, i64 ud_stride,
                                                        const long long* __restrict__ ud_table,
                                                        const int32_t* __restrict__ qmetric, int32_t n_metrics,
@@ -105,7 +141,7 @@ __global__ void __launch_bounds__(128 * NWS) k_query_blk(const VInfo* __restrict
   const int32_t mq = ud_table ? qmetric[q] : 0;
   const int2* ud_p =
       ud_table ? ((mq >= 0 && mq < n_metrics) ? reinterpret_cast<const int2*>(ud_table[mq]) : nullptr) : ud_one;
-  if (rs < 0 || rs >= n || rt < 0 || rt >= n || !ud_p) {  // uniform over the workgroup
+  if (rs < 0 || rs >= n || rt < 0 || rt >= n || (!SPLIT && !ud_p)) {  // uniform over the workgroup
     if (tid == 0) {
       dist[q] = kInf;
       if (scanned) scanned[q] = 0;
@@ -113,7 +149,7 @@ __global__ void __launch_bounds__(128 * NWS) k_query_blk(const VInfo* __restrict
     }
     return;
   }
-  const int2* ud = ud_p;
+  const QMetric<SPLIT> met{ud_p, ud_stride, up, dn, adep16};
   int32_t* lab = lds + side * maxlen;
   int32_t* M = lds + 2 * maxlen + side * (kBlk * kMStride);
   int32_t* rows = lds + 2 * maxlen + 2 * kBlk * kMStride + side * 4 * kBlk;
@@ -183,9 +219,8 @@ __global__ void __launch_bounds__(128 * NWS) k_query_blk(const VInfo* __restrict
         const int j = (r * NWS + swave) * 2 + half;
         const bool ok = j < bsz && hl < min(rdeg[j], bsz - 1 - j);
         const int32_t a = ok ? rfirst[j] + hl : 0;
-        ad1[r] = adep[a];
-        const int2 u = load_pair(ud, (i64)a * ud_stride);
-        w1[r] = side ? u.y : u.x;
+        ad1[r] = SPLIT ? (int32_t)adep16[a] : adep[a];
+        w1[r] = met.w(side, a);
         if (!ok) ad1[r] = -1;
       }
 #pragma unroll
@@ -267,9 +302,8 @@ __global__ void __launch_bounds__(128 * NWS) k_query_blk(const VInfo* __restrict
 #pragma unroll
         for (int u = 0; u < U; u++) {
           const int32_t a = aa[u] >= 0 ? aa[u] : 0;
-          ad[u] = adep[a];
-          const int2 uv = load_pair(ud, (i64)a * ud_stride);
-          w[u] = side ? uv.y : uv.x;
+          ad[u] = SPLIT ? (int32_t)adep16[a] : adep[a];
+          w[u] = met.w(side, a);
         }
 #pragma unroll
         for (int u = 0; u < U; u++) {  // INF + w is INF: lanes with nothing to relax add nothing
@@ -317,21 +351,28 @@ __global__ void __launch_bounds__(128 * NWS) k_query_blk(const VInfo* __restrict
 size_t query_blk_lds_bytes(int32_t maxlen) { return blk_lds_words(maxlen) * sizeof(int32_t); }
 
 void launch_query_blk(const DevTopo& d, int nws, int unroll, const int2* ud, i64 ud_stride,
-                      const long long* ud_table, const int32_t* qmetric, int32_t n_metrics, const int32_t* s_rank,
-                      const int32_t* t_rank, i64 nq, const int32_t* order, int32_t* dist, int32_t* scanned,
-                      hipStream_t s) {
+                      const long long* ud_table, const int32_t* qmetric, int32_t n_metrics, const int32_t* up,
+                      const int32_t* dn, const int32_t* s_rank, const int32_t* t_rank, i64 nq,
+                      const int32_t* order, int32_t* dist, int32_t* scanned, hipStream_t s) {
   const int32_t maxlen = d.depth + 1;
   const size_t lds = query_blk_lds_bytes(maxlen);
   const dim3 grid((unsigned)nq);
-#define NWCCH_BLK_LAUNCH(NW, U)                                                                                \
-  hipLaunchKernelGGL((k_query_blk<NW, U>), grid, dim3(128 * NW), lds, s, d.vinfo, d.euler, d.adep, d.anc, ud,  \
-                     ud_stride, ud_table, qmetric, n_metrics, s_rank, t_rank, nq, order, (int32_t)d.n, maxlen, \
-                     dist, scanned, d.status)
-#define NWCCH_BLK_UNROLL(NW)                          \
-  switch (unroll) {                                   \
-    case 4: NWCCH_BLK_LAUNCH(NW, 4); break;           \
-    case 16: NWCCH_BLK_LAUNCH(NW, 16); break;         \
-    default: NWCCH_BLK_LAUNCH(NW, 8); break;          \
+  const bool split = up != nullptr;
+  if (split && !d.adep16) throw std::runtime_error("split-metric query: hierarchy too deep for 16-bit depths");
+#define NWCCH_BLK_LAUNCH(NW, U, SP)                                                                              \
+  hipLaunchKernelGGL((k_query_blk<NW, U, SP>), grid, dim3(128 * NW), lds, s, d.vinfo, d.euler, d.adep, d.adep16, \
+                     d.anc, up, dn, ud, ud_stride, ud_table, qmetric, n_metrics, s_rank, t_rank, nq, order,         \
+                     (int32_t)d.n, maxlen, dist, scanned, d.status)
+#define NWCCH_BLK_UNROLL(NW)                                                   \
+  if (split) {                                                                 \
+    if (unroll == 16) NWCCH_BLK_LAUNCH(NW, 16, true);                          \
+    else NWCCH_BLK_LAUNCH(NW, 8, true);                                        \
+  } else {                                                                     \
+    switch (unroll) {                                                          \
+      case 4: NWCCH_BLK_LAUNCH(NW, 4, false); break;                           \
+      case 16: NWCCH_BLK_LAUNCH(NW, 16, false); break;                         \
+      default: NWCCH_BLK_LAUNCH(NW, 8, false); break;                          \
+    }                                                                          \
   }
   switch (nws) {
     case 1: NWCCH_BLK_UNROLL(1); break;

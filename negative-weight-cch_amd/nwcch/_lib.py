@@ -64,6 +64,8 @@ def lib():
             "nwcch_profile_end": ([vp, p, p, i64, p], C.c_int),
             "nwcch_query_strided": ([vp, p, i64, p, p, i64, p, p, p, vp], C.c_int),
             "nwcch_query_multi": ([vp, p, i32, p, p, p, i64, p, p, vp], C.c_int),
+            "nwcch_metric_split": ([p, i64, p, p, vp], C.c_int),
+            "nwcch_query_split": ([vp, p, p, p, p, i64, p, p, p, vp], C.c_int),
             "nwcch_customize_batch": ([vp, p, i32, i32, p, vp], C.c_int),
             "nwcch_negcycle_scan_batch": ([p, i64, i32, i32, p, vp], C.c_int),
             "nwcch_batch_slab_width": ([i32, i32], C.c_int),

@@ -187,6 +187,35 @@ class DeviceTopology:
                                         ptr(sc), _stream(stream)), "nwcch_query_strided")
         return dist, sc
 
+    def split(self, ud: torch.Tensor, out: tuple[torch.Tensor, torch.Tensor] | None = None, stream=None):
+        """One metric's (up, down) pairs [mp, 2] as separate (up [mp], dn [mp])
+        arrays, the reference's layout (nwcch_metric_split)."""
+        assert ud.shape == (self.mp, 2) and ud.is_contiguous()
+        up, dn = out if out is not None else (torch.empty(self.mp, dtype=torch.int32, device=self.dev),
+                                               torch.empty(self.mp, dtype=torch.int32, device=self.dev))
+        check(lib().nwcch_metric_split(ud.data_ptr(), self.mp, ptr(up), ptr(dn), _stream(stream)),
+              "nwcch_metric_split")
+        return up, dn
+
+    def query_split(self, up: torch.Tensor, dn: torch.Tensor, s_rank, t_rank, want_scanned=True, stream=None,
+                    dist: torch.Tensor | None = None, order: torch.Tensor | None = None):
+        """ET queries on a metric given as separate up / down arrays
+        (nwcch_query_split) -> (dist, scanned or None), as `query`."""
+        qs = torch.as_tensor(s_rank).to(self.dev, torch.int32).contiguous()
+        qt = torch.as_tensor(t_rank).to(self.dev, torch.int32).contiguous()
+        if qs.shape != qt.shape or qs.dim() != 1:
+            raise ValueError("s_rank and t_rank must be 1-d of equal length")
+        assert up.shape == (self.mp,) and dn.shape == (self.mp,) and up.dtype == dn.dtype == torch.int32
+        if order is not None:
+            assert order.dtype == torch.int32 and order.is_cuda and order.shape == qs.shape
+        nq = qs.shape[0]
+        if dist is None:
+            dist = torch.empty(nq, dtype=torch.int32, device=self.dev)
+        sc = torch.empty(nq, dtype=torch.int32, device=self.dev) if want_scanned else None
+        check(lib().nwcch_query_split(self._h, ptr(up), ptr(dn), ptr(qs), ptr(qt), nq, ptr(order), ptr(dist),
+                                      ptr(sc), _stream(stream)), "nwcch_query_split")
+        return dist, sc
+
     def query_raw(self, ud: torch.Tensor, qs: torch.Tensor, qt: torch.Tensor, n: int, dist: torch.Tensor,
                   stream) -> None:
         """The first n queries of device int32 rank arrays qs/qt on a one-metric ud,

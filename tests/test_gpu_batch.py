@@ -14,7 +14,7 @@ INF = O.INF
 
 
 @pytest.mark.parametrize("slab", [0, 8])
-@pytest.mark.parametrize("K", [1, 5, 16, 64, 70])
+@pytest.mark.parametrize("K", [1, 2, 4, 5, 8, 16, 64, 70])
 def test_batch_equals_single(c0, K, slab):
     g, rank = c0
     print(f"case: batch K={K} slab={slab} on config 0", flush=True)

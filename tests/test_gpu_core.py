@@ -116,6 +116,16 @@ def test_road_config1_bit_exact():
         dist1, sc1 = dt.query(ud, rank[s], rank[t])
         assert torch.equal(dist1, dist) and torch.equal(sc1, sc), unroll
     dt.set_tuning("query_blk_unroll", 8)
+    # the metric as separate up / down arrays (the reference's layout)
+    up_d, dn_d = dt.split(ud)
+    assert np.array_equal(up_d.cpu().numpy(), up) and np.array_equal(dn_d.cpu().numpy(), dn)
+    for waves in (0, 1, 2, 4, 8):
+        for unroll in (8, 16):
+            dt.set_tuning("query_blk_waves", waves)
+            dt.set_tuning("query_blk_unroll", unroll)
+            dist1, sc1 = dt.query_split(up_d, dn_d, rank[s], rank[t])
+            assert torch.equal(dist1, dist) and torch.equal(sc1, sc), ("split", waves, unroll)
+    dt.set_tuning("query_blk_unroll", 8)
     dt.set_tuning("query_blk_waves", 0)
     dt.set_tuning("query_blk", 0)
     chunk0 = dt.get_tuning("query_chunk")

@@ -129,7 +129,7 @@ def test_path_limit_on_negative_cycle():
 
 
 @pytest.mark.parametrize("form", ["chain_walk_throughput", "chain_walk_latency", "chunked_latency",
-                                  "blocked_1", "blocked_2", "blocked_4", "blocked_8"])
+                                  "blocked_1", "blocked_2", "blocked_4", "blocked_8", "split_2", "split_8"])
 @pytest.mark.parametrize("name", ["grid", "small0", "small1", "small2"])
 def test_query_scanned_equals_reference_python(name, form):
     """Every query kernel form against the distances and scanned-arc counts the
@@ -142,7 +142,7 @@ def test_query_scanned_equals_reference_python(name, form):
     topo = _lib.Topology.from_graph(n, off, tgt, rank.astype(np.int32))
     dt = DeviceTopology(topo)
     ud = dt.customize(w.astype(np.int32))
-    if form.startswith("blocked"):
+    if form.startswith("blocked") or form.startswith("split"):
         dt.set_tuning("query_blk", 1)
         dt.set_tuning("query_blk_waves", int(form.split("_")[1]))
     else:
@@ -152,7 +152,10 @@ def test_query_scanned_equals_reference_python(name, form):
         dt.set_tuning("query_chunk", chunk)
         dt.set_tuning("query_deep_max", deep)
     qs, qt = rank[G[f"{name}/q_s"]], rank[G[f"{name}/q_t"]]
-    dist, sc = dt.query(ud, qs, qt)
+    if form.startswith("split"):
+        dist, sc = dt.query_split(*dt.split(ud), qs, qt)
+    else:
+        dist, sc = dt.query(ud, qs, qt)
     assert np.array_equal(dist.cpu().numpy().astype(np.int64), G[f"{name}/q_dist"])
     assert np.array_equal(sc.cpu().numpy().astype(np.int64), G[f"{name}/q_scanned"])
     dt.check_status()
