@@ -1660,7 +1660,7 @@ int launch_queries(nwcch_dev* h, const int32_t* ud, int64_t ud_stride, const int
     const size_t lds = (size_t)2 * maxlen * sizeof(int32_t);
     if (lds > 160 * 1024) throw std::runtime_error("elimination tree too deep for the LDS query kernel");
     ProfScope ps(d, s, kKQuery);
-    if (d.query_blk && query_blk_lds_bytes(maxlen) <= 64 * 1024) {
+    if (d.query_blk && query_blk_lds_bytes(maxlen) <= 64 * 1024 && (d.depth > 1024 || nq <= d.query_blk_small)) {
       // blocks of 32 ancestors, one query per workgroup; more wavefronts per
       // query for small launches (latency), fewer for large ones (throughput)
       int nws = d.query_blk_waves;
@@ -1881,9 +1881,12 @@ extern "C" int nwcch_set_tuning(nwcch_dev* h, const char* key, int32_t value) {
     h->d.rows_min_len = value;
     return NWCCH_OK;
   }
-  if (k == "rows_min_rows" || k == "rows_max_rows") {
+  if (k == "rows_min_rows" || k == "rows_max_rows" || k == "rows_min_level_tri" || k == "query_blk_small") {
     if (value < 0) { set_error(k + " must be >= 0"); return NWCCH_ERR_ARG; }
-    (k == "rows_min_rows" ? h->d.rows_min_rows : h->d.rows_max_rows) = value;
+    (k == "rows_min_rows" ? h->d.rows_min_rows
+                          : k == "rows_max_rows" ? h->d.rows_max_rows
+                                                 : k == "rows_min_level_tri" ? h->d.rows_min_level_tri
+                                                                             : h->d.query_blk_small) = value;
     return NWCCH_OK;
   }
   if (k == "query_blk") {
@@ -1940,6 +1943,8 @@ extern "C" int nwcch_get_tuning(const nwcch_dev* h, const char* key, int32_t* va
   else if (k == "rows_min_len") *value = h->d.rows_min_len;
   else if (k == "rows_min_rows") *value = h->d.rows_min_rows;
   else if (k == "rows_max_rows") *value = h->d.rows_max_rows;
+  else if (k == "rows_min_level_tri") *value = h->d.rows_min_level_tri;
+  else if (k == "query_blk_small") *value = h->d.query_blk_small;
   else if (k == "query_blk_waves") *value = h->d.query_blk_waves;
   else if (k == "query_blk_unroll") *value = h->d.query_blk_unroll;
   else if (k == "customize_graph") *value = h->d.customize_graph;

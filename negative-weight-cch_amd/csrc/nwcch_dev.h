@@ -96,7 +96,10 @@ struct DevTopo {
   int32_t query_chunk = 0;                   // queries read ancestor chains 32 at a time
   int32_t query_wpb = 1;                     // queries (wavefronts) per workgroup: 1, 2 or 4
                                              // (set at creation for deep hierarchies)
-  int32_t query_blk = 1;                     // blocked query kernel (blocks of 32 ancestors)
+  int32_t query_blk = 1;                     // blocked query kernel (blocks of 32 ancestors) for
+                                             // hierarchies deeper than 1,024 and, on shallower
+                                             // ones, launches of up to query_blk_small queries
+  int32_t query_blk_small = 4096;
   int32_t query_blk_waves = 0;               // its wavefronts per search side (0: by launch size)
   int32_t query_blk_unroll = 8;              // its 64-arc chunks in flight per wavefront: 4, 8, 16
   int32_t query_deep_max = 32768;            // query launches up to this size prefetch 4 x 32
@@ -110,7 +113,8 @@ struct DevTopo {
   int32_t rows_min_len = 16;                 // levels whose suffix pieces average at least this
                                              // many arcs run in row-merge form (65: never) ...
   int32_t rows_min_rows = 3;                 // ... if they have this many rows with triangles
-  int32_t rows_max_rows = 200;               // ... and at most this many
+  int32_t rows_max_rows = 200;               // ... and at most this many ...
+  int32_t rows_min_level_tri = 262144;       // ... and at least this many triangles
   std::vector<LevelLaunch> launches;         // per level, for tri_per_lane
   int32_t launches_for = -1;
   int32_t* wt = nullptr;             // batched weights transposed to [m][K] (scratch)

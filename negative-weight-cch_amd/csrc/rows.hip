@@ -196,6 +196,7 @@ bool rows_level_enabled(const DevTopo& d, size_t h) {
   if (!d.rows_items || d.rows_min_len > kPieceLen || h >= d.rows_pieces_lv.size() || d.rows_pieces_lv[h] == 0)
     return false;
   if (d.rows_n_lv[h] < d.rows_min_rows || d.rows_n_lv[h] > d.rows_max_rows) return false;
+  if (d.rows_tri_lv[h] < d.rows_min_level_tri) return false;  // small levels: the latency of its four round trips
   return d.rows_tri_lv[h] >= (i64)d.rows_min_len * d.rows_pieces_lv[h];  // mean piece length
 }
 

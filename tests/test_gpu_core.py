@@ -127,6 +127,14 @@ def test_road_config1_bit_exact():
             assert torch.equal(dist1, dist) and torch.equal(sc1, sc), ("split", waves, unroll)
     dt.set_tuning("query_blk_unroll", 8)
     dt.set_tuning("query_blk_waves", 0)
+    # a launch above query_blk_small on this shallow hierarchy takes the chain
+    # walk; forced onto the blocked kernel it gives the same answers
+    s5, t5 = graphs.random_pairs(g.n, 5000, seed=12)
+    d5, c5 = dt.query(ud, rank[s5], rank[t5])
+    dt.set_tuning("query_blk_small", 1 << 30)
+    d6, c6 = dt.query(ud, rank[s5], rank[t5])
+    dt.set_tuning("query_blk_small", 4096)
+    assert torch.equal(d5, d6) and torch.equal(c5, c6)
     dt.set_tuning("query_blk", 0)
     chunk0 = dt.get_tuning("query_chunk")
     for chunk in (0, 1):
@@ -312,6 +320,7 @@ def test_row_merge_levels(min_len):
         dt.set_tuning("rows_min_len", min_len)
         dt.set_tuning("rows_min_rows", 0)
         dt.set_tuning("rows_max_rows", 1 << 30)
+        dt.set_tuning("rows_min_level_tri", 0)
         for _ in range(2):
             _check_customize(dt, st, g.w)
         W, _ = graphs.metric_family(g, 2, seed=5, hmax=800) if g.name.startswith("c") else (None, None)
