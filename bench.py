@@ -788,11 +788,14 @@ def counter_run(args, out):
     from nwcch.device import DeviceTopology
 
     os.environ["NWCCH_PROGRESS"] = "1"
-    args.no_check = True
-    g, rank, W, _, host = config4_inputs(args, lambda msg: log(f"config {args.graph}: {msg}"))
-    K = W.shape[0]
+    t0 = time.perf_counter()
+    g = graphs.make_config(args.graph, progress=lambda msg: log(f"config {args.graph}: {msg}"))
+    rank = _lib.order(g.n, g.off, g.tgt, g.x, g.y, 64)
+    host = {"gen_and_order_s": time.perf_counter() - t0}
     topo = _lib.Topology.from_graph(g.n, g.off, g.tgt, rank, device_index=0, device_symbolic=0)
     dt = DeviceTopology(topo, 0)
+    W, _ = graphs.metric_family(g, args.metrics, 4, graphs.CONFIGS[args.graph].get("hmax", 3000))
+    K = W.shape[0]
     Wd = dt.weights_batch(W)
     UD = torch.empty(dt.batch_shape(K), dtype=torch.int32, device=dt.dev)
     for _ in range(2):

@@ -1360,7 +1360,8 @@ extern "C" int nwcch_dev_create(const nwcch_topo* topo, int device, nwcch_dev** 
     for (i64 a = 0; a < t.mp; a++) adep[a] = t.vdepth[t.head[a]];
     d.adep = dev_upload(d, adep.data(), adep.size());
     if (t.depth < 32768) {
-      const std::vector<int16_t> a16(adep.begin(), adep.end());
+      std::vector<int16_t> a16(adep.begin(), adep.end());
+      a16.resize((a16.size() + 127) / 128 * 128, 0);  // whole 256-byte words
       d.adep16 = dev_upload(d, a16.data(), a16.size());
     }
     build_rows(d, t);

@@ -73,3 +73,4 @@ def test_batch_layouts_equal_default(order, xcd):
         dt.set_tuning("batch_order", 0)
         dt.set_tuning("batch_xcd", 0)
     assert dt.status() == 0
+
