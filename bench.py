@@ -696,11 +696,31 @@ def serving(args, out):
     ver = srv.verify_alone(all_reqs, w0) if not args.no_check else None
     if ver is not None:
         log(f"serving: every answer vs alone recompute: {ver}")
+    # the plain form beside it on the same graph and metric 0: one query per
+    # launch (latency) and one large launch (throughput), no updates
+    import torch
+    ud0 = srv.metrics[0].ud
+    qs_h, qt_h = graphs.random_pairs(g.n, 100_000, seed=33)
+    qs = torch.as_tensor(rank[qs_h]).to(dt.dev)
+    qt = torch.as_tensor(rank[qt_h]).to(dt.dev)
+    dt.query(ud0, qs[:1], qt[:1], want_scanned=False)
+    one = [event_ms(lambda i=i: dt.query(ud0, qs[i:i + 1], qt[i:i + 1], want_scanned=False), 1)[0]
+           for i in range(args.single)]
+    dist = torch.empty(len(qs_h), dtype=torch.int32, device=dt.dev)
+    dt.query(ud0, qs, qt, want_scanned=False, dist=dist)
+    bms = statistics.median(event_ms(lambda: dt.query(ud0, qs, qt, want_scanned=False, dist=dist), 3))
+    plain = {"single_launches": len(one), "single_p50_ms": float(np.percentile(one, 50)),
+             "single_p99_ms": float(np.percentile(one, 99)),
+             "single_launch_queries_per_s": 1e3 / float(np.mean(one)),
+             "batch": len(qs_h), "batch_ms": bms, "batch_queries_per_s": len(qs_h) / (bms / 1e3)}
+    log(f"config 3 plain: one query per launch p50 {plain['single_p50_ms']:.3f} ms p99 "
+        f"{plain['single_p99_ms']:.3f} ms; {len(qs_h)} in one launch {bms:.2f} ms "
+        f"({plain['batch_queries_per_s']:.3g} q/s)")
     out["serving"] = {
         "workload": f"config 3: n={g.n} m={g.m} m+={topo.mp} T={topo.triangles} depth={topo.depth}; {K} "
                     f"resident metrics; Poisson queries (20% within 5 km) + 1/10/1000-arc updates",
         "update_mode": "snapshot (updates on a second copy, committed by swap)",
-        "runs": runs, "verify_alone": ver, "host": host,
+        "runs": runs, "verify_alone": ver, "plain_one_metric": plain, "host": host,
     }
     del srv, dt
 

@@ -142,6 +142,7 @@ struct DevTopo {
   // partial update (lazy)
   bool has_update = false;
   int32_t* arc_pos = nullptr;        // [mp]
+  PosRec* arc_rec = nullptr;         // [mp] triangle run of every arc, by arc id
   int32_t* arc_lvl = nullptr;        // [mp] ET height of the tail
   i64* side_off = nullptr;           // [mp+1]
   int32_t* side_top = nullptr;       // [2T]

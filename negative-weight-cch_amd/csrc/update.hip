@@ -50,8 +50,7 @@ __device__ __forceinline__ int32_t inf_add_u(int32_t a, int32_t b, uint32_t& st)
 struct UpdArgs {
   const int32_t* ae_off;
   const int32_t* ae_idx;
-  const int32_t* arc_pos;
-  const PosRec* prec;
+  const PosRec* arc_rec;  // triangle run of every arc, by arc id
   const int2* tri_ab;
   const i64* side_off;
   const int32_t* side_top;
@@ -80,20 +79,50 @@ __device__ __forceinline__ void enqueue(const UpdArgs& A, int32_t c) {
 
 // One wavefront re-evaluates arc c from its original arcs and lower triangles;
 // if the value changed it is stored, recorded, checked for a negative 2-cycle
-// (Corollary 3a) and the top arcs of its side triangles are queued.
+// (Corollary 3a) and the top arcs of its side triangles are queued.  Every
+// load that does not depend on the new value -- the arc's original-arc range,
+// triangle run, old value, side-list range and first 64 side entries -- is
+// issued at once, so an evaluation costs three dependent memory round trips
+// (ranges; arc list and triangle pairs; weights and operand pairs) and a change
+// two more (queue stamp, bucket slot).
 __device__ void wave_update_arc(const UpdArgs& A, int32_t c, int lane, uint32_t& st) {
-  int32_t up = kInf, dn = kInf;
   const int32_t e0 = A.ae_off[c], e1 = A.ae_off[c + 1];
-  for (int32_t j = e0 + lane; j < e1; j += 64) {
+  const PosRec r = A.arc_rec[c];
+  const int2 old = A.ud[c];
+  const i64 s0 = A.side_off[c], s1 = A.side_off[c + 1];
+  const int32_t top0 = s0 + lane < s1 ? A.side_top[s0 + lane] : -1;
+  int32_t up = kInf, dn = kInf;
+  const i64 t1 = r.tri_begin + r.tri_count;
+  // first pass: one original arc and one triangle per lane, loads interleaved
+  {
+    const int32_t j = e0 + lane;
+    const i64 q = r.tri_begin + lane;
+    const int32_t e = j < e1 ? A.ae_idx[j] : -1;
+    const int2 ab = q < t1 ? A.tri_ab[q] : make_int2(-1, -1);
+    const int32_t wk = e >= 0 ? A.w[e >> 1] : kInf;
+    int2 X = make_int2(kInf, kInf), Y = make_int2(kInf, kInf);
+    if (ab.x >= 0) {
+      X = A.ud[ab.x];
+      Y = A.ud[ab.y];
+    }
+    if (e >= 0) {
+      if (wk == (int32_t)0x80000000) st |= kStatusBadWeight;
+      if (e & 1) up = min(up, wk);
+      else dn = min(dn, wk);
+    }
+    if (ab.x >= 0) {
+      up = min(up, inf_add_u(X.y, Y.x, st));
+      dn = min(dn, inf_add_u(Y.y, X.x, st));
+    }
+  }
+  for (int32_t j = e0 + 64 + lane; j < e1; j += 64) {
     const int32_t e = A.ae_idx[j];
     const int32_t wk = A.w[e >> 1];
     if (wk == (int32_t)0x80000000) st |= kStatusBadWeight;
     if (e & 1) up = min(up, wk);
     else dn = min(dn, wk);
   }
-  const PosRec r = A.prec[A.arc_pos[c]];
-  const i64 t1 = r.tri_begin + r.tri_count;
-  for (i64 q = r.tri_begin + lane; q < t1; q += 64) {
+  for (i64 q = r.tri_begin + 64 + lane; q < t1; q += 64) {
     const int2 ab = A.tri_ab[q];
     const int2 X = A.ud[ab.x];
     const int2 Y = A.ud[ab.y];
@@ -105,7 +134,6 @@ __device__ void wave_update_arc(const UpdArgs& A, int32_t c, int lane, uint32_t&
     up = min(up, __shfl_xor(up, o, 64));
     dn = min(dn, __shfl_xor(dn, o, 64));
   }
-  const int2 old = A.ud[c];
   if (up == old.x && dn == old.y) return;  // wave-uniform
   if (lane == 0) {
     A.ud[c] = make_int2(up, dn);
@@ -116,8 +144,8 @@ __device__ void wave_update_arc(const UpdArgs& A, int32_t c, int lane, uint32_t&
       atomicMin(&A.res[3], (long long)c);
     }
   }
-  const i64 s1 = A.side_off[c + 1];
-  for (i64 j = A.side_off[c] + lane; j < s1; j += 64) enqueue(A, A.side_top[j]);
+  if (top0 >= 0) enqueue(A, top0);
+  for (i64 j = s0 + 64 + lane; j < s1; j += 64) enqueue(A, A.side_top[j]);
 }
 
 __global__ void k_upd_reset(UpdArgs A, int32_t levels) {
@@ -174,35 +202,45 @@ __global__ void k_upd_finish(UpdArgs A, int32_t levels) {
 }
 
 // Single workgroup: all non-empty buckets from `L0` up, barrier between heights.
+// The scan for the next non-empty bucket keeps the counts it reads, so the
+// bucket's size needs no second round trip.
 __global__ void __launch_bounds__(1024) k_upd_persistent(UpdArgs A, int32_t L0, int32_t levels) {
-  __shared__ int32_t s_next;
+  __shared__ int32_t s_next, s_n;
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int nwaves = blockDim.x >> 6;
   uint32_t st = 0;
   int32_t L = L0;
   while (true) {
-    // next non-empty bucket at height >= L
+    // next non-empty bucket at height >= L, and its size
     if (threadIdx.x == 0) s_next = levels;
     __syncthreads();
     for (int32_t base = L; base < levels; base += blockDim.x) {
       const int32_t l = base + threadIdx.x;
-      if (l < levels && __hip_atomic_load(&A.qcnt[l], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) > 0)
-        atomicMin(&s_next, l);
+      const int32_t cnt = l < levels ? __hip_atomic_load(&A.qcnt[l], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
+      if (cnt > 0) atomicMin(&s_next, l);
+      __syncthreads();
+      if (l == s_next) s_n = cnt;
       __syncthreads();
       if (s_next < levels) break;
     }
     const int32_t nl = s_next;
+    const int32_t n = s_n;
     __syncthreads();
     if (nl >= levels) break;
     L = nl;
-    const int32_t n = __hip_atomic_load(&A.qcnt[L], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const int32_t base = A.lvl_cap_off[L];
     for (int32_t j = wave; j < n; j += nwaves) wave_update_arc(A, A.qarcs[base + j], lane, st);
     __syncthreads();
     L++;
   }
   if (st) atomicOr(A.status, st);
+}
+
+// The triangle run of every arc by arc id: arc_rec[prec[p].arc] = prec[p].
+__global__ void __launch_bounds__(256) k_arc_rec(const PosRec* __restrict__ prec, i64 mp, PosRec* arc_rec) {
+  const i64 stride = (i64)gridDim.x * blockDim.x;
+  for (i64 p = (i64)blockIdx.x * blockDim.x + threadIdx.x; p < mp; p += stride) arc_rec[prec[p].arc] = prec[p];
 }
 
 // The partial-update side lists (python/cch.py:309-324): arc c = (x, y) is a
@@ -264,7 +302,7 @@ T* up_alloc(DevTopo& d, size_t count) {
 
 UpdArgs make_args(DevTopo& d, int32_t* w, int32_t* ud, int32_t* chg, i64 chg_cap) {
   UpdArgs A;
-  A.ae_off = d.ae_off; A.ae_idx = d.ae_idx; A.arc_pos = d.arc_pos; A.prec = d.prec;
+  A.ae_off = d.ae_off; A.ae_idx = d.ae_idx; A.arc_rec = d.arc_rec;
   A.tri_ab = d.tri_ab; A.side_off = d.side_off; A.side_top = d.side_top; A.arc_lvl = d.arc_lvl;
   A.lvl_cap_off = d.lvl_cap_off; A.e_arc32 = d.e_arc32; A.w = w;
   A.ud = reinterpret_cast<int2*>(ud); A.qstamp = d.qstamp; A.qarcs = d.qarcs; A.qcnt = d.qcnt;
@@ -313,6 +351,12 @@ extern "C" int nwcch_update_prepare(nwcch_dev* h, nwcch_topo* topo) {
       if (hb) throw std::runtime_error("upward CSR is not chordal (a side list lacks its top arc)");
     }
     if (!d.arc_pos) d.arc_pos = up_upload(d, t.arc_pos.data(), t.arc_pos.size());
+    d.arc_rec = up_alloc<PosRec>(d, t.mp);
+    if (t.mp > 0) {
+      const i64 blocks = std::min<i64>((t.mp + 255) / 256, 65536);
+      hipLaunchKernelGGL(k_arc_rec, dim3((unsigned)blocks), dim3(256), 0, 0, d.prec, t.mp, d.arc_rec);
+      HIP_OK(hipGetLastError());
+    }
     std::vector<int32_t> alvl(t.mp);
     for (i64 a = 0; a < t.mp; a++) alvl[a] = t.vlevel[t.arc_tail[a]];
     d.arc_lvl_host = alvl;
