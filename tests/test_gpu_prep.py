@@ -161,3 +161,21 @@ def test_device_symbolic_deep_trees():
         u = np.arange(n - 1)
         g = graphs.from_arcs(n, [(int(a), int(a) + 1, 1) for a in u] + [(int(a) + 1, int(a), 2) for a in u])
         _check_symbolic(g, np.arange(n, dtype=np.int32), f"path of {n}")
+
+
+@pytest.mark.parametrize("cap", [1, 64, 1000])
+def test_device_symbolic_staging_rerun(cap, c0, monkeypatch):
+    """A staging buffer too small for the rows (NWCCH_SYM_STAGE_CAP) makes the
+    device completion flag the overflow, skip the rows that did not fit, and
+    rerun every height with a larger buffer: the result still equals the host
+    build array for array."""
+    g, rank = c0
+    graphs_ = [(g, rank), (small_graphs()[5], order_of(small_graphs()[5]))]
+    monkeypatch.setenv("NWCCH_SYM_STAGE_CAP", str(cap))
+    for gg, rr in graphs_:
+        print(f"case: device completion with a staging buffer of {cap} entries, {gg.name}", flush=True)
+        th = _lib.Topology.from_graph(gg.n, gg.off, gg.tgt, rr)
+        td = _lib.Topology.from_graph(gg.n, gg.off, gg.tgt, rr, device_symbolic=0)
+        eh, ed = th.export(), td.export()
+        for k in eh:
+            assert np.array_equal(eh[k], ed[k]), (cap, gg.name, k)

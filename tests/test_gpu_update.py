@@ -213,9 +213,14 @@ def test_stop_state_equals_oracle(idx):
     rank = order_of(g) if idx < 12 else _lib.order(g.n, g.off, g.tgt, g.x, g.y, 64)
     print(f"case: stop state, graph {idx}", flush=True)
     tri = graphs.find_directed_triangle(g, np.random.default_rng(idx))
-    if tri is None:
-        pytest.skip("no directed triangle")
-    wn = graphs.inject_negative_cycle(g.w, tri, total=-3)
+    if tri is not None:
+        wn = graphs.inject_negative_cycle(g.w, tri, total=-3)
+    else:  # a grid has no triangle: a two-way street of total -3 instead
+        k = int(g.off[50])
+        v = int(g.tgt[k])
+        back = [j for j in range(g.off[v], g.off[v + 1]) if g.tgt[j] == 50][0]
+        wn = g.w.astype(np.int32).copy()
+        wn[k], wn[back] = 0, -3
     gn = g.with_weights(wn)
     st = O.Structure(g.n, g.off, g.tgt, rank)
     wit, up, dn = st.customize_stop(wn, with_state=True)
