@@ -281,8 +281,11 @@ int nwcch_update_prepare(nwcch_dev* h, nwcch_topo* t);
  * up + down < 0, smallest such arc or -1}; the changed arcs (unordered) go to
  * `changed` (device int32, capacity changed_cap) or, if NULL, to an internal
  * buffer read by nwcch_update_changed.  mode 0 = auto, 1 = one persistent
- * workgroup, 2 = one launch per ET height.  Updates on one handle must be
- * issued on one stream.  Replaces CCH.update_edges (python/cch.py:349-402) and
+ * workgroup, 2 = one launch per ET height, 3 = a full recustomization of the
+ * new weights diffed against the old values (the same values, evaluated count,
+ * changed set and status bits as the frontier; auto picks it from
+ * upd_full_min changes on, the persistent workgroup below).  Updates on one handle must be issued on one
+ * stream.  Replaces CCH.update_edges (python/cch.py:349-402) and
  * CCH.update_created_negative_cycle (python/cch.py:666-671). */
 int nwcch_update(nwcch_dev* h, int32_t* w, int32_t* ud, const int64_t* edge_idx, const int32_t* new_w,
                  int64_t k, int32_t mode, int32_t* changed, int64_t changed_cap, int64_t* result,
@@ -294,7 +297,8 @@ int nwcch_update(nwcch_dev* h, int32_t* w, int32_t* ud, const int64_t* edge_idx,
  * Same arguments, semantics and results as nwcch_update, except that the k
  * changes are staged in a static device array of the library (one per GPU,
  * held until the process ends) instead of the library's growable pinned and
- * device slots: k is at most 65536 (NWCCH_ERR_CAPACITY beyond), and the host
+ * device slots, and its per-height launches go out one by one (mode 3 runs
+ * as mode 2): k is at most 65536 (NWCCH_ERR_CAPACITY beyond), and the host
  * arrays edge_idx and new_w must stay unchanged until the stream has reached
  * the update.  All calls on one GPU share that array: calls that overlap on
  * one GPU (several streams or threads) race.  nwcch_update_prepare must have
@@ -325,7 +329,7 @@ int nwcch_status(nwcch_dev* h, uint32_t* out, int clear, void* stream);
  *                   team of lanes, up to 64, with at most this many lower
  *                   triangles per lane; default 4, values 1..256.
  *   "batch_tri_per_lane"  batched sweep: wavefronts per arc, up to 16, likewise;
- *                   default 8, values 1..256.
+ *                   default 16, values 1..256.
  *   "batch_order"   batched sweep: 1 = the arcs of each wave class taken in id
  *                   order (a position table built per shape; the first call
  *                   after a shape change synchronises the device), 0 = in
@@ -343,6 +347,34 @@ int nwcch_status(nwcch_dev* h, uint32_t* out, int clear, void* stream);
  *   "upd_persistent_max"  nwcch_update in mode 0 runs updates of at most this
  *                   many changed arcs in one persistent workgroup, larger ones
  *                   one launch per height; default 2.
+ *   "upd_full_min"  nwcch_update in mode 0 recustomizes fully (mode 3) from
+ *                   this many changed arcs on (above upd_persistent_max);
+ *                   default 3, 0 = never.
+ *   "update_graph"  1 = the per-height launches of nwcch_update replay
+ *                   captured graphs of 32 heights each, per stream (default);
+ *                   0 = they go out one by one.  nwcch_update_noalloc always
+ *                   launches them one by one (capturing allocates).
+ *   "rows_min_len", "rows_min_rows", "rows_max_rows", "rows_min_level_tri"
+ *                   nwcch_customize runs a level in row-merge form (one
+ *                   workgroup per row, the row's values in LDS, the suffixes
+ *                   of the rows below streamed) when its suffix pieces average
+ *                   at least rows_min_len arcs (default 16; 65 = never), it
+ *                   has rows_min_rows..rows_max_rows rows with triangles
+ *                   (default 3..200) and at least rows_min_level_tri
+ *                   triangles (default 262,144).
+ *   "rows_batch", "rows_batch_min_rows", "rows_batch_max_rows",
+ *   "rows_batch_min_level_tri", "rows_batch_min_len"
+ *                   the same form in nwcch_customize_batch for slabs of 8
+ *                   metrics and K a multiple of 8 (rows_batch 1 = on,
+ *                   default; 8..40 rows, 262,144 triangles, pieces of 16 arcs
+ *                   by default).  Every form gives identical values.
+ *   "query_blk", "query_blk_small", "query_blk_waves", "query_blk_unroll"
+ *                   the blocked query kernel (blocks of 32 ancestors, a 32 x 32
+ *                   (min, +) matrix in LDS): 1 = used for hierarchies deeper
+ *                   than 1,024 and for launches of at most query_blk_small
+ *                   queries (default 4,096), 0 = never; wavefronts per search
+ *                   side 1, 2, 4 or 8 (0 = by launch size, default); 64-arc
+ *                   chunks in flight per wavefront 4, 8 (default) or 16.
  *   "query_deep_max"  query launches of at most this many queries use the
  *                   latency-oriented kernels (deeper prefetch), larger launches
  *                   the throughput-oriented one (more wavefronts per CU);

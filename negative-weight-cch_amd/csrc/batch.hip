@@ -527,9 +527,15 @@ extern "C" int nwcch_customize_batch(nwcch_dev* h, const int32_t* W, int32_t K, 
       }
     }
     if (d.batch_order) ensure_bpos(d, 64 / KL);
-    for (const LevelRange& r : d.lv) {
+    const bool aligned16 = reinterpret_cast<uintptr_t>(UD) % 16 == 0;
+    for (size_t h = 0; h < d.lv.size(); h++) {
+      const LevelRange& r = d.lv[h];
       if (r.end == r.begin || r.triangles == 0) continue;
       ProfScope ps(d, s, kKBatch);
+      if (aligned16 && rows_batch_enabled(d, h, K, KL)) {
+        launch_rows_batch(d, h, slices, ud, s);
+        continue;
+      }
       switch (KL) {
         case 1: launch_level<1>(d, r, slices, K, ud, s); break;
         case 2: launch_level<2>(d, r, slices, K, ud, s); break;

@@ -1527,10 +1527,13 @@ void drop_cust_graphs(DevTopo& d) {
   bool any = false;
   for (GraphCache* c : {&d.cust, &d.pot})
     for (hipGraphExec_t e : c->exec) any |= e != nullptr;
+  for (const auto& v : d.upd_exec)
+    for (hipGraphExec_t e : v) any |= e != nullptr;
   if (!any) return;
   (void)hipDeviceSynchronize();
   for (GraphCache* c : {&d.cust, &d.pot})
     for (size_t i = 0; i < c->exec.size(); i++) drop_slot_graph(*c, (int)i);
+  drop_update_graphs(d);
 }
 
 // One call through a cache: its buffers into the stream's slot, then the
@@ -1573,6 +1576,12 @@ void run_cached(DevTopo& d, GraphCache& c, hipStream_t s, const void* in, void* 
 }
 
 }  // namespace
+
+namespace nwcch {
+void customize_on(DevTopo& d, const int32_t* w, int32_t* ud, hipStream_t s) {
+  run_cached(d, d.cust, s, w, ud, [&](const SlotPtrs* P, hipStream_t st) { launch_customize(d, P, st); });
+}
+}  // namespace nwcch
 
 extern "C" int nwcch_customize(nwcch_dev* h, const int32_t* w, int32_t* ud, void* stream) {
   if (!h || !w || !ud) { set_error("null argument"); return NWCCH_ERR_ARG; }
@@ -1857,6 +1866,16 @@ extern "C" int nwcch_set_tuning(nwcch_dev* h, const char* key, int32_t value) {
     h->d.upd_persistent_max = value;
     return NWCCH_OK;
   }
+  if (k == "upd_full_min") {
+    if (value < 0) { set_error("upd_full_min must be >= 0"); return NWCCH_ERR_ARG; }
+    h->d.upd_full_min = value;
+    return NWCCH_OK;
+  }
+  if (k == "update_graph") {
+    if (value < 0 || value > 1) { set_error("update_graph is 0 or 1"); return NWCCH_ERR_ARG; }
+    h->d.update_graph = value;
+    return NWCCH_OK;
+  }
   if (k == "query_chunk") {
     if (value < 0 || value > 1) { set_error("query_chunk is 0 or 1"); return NWCCH_ERR_ARG; }
     h->d.query_chunk = value;
@@ -1888,6 +1907,20 @@ extern "C" int nwcch_set_tuning(nwcch_dev* h, const char* key, int32_t value) {
                           : k == "rows_max_rows" ? h->d.rows_max_rows
                                                  : k == "rows_min_level_tri" ? h->d.rows_min_level_tri
                                                                              : h->d.query_blk_small) = value;
+    return NWCCH_OK;
+  }
+  if (k == "rows_batch") {
+    if (value < 0 || value > 1) { set_error("rows_batch is 0 or 1"); return NWCCH_ERR_ARG; }
+    h->d.rows_batch = value;
+    return NWCCH_OK;
+  }
+  if (k == "rows_batch_min_rows" || k == "rows_batch_max_rows" || k == "rows_batch_min_level_tri" ||
+      k == "rows_batch_min_len") {
+    if (value < 0) { set_error(k + " must be >= 0"); return NWCCH_ERR_ARG; }
+    (k == "rows_batch_min_rows"        ? h->d.rows_batch_min_rows
+     : k == "rows_batch_max_rows"      ? h->d.rows_batch_max_rows
+     : k == "rows_batch_min_level_tri" ? h->d.rows_batch_min_level_tri
+                                       : h->d.rows_batch_min_len) = value;
     return NWCCH_OK;
   }
   if (k == "query_blk") {
@@ -1945,6 +1978,11 @@ extern "C" int nwcch_get_tuning(const nwcch_dev* h, const char* key, int32_t* va
   else if (k == "rows_min_rows") *value = h->d.rows_min_rows;
   else if (k == "rows_max_rows") *value = h->d.rows_max_rows;
   else if (k == "rows_min_level_tri") *value = h->d.rows_min_level_tri;
+  else if (k == "rows_batch") *value = h->d.rows_batch;
+  else if (k == "rows_batch_min_rows") *value = h->d.rows_batch_min_rows;
+  else if (k == "rows_batch_max_rows") *value = h->d.rows_batch_max_rows;
+  else if (k == "rows_batch_min_level_tri") *value = h->d.rows_batch_min_level_tri;
+  else if (k == "rows_batch_min_len") *value = h->d.rows_batch_min_len;
   else if (k == "query_blk_small") *value = h->d.query_blk_small;
   else if (k == "query_blk_waves") *value = h->d.query_blk_waves;
   else if (k == "query_blk_unroll") *value = h->d.query_blk_unroll;
@@ -1954,6 +1992,8 @@ extern "C" int nwcch_get_tuning(const nwcch_dev* h, const char* key, int32_t* va
   else if (k == "query_chunk") *value = h->d.query_chunk;
   else if (k == "upd_persistent_max") *value = h->d.upd_persistent_max;
   else if (k == "upd_max_blocks") *value = h->d.upd_max_blocks;
+  else if (k == "update_graph") *value = h->d.update_graph;
+  else if (k == "upd_full_min") *value = h->d.upd_full_min;
   else { set_error("unknown tuning key: " + k); return NWCCH_ERR_ARG; }
   return NWCCH_OK;
 }

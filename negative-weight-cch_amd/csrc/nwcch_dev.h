@@ -93,6 +93,9 @@ struct DevTopo {
   int32_t upd_max_blocks = 1024;             // workgroups of a per-height update launch, at most
   int32_t upd_persistent_max = 2;            // updates of at most this many changed arcs run
                                              // in one persistent workgroup (mode auto)
+  int32_t update_graph = 1;                  // 0: per-height update launches one by one
+  int32_t upd_full_min = 3;                  // mode 0: updates of at least this many changed arcs
+                                             // recustomize fully and diff (0: never)
   int32_t query_chunk = 0;                   // queries read ancestor chains 32 at a time
   int32_t query_wpb = 1;                     // queries (wavefronts) per workgroup: 1, 2 or 4
                                              // (set at creation for deep hierarchies)
@@ -115,6 +118,11 @@ struct DevTopo {
   int32_t rows_min_rows = 3;                 // ... if they have this many rows with triangles
   int32_t rows_max_rows = 200;               // ... and at most this many ...
   int32_t rows_min_level_tri = 262144;       // ... and at least this many triangles
+  int32_t rows_batch = 1;                    // batched sweep with 8 metrics per slab: row-merge
+  int32_t rows_batch_min_rows = 8;           // form for levels of this many rows with triangles
+  int32_t rows_batch_max_rows = 40;          // ... to this many ...
+  int32_t rows_batch_min_level_tri = 262144; // ... with at least this many triangles ...
+  int32_t rows_batch_min_len = 16;           // ... in suffix pieces of this many arcs on average
   std::vector<LevelLaunch> launches;         // per level, for tri_per_lane
   int32_t launches_for = -1;
   int32_t* wt = nullptr;             // batched weights transposed to [m][K] (scratch)
@@ -129,6 +137,7 @@ struct DevTopo {
   std::vector<i64> rows_pieces_lv;   // [levels] pieces of each level
   std::vector<i64> rows_tri_lv;      // [levels] triangles of each level
   std::vector<i64> rows_n_lv;        // [levels] rows with triangles in each level
+  std::vector<int32_t> rows_maxdeg_lv;  // [levels] largest of those rows
   size_t rows_lds = 0;               // LDS bytes of a row-merge workgroup
   // query
   VInfo* vinfo = nullptr;            // [n]
@@ -154,8 +163,11 @@ struct DevTopo {
   int32_t* qstamp = nullptr;         // [mp] epoch at which an arc was queued
   int32_t* qarcs = nullptr;          // [mp] frontier buckets, bucket L at lvl_cap_off[L]
   int32_t* qcnt = nullptr;           // [levels] bucket fill
+  std::vector<int32_t> lvl_cap_host; // [levels+1] bucket offsets (host copy of lvl_cap_off)
   int32_t* chg = nullptr;            // [mp] changed arcs of the last update (unordered)
   long long* upd_res = nullptr;      // [4] evaluated, changed, negative changed, first negative
+  int2* upd_old = nullptr;           // [mp] values before a full-recustomization update
+  uint32_t* upd_st = nullptr;        // [2] status before it; 1 if it set a status bit anew
   // staging of the changes: a ring of slots (device + pinned host buffers of
   // kStageCap0 changes from nwcch_update_prepare on, grown for larger
   // updates), each reused only after its previous copy has completed
@@ -182,7 +194,13 @@ struct DevTopo {
   struct Rec { int kernel; hipEvent_t a, b; };
   // customization and CCH potential replayed from HIP graphs per stream
   GraphCache cust, pot;
-  std::mutex graph_mu;                           // guards both caches (calls from several host threads)
+  // per-height update launches replayed from HIP graphs: per stream slot one
+  // device argument record and one graph per segment of kUpdSegment heights
+  void* upd_args = nullptr;                      // [kCustSlots] per-call argument records (device)
+  std::vector<void*> upd_stream;                 // stream of each slot in use
+  std::vector<std::vector<hipGraphExec_t>> upd_exec;  // [slot][segment], nullptr until built
+  int32_t upd_next = 0;                          // next slot handed over once all are taken
+  std::mutex graph_mu;                           // guards the graph caches (calls from several host threads)
   int32_t customize_graph = 1;                   // 0: launch the kernels one by one
   bool prof_on = false;
   std::vector<hipEvent_t> ev_pool;
@@ -237,6 +255,14 @@ void launch_query_blk(const DevTopo& d, int nws, int unroll, const int2* ud, i64
                       const int32_t* dn, const int32_t* s_rank, const int32_t* t_rank, i64 nq,
                       const int32_t* order, int32_t* dist, int32_t* scanned, hipStream_t s);
 
+// A full customization w -> ud on stream s through the per-stream graph cache
+// (device.hip), as nwcch_customize runs it.
+void customize_on(DevTopo& d, const int32_t* w, int32_t* ud, hipStream_t s);
+
+// Destroys the captured per-height update graphs (update.hip).  The caller
+// holds d.graph_mu and the device has finished with them.
+void drop_update_graphs(DevTopo& d);
+
 // A device copy of `bytes` host bytes owned by d.
 void* dev_upload_bytes(DevTopo& d, const void* host, size_t bytes);
 
@@ -245,6 +271,10 @@ void* dev_upload_bytes(DevTopo& d, const void* host, size_t bytes);
 void build_rows(DevTopo& d, const Topology& t);
 bool rows_level_enabled(const DevTopo& d, size_t h);
 void launch_rows_level(const DevTopo& d, size_t h, const SlotPtrs* P, hipStream_t s);
+// The same form for a level of the batched sweep: 8 metrics per slab, K a
+// multiple of 8, UD laid out [slices][mp][8] (int2).
+bool rows_batch_enabled(const DevTopo& d, size_t h, int32_t K, int32_t KL);
+void launch_rows_batch(const DevTopo& d, size_t h, int32_t slices, int2* UD, hipStream_t s);
 
 // Down lists of the upward CSR: for every vertex y the arcs (x, y) with x < y,
 // in ascending arc id (so ascending x).

@@ -121,6 +121,107 @@ __global__ void __launch_bounds__(256) k_rows_level(const int4* __restrict__ ite
   if (st) atomicOr(status, st);
 }
 
+// The batched form: one slab of 8 metrics, UD laid out [slices][mp][8] (int2,
+// 64 bytes per arc), slice = blockIdx.y.  Each row position keeps its 8
+// (up, down) pairs in 17 LDS words (an odd stride, so that lanes on
+// consecutive positions hit distinct banks).  A wavefront takes a suffix piece
+// with one lane per arc b: it reads b's 64 bytes (a piece of the upward CSR is
+// one contiguous run of up to 4 KB) and the 64 bytes of the piece's arc a, the
+// same for every lane, and folds 16 minima into the row.  Per lower triangle
+// that is the 64 bytes of b, against two 64-byte gathers (a and b) in the
+// triangle-list kernel.
+constexpr int kBatchKL = 8;
+constexpr int kBatchRS = 2 * kBatchKL + 1;   // LDS words per row position
+constexpr int kBatchThreads = 512;
+constexpr int kBatchUnroll = 2;              // pieces per wavefront with their loads in flight
+
+__global__ void __launch_bounds__(512) k_rows_batch(const int4* __restrict__ items, const int2* __restrict__ pieces,
+                                                    const VInfo* __restrict__ vinfo,
+                                                    const int32_t* __restrict__ adep, int2* UD, i64 mp,
+                                                    uint32_t* status) {
+  extern __shared__ int32_t lds[];
+  constexpr int KL = kBatchKL, RS = kBatchRS;
+  int2* ud = UD + (i64)blockIdx.y * mp * KL;  // element (arc c, metric k): ud[c * KL + k]
+  const int4 it = items[blockIdx.x];
+  const VInfo v = vinfo[it.x];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int nwaves = blockDim.x >> 6;
+  int32_t* acc = lds;                                                    // [deg][RS]
+  int16_t* map = reinterpret_cast<int16_t*>(lds + (i64)v.deg * RS);     // head depth -> row position
+  for (int32_t i = tid; i < v.deg * KL; i += blockDim.x) {
+    const int32_t k = i / KL, km = i % KL;
+    int2 b = make_int2(kInf, kInf);
+    if (it.w) b = ud[(i64)v.first * KL + i];
+    acc[k * RS + 2 * km] = b.x;
+    acc[k * RS + 2 * km + 1] = b.y;
+  }
+  for (int32_t k = tid; k < v.deg; k += blockDim.x) map[adep[v.first + k]] = (int16_t)k;
+  __syncthreads();
+  uint32_t st = 0;
+  for (int32_t p0 = wave; p0 < it.z; p0 += nwaves * kBatchUnroll) {
+    int32_t aa[kBatchUnroll], bb[kBatchUnroll];
+#pragma unroll
+    for (int u = 0; u < kBatchUnroll; u++) {
+      const int32_t p = p0 + nwaves * u;
+      aa[u] = -1;
+      bb[u] = -1;
+      if (p < it.z) {  // uniform over the wavefront
+        const int2 pc = pieces[it.y + p];
+        aa[u] = pc.x;
+        if (lane < (pc.y & 127)) bb[u] = pc.x + 1 + kPieceLen * (pc.y >> 7) + lane;
+      }
+    }
+    int4 A[kBatchUnroll][KL / 2], B[kBatchUnroll][KL / 2];
+    int32_t ad[kBatchUnroll];
+#pragma unroll
+    for (int u = 0; u < kBatchUnroll; u++) {  // loads on a valid arc for every lane
+      const int32_t a = aa[u] >= 0 ? aa[u] : 0;
+      const int32_t b = bb[u] >= 0 ? bb[u] : a;
+      const int4* pa = reinterpret_cast<const int4*>(ud + (i64)a * KL);
+      const int4* pb = reinterpret_cast<const int4*>(ud + (i64)b * KL);
+#pragma unroll
+      for (int q = 0; q < KL / 2; q++) {
+        A[u][q] = pa[q];
+        B[u][q] = pb[q];
+      }
+      ad[u] = adep[b];
+    }
+#pragma unroll
+    for (int u = 0; u < kBatchUnroll; u++) {
+      if (bb[u] >= 0) {
+        int32_t* row = acc + map[ad[u]] * RS;
+#pragma unroll
+        for (int q = 0; q < KL / 2; q++) {
+          // two metrics per int4: (up, down) of metric 2q and of metric 2q + 1
+          const int32_t u0 = radd(A[u][q].y, B[u][q].x, st), d0 = radd(B[u][q].y, A[u][q].x, st);
+          const int32_t u1 = radd(A[u][q].w, B[u][q].z, st), d1 = radd(B[u][q].w, A[u][q].z, st);
+          if (u0 != kInf) atomicMin(&row[4 * q], u0);
+          if (d0 != kInf) atomicMin(&row[4 * q + 1], d0);
+          if (u1 != kInf) atomicMin(&row[4 * q + 2], u1);
+          if (d1 != kInf) atomicMin(&row[4 * q + 3], d1);
+        }
+      }
+    }
+  }
+  __syncthreads();
+  for (int32_t i = tid; i < v.deg * KL; i += blockDim.x) {
+    const int32_t k = i / KL, km = i % KL;
+    const int32_t up = acc[k * RS + 2 * km], dn = acc[k * RS + 2 * km + 1];
+    if (it.w) {
+      ud[(i64)v.first * KL + i] = make_int2(up, dn);
+    } else {
+      int32_t* g = reinterpret_cast<int32_t*>(ud + (i64)v.first * KL + i);
+      if (up != kInf) atomicMin(g, up);
+      if (dn != kInf) atomicMin(g + 1, dn);
+    }
+  }
+  if (st) atomicOr(status, st);
+}
+
+size_t rows_batch_lds(const DevTopo& d, size_t h) {
+  return (size_t)4 * kBatchRS * d.rows_maxdeg_lv[h] + 2 * (size_t)(d.depth + 1) + 16;
+}
+
 }  // namespace
 
 void build_rows(DevTopo& d, const Topology& t) {
@@ -140,6 +241,7 @@ void build_rows(DevTopo& d, const Topology& t) {
   d.rows_pieces_lv.assign(L, 0);
   d.rows_tri_lv.assign(L, 0);
   d.rows_n_lv.assign(L, 0);
+  d.rows_maxdeg_lv.assign(L, 0);
   int32_t maxdeg = 0;
   for (size_t h = 0; h < L; h++) {
     d.rows_it_off[h] = (i64)items.size();
@@ -169,7 +271,9 @@ void build_rows(DevTopo& d, const Topology& t) {
       if (cur.z > 0) items.push_back(cur);
       if (items.size() - item0 == 1) items.back().w = 1;  // the whole row in one item
       if (items.size() > item0) {
-        maxdeg = std::max<int32_t>(maxdeg, (int32_t)(t.first[y + 1] - t.first[y]));
+        const int32_t deg = (int32_t)(t.first[y + 1] - t.first[y]);
+        maxdeg = std::max<int32_t>(maxdeg, deg);
+        d.rows_maxdeg_lv[h] = std::max<int32_t>(d.rows_maxdeg_lv[h], deg);
         d.rows_n_lv[h]++;
       }
     }
@@ -198,6 +302,25 @@ bool rows_level_enabled(const DevTopo& d, size_t h) {
   if (d.rows_n_lv[h] < d.rows_min_rows || d.rows_n_lv[h] > d.rows_max_rows) return false;
   if (d.rows_tri_lv[h] < d.rows_min_level_tri) return false;  // small levels: the latency of its four round trips
   return d.rows_tri_lv[h] >= (i64)d.rows_min_len * d.rows_pieces_lv[h];  // mean piece length
+}
+
+// Batched levels: the same trade as above, with 8 metrics per triangle; the
+// rows must fit 64 KB of LDS (17 words per row position).
+bool rows_batch_enabled(const DevTopo& d, size_t h, int32_t K, int32_t KL) {
+  if (!d.rows_batch || KL != kBatchKL || K % KL != 0 || !d.rows_items || h >= d.rows_pieces_lv.size() ||
+      d.rows_pieces_lv[h] == 0)
+    return false;
+  if (d.rows_n_lv[h] < d.rows_batch_min_rows || d.rows_n_lv[h] > d.rows_batch_max_rows) return false;
+  if (d.rows_tri_lv[h] < d.rows_batch_min_level_tri) return false;
+  if (d.rows_tri_lv[h] < (i64)d.rows_batch_min_len * d.rows_pieces_lv[h]) return false;
+  return rows_batch_lds(d, h) <= 64 * 1024;
+}
+
+void launch_rows_batch(const DevTopo& d, size_t h, int32_t slices, int2* UD, hipStream_t s) {
+  const i64 i0 = d.rows_it_off[h], n = d.rows_it_off[h + 1] - i0;
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_rows_batch, dim3((unsigned)n, (unsigned)slices), dim3(kBatchThreads), rows_batch_lds(d, h), s,
+                     d.rows_items + i0, d.rows_pieces, d.vinfo, d.adep, UD, d.mp, d.status);
 }
 
 void launch_rows_level(const DevTopo& d, size_t h, const SlotPtrs* P, hipStream_t s) {

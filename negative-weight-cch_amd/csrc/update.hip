@@ -69,11 +69,25 @@ struct UpdArgs {
   int32_t epoch;
 };
 
+// The per-call fields of UpdArgs, read by replayed graphs from a device record.
+struct UpdCall {
+  int32_t* w;
+  int2* ud;
+  int32_t* chg;
+  i64 chg_cap;
+  int32_t epoch;
+};
+
+// Queue arc c once per update.  Its height and bucket offset are loaded beside
+// the stamp exchange and the slot claim rather than after them: two memory
+// round trips instead of four before the entry is written.
 __device__ __forceinline__ void enqueue(const UpdArgs& A, int32_t c) {
-  if (atomicExch(&A.qstamp[c], A.epoch) != A.epoch) {
-    const int32_t L = A.arc_lvl[c];
+  const int32_t L = A.arc_lvl[c];
+  const int32_t old = atomicExch(&A.qstamp[c], A.epoch);
+  const int32_t off = A.lvl_cap_off[L];
+  if (old != A.epoch) {
     const int32_t slot = atomicAdd(&A.qcnt[L], 1);
-    A.qarcs[A.lvl_cap_off[L] + slot] = c;
+    A.qarcs[off + slot] = c;
   }
 }
 
@@ -122,7 +136,25 @@ __device__ void wave_update_arc(const UpdArgs& A, int32_t c, int lane, uint32_t&
     if (e & 1) up = min(up, wk);
     else dn = min(dn, wk);
   }
-  for (i64 q = r.tri_begin + 64 + lane; q < t1; q += 64) {
+  // heavy arcs (thousands of triangles near the top of the hierarchy): four
+  // triangles per lane with their loads in flight at once
+  i64 q = r.tri_begin + 64 + lane;
+  for (; q + 3 * 64 < t1; q += 4 * 64) {
+    int2 ab[4], X[4], Y[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) ab[u] = A.tri_ab[q + 64 * u];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      X[u] = A.ud[ab[u].x];
+      Y[u] = A.ud[ab[u].y];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      up = min(up, inf_add_u(X[u].y, Y[u].x, st));
+      dn = min(dn, inf_add_u(Y[u].y, X[u].x, st));
+    }
+  }
+  for (; q < t1; q += 64) {
     const int2 ab = A.tri_ab[q];
     const int2 X = A.ud[ab.x];
     const int2 Y = A.ud[ab.y];
@@ -174,16 +206,43 @@ __global__ void k_upd_seed(UpdArgs A, const i64* __restrict__ edge_idx, const in
   if (a >= 0) enqueue(A, a);
 }
 
-__global__ void __launch_bounds__(256) k_upd_level(UpdArgs A, int32_t L) {
-  const int32_t n = A.qcnt[L];
-  const int32_t base = A.lvl_cap_off[L];
+// Bucket L (base = its offset, cap = its capacity, the arcs of height L).  A
+// wavefront reads its first bucket entry beside the bucket's fill rather than
+// after it: entries past the fill are stale ones of earlier updates, read in
+// bounds and skipped.
+__device__ __forceinline__ void upd_level(const UpdArgs& A, int32_t L, int32_t base, int32_t cap) {
   const int lane = threadIdx.x & 63;
   const int32_t wave = (int32_t)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
   const int32_t nwaves = (int32_t)((gridDim.x * blockDim.x) >> 6);
+  int32_t c = wave < cap ? A.qarcs[base + wave] : -1;
+  const int32_t n = A.qcnt[L];
   uint32_t st = 0;
-  for (int32_t j = wave; j < n; j += nwaves) wave_update_arc(A, A.qarcs[base + j], lane, st);
+  for (int32_t j = wave; j < n; j += nwaves) {
+    if (j != wave) c = A.qarcs[base + j];
+    wave_update_arc(A, c, lane, st);
+  }
   if (st) atomicOr(A.status, st);
 }
+
+__global__ void __launch_bounds__(256) k_upd_level(UpdArgs A, int32_t L, int32_t base, int32_t cap) {
+  upd_level(A, L, base, cap);
+}
+
+// The same for replayed graphs: the per-call fields come from a device record
+// (loaded beside the bucket's fill), the handle's own arrays from the captured
+// arguments.
+__global__ void __launch_bounds__(256) k_upd_level_rec(UpdArgs A, const UpdCall* __restrict__ rec, int32_t L,
+                                                       int32_t base, int32_t cap) {
+  const UpdCall u = *rec;
+  A.w = u.w;
+  A.ud = u.ud;
+  A.chg = u.chg;
+  A.chg_cap = u.chg_cap;
+  A.epoch = u.epoch;
+  upd_level(A, L, base, cap);
+}
+
+__global__ void k_upd_set_rec(UpdCall* rec, UpdCall u) { *rec = u; }
 
 __global__ void k_upd_finish(UpdArgs A, int32_t levels) {
   __shared__ long long s[256];
@@ -196,9 +255,105 @@ __global__ void k_upd_finish(UpdArgs A, int32_t levels) {
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    A.res[0] = s[0];
+    A.res[0] += s[0];  // the full-recustomization driver counted its side-list stamps already
     if (A.res[3] == 0x7fffffffffffffffll) A.res[3] = -1;
   }
+}
+
+// ---- full-recustomization driver.  A partial update leaves exactly the values
+// of a full customization of the new weights (Theorem 4), so for large updates
+// the device recustomizes every level (one replayed sweep) and derives the
+// rest: the changed arcs are those whose values differ from a copy taken
+// before, and the arcs the frontier would evaluate are the touched arcs plus
+// the top arcs of every triangle with a changed side -- queued into the same
+// buckets, so `evaluated` is counted as before.  Status bits the sweep sets
+// anew (it evaluates every arc) are cleared and set again only for the arcs
+// the frontier evaluates.
+
+__global__ void __launch_bounds__(256) k_upd_copy(const int2* __restrict__ src, int2* dst, i64 mp) {
+  const i64 stride = (i64)gridDim.x * blockDim.x;
+  for (i64 c = (i64)blockIdx.x * blockDim.x + threadIdx.x; c < mp; c += stride) dst[c] = src[c];
+}
+
+__global__ void k_upd_st_save(const uint32_t* status, uint32_t* st) {
+  st[0] = *status;
+  st[1] = 0;
+}
+
+// changed arcs into `all` (capacity mp; one counter claim per wavefront) and
+// the negative-arc check of each
+__global__ void __launch_bounds__(256) k_upd_diff(UpdArgs A, const int2* __restrict__ old, int32_t* all, i64 mp) {
+  const int lane = threadIdx.x & 63;
+  const i64 stride = (i64)gridDim.x * blockDim.x;
+  for (i64 c = (i64)blockIdx.x * blockDim.x + threadIdx.x; c < mp; c += stride) {
+    const int2 o = old[c], v = A.ud[c];
+    const bool ch = o.x != v.x || o.y != v.y;
+    const unsigned long long m = __ballot(ch);
+    if (m == 0) continue;  // uniform over the wavefront
+    const int leader = __ffsll((long long)m) - 1;
+    unsigned long long base = 0;
+    if (lane == leader) base = atomicAdd((unsigned long long*)&A.res[1], (unsigned long long)__popcll(m));
+    base = __shfl(base, leader, 64);
+    if (!ch) continue;
+    all[base + __popcll(m & ((1ull << lane) - 1))] = (int32_t)c;
+    if (v.x != kInf && v.y != kInf && (long long)v.x + (long long)v.y < 0) {
+      atomicAdd((unsigned long long*)&A.res[2], 1ull);
+      atomicMin(&A.res[3], (long long)c);
+    }
+  }
+}
+
+// The top arcs of the triangles each changed arc is a side of, stamped once
+// and counted into res[0] (one atomic per wavefront); the touched arcs are
+// in the buckets already and k_upd_finish adds them.
+__global__ void __launch_bounds__(256) k_upd_sides(UpdArgs A, const int32_t* __restrict__ all) {
+  const long long n = A.res[1];
+  const int lane = threadIdx.x & 63;
+  const long long wstride = (long long)gridDim.x * (blockDim.x >> 6);
+  unsigned long long cnt = 0;
+  for (long long i = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); i < n; i += wstride) {
+    const int32_t c = all[i];
+    const i64 s0 = A.side_off[c], s1 = A.side_off[c + 1];
+    for (i64 j = s0 + lane; j < s1; j += 64) cnt += atomicExch(&A.qstamp[A.side_top[j]], A.epoch) != A.epoch;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o, 64);
+  if (lane == 0 && cnt) atomicAdd((unsigned long long*)&A.res[0], cnt);
+}
+
+__global__ void k_upd_st_fix(uint32_t* status, uint32_t* st) {
+  const uint32_t fresh = *status & ~st[0] & (kStatusOverflow | kStatusBadWeight);
+  st[1] = fresh;
+  if (fresh) atomicAnd(status, ~fresh);
+}
+
+// the status checks of the frontier's evaluations, where the sweep set a bit anew
+__global__ void __launch_bounds__(256) k_upd_recheck(UpdArgs A, const uint32_t* __restrict__ st, i64 mp) {
+  const uint32_t fresh = st[1];
+  if (!fresh) return;
+  uint32_t b = 0;
+  const i64 stride = (i64)gridDim.x * blockDim.x;
+  for (i64 c = (i64)blockIdx.x * blockDim.x + threadIdx.x; c < mp; c += stride) {
+    if (A.qstamp[c] != A.epoch) continue;
+    for (int32_t j = A.ae_off[c]; j < A.ae_off[c + 1]; j++)
+      if (A.w[A.ae_idx[j] >> 1] == (int32_t)0x80000000) b |= kStatusBadWeight;
+    const PosRec r = A.arc_rec[c];
+    for (i64 q = r.tri_begin; q < r.tri_begin + r.tri_count; q++) {
+      const int2 ab = A.tri_ab[q];
+      const int2 X = A.ud[ab.x], Y = A.ud[ab.y];
+      (void)inf_add_u(X.y, Y.x, b);
+      (void)inf_add_u(Y.y, X.x, b);
+    }
+  }
+  b &= fresh;
+  if (b) atomicOr(A.status, b);
+}
+
+__global__ void __launch_bounds__(256) k_upd_copy_chg(const int32_t* __restrict__ all, int32_t* out, i64 cap,
+                                                      const long long* __restrict__ res) {
+  const i64 n = min((i64)res[1], cap);
+  const i64 stride = (i64)gridDim.x * blockDim.x;
+  for (i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) out[i] = all[i];
 }
 
 // Single workgroup: all non-empty buckets from `L0` up, barrier between heights.
@@ -273,6 +428,16 @@ __global__ void __launch_bounds__(256) k_side_fill(i64 mp, const i64* __restrict
 }
 
 }  // namespace
+
+void drop_update_graphs(DevTopo& d) {
+  for (auto& v : d.upd_exec)
+    for (hipGraphExec_t& e : v)
+      if (e) {
+        (void)hipGraphExecDestroy(e);
+        e = nullptr;
+      }
+}
+
 }  // namespace nwcch
 
 // ================================================================ C ABI (update)
@@ -374,6 +539,10 @@ extern "C" int nwcch_update_prepare(nwcch_dev* h, nwcch_topo* topo) {
     d.qcnt = up_alloc<int32_t>(d, t.levels);
     d.chg = up_alloc<int32_t>(d, t.mp);
     d.upd_res = up_alloc<long long>(d, 4);
+    d.upd_args = up_alloc<UpdCall>(d, kCustSlots);
+    d.upd_old = up_alloc<int2>(d, t.mp);
+    d.upd_st = up_alloc<uint32_t>(d, 2);
+    d.lvl_cap_host = cap;
     // staging slots sized up front: growing one later synchronises the
     // caller's stream and allocates pinned memory in the middle of its work
     for (auto& sg : d.stage) {
@@ -421,12 +590,75 @@ int check_changes(const DevTopo& d, const int64_t* edge_idx, const int32_t* new_
   return NWCCH_OK;
 }
 
+constexpr int32_t kUpdSegment = 32;  // heights per captured graph of per-height launches
+
+// Workgroups of the launch for height L: four arcs of the height per workgroup
+// (one per wavefront), at most upd_max_blocks.
+int upd_blocks(const DevTopo& d, int32_t L) {
+  const i64 cap = d.lv[L].end - d.lv[L].begin;
+  return (int)std::max<i64>(1, std::min<i64>((cap + 3) / 4, d.upd_max_blocks));
+}
+
+bool capturing(hipStream_t s) {
+  hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(s, &st) != hipSuccess) return true;  // unknown: launch directly
+  return st != hipStreamCaptureStatusNone;
+}
+
+// The argument-record slot of stream s (d.graph_mu held).  With every slot
+// taken, the oldest is handed over once the device has finished with it.
+int upd_slot(DevTopo& d, hipStream_t s) {
+  for (size_t i = 0; i < d.upd_stream.size(); i++)
+    if (d.upd_stream[i] == (void*)s) return (int)i;
+  const size_t nseg = (size_t)((d.levels + kUpdSegment - 1) / kUpdSegment);
+  if ((int)d.upd_stream.size() < kCustSlots) {
+    d.upd_stream.push_back((void*)s);
+    d.upd_exec.emplace_back(nseg, nullptr);
+    return (int)d.upd_stream.size() - 1;
+  }
+  HIP_OK(hipDeviceSynchronize());
+  const int i = d.upd_next++ % kCustSlots;
+  for (hipGraphExec_t& e : d.upd_exec[i])
+    if (e) {
+      (void)hipGraphExecDestroy(e);
+      e = nullptr;
+    }
+  d.upd_stream[i] = (void*)s;
+  return i;
+}
+
+// The launches for heights [L0, L1) reading the per-call arguments from `rec`
+// (the handle's arrays in A), captured on a stream that lives only for the
+// capture, and instantiated.
+hipGraphExec_t capture_heights(const DevTopo& d, const UpdArgs& A, const UpdCall* rec, int32_t L0, int32_t L1) {
+  hipStream_t cap = nullptr;
+  HIP_OK(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
+  if (e == hipSuccess) {
+    for (int32_t L = L0; L < L1; L++)
+      hipLaunchKernelGGL(k_upd_level_rec, dim3(upd_blocks(d, L)), dim3(256), 0, cap, A, rec, L,
+                         d.lvl_cap_host[L], d.lvl_cap_host[L + 1] - d.lvl_cap_host[L]);
+    e = hipStreamEndCapture(cap, &g);
+  }
+  (void)hipStreamDestroy(cap);
+  HIP_OK(e);
+  hipGraphExec_t x = nullptr;
+  e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  HIP_OK(e);
+  return x;
+}
+
 // The update's launches once the k changes sit in device memory (e_dev,
 // w_dev): reset, seed, the frontier by height (one persistent workgroup or
-// one launch per height) and the result.  Allocates nothing.
+// one launch per height) and the result.  With `graphs`, the per-height
+// launches replay the stream's captured graphs of kUpdSegment heights from the
+// segment holding L0 up (heights below L0 find empty buckets); capturing them
+// on first use allocates.  Without, nothing is allocated.
 void launch_update(DevTopo& d, int32_t* w, int32_t* ud, const i64* e_dev, const int32_t* w_dev, int64_t k,
                    int32_t L0, int32_t mode, int32_t* changed, int64_t changed_cap, int64_t* result,
-                   hipStream_t s, bool profile) {
+                   hipStream_t s, bool profile, bool graphs, bool allow_full) {
   d.epoch++;
   UpdArgs A = make_args(d, w, ud, changed, changed_cap);
   hipLaunchKernelGGL(k_upd_reset, dim3(std::max(1, std::min(64, (d.levels + 255) / 256))), dim3(256), 0, s, A,
@@ -434,11 +666,28 @@ void launch_update(DevTopo& d, int32_t* w, int32_t* ud, const i64* e_dev, const 
   if (k > 0)
     hipLaunchKernelGGL(k_upd_seed, dim3((unsigned)((k + 255) / 256)), dim3(256), 0, s, A, e_dev, w_dev, (i64)k,
                        d.m);
-  // mode: 0 = auto, 1 = one persistent workgroup, 2 = one launch per height.
-  // One or two changed arcs usually leave a frontier that one workgroup
-  // handles fastest; more spread over thousands of arcs.
+  // mode: 0 = auto, 1 = one persistent workgroup, 2 = one launch per height,
+  // 3 = full recustomization.  One or two changed arcs usually leave a
+  // frontier that one workgroup handles fastest; more spread over thousands of
+  // arcs, and from a few changes on one sweep over every level is faster than
+  // climbing the heights one launch at a time.
   const bool persistent = mode == 1 || (mode == 0 && k <= d.upd_persistent_max);
-  if (L0 < d.levels) {
+  const bool full = allow_full && !persistent &&
+                    (mode == 3 || (mode == 0 && d.upd_full_min > 0 && k >= d.upd_full_min));
+  if (full) {
+    const int blocks = (int)std::max<i64>(1, std::min<i64>((d.mp + 255) / 256, 8192));
+    int2* cur = reinterpret_cast<int2*>(ud);
+    hipLaunchKernelGGL(k_upd_copy, dim3(blocks), dim3(256), 0, s, cur, d.upd_old, d.mp);
+    hipLaunchKernelGGL(k_upd_st_save, dim3(1), dim3(1), 0, s, d.status, d.upd_st);
+    customize_on(d, w, ud, s);
+    hipLaunchKernelGGL(k_upd_diff, dim3(blocks), dim3(256), 0, s, A, d.upd_old, d.chg, d.mp);
+    hipLaunchKernelGGL(k_upd_sides, dim3((unsigned)std::max<i64>(1, std::min<i64>((d.mp + 3) / 4, 4096))),
+                       dim3(256), 0, s, A, d.chg);
+    hipLaunchKernelGGL(k_upd_st_fix, dim3(1), dim3(1), 0, s, d.status, d.upd_st);
+    hipLaunchKernelGGL(k_upd_recheck, dim3(blocks), dim3(256), 0, s, A, d.upd_st, d.mp);
+    if (A.chg != d.chg)
+      hipLaunchKernelGGL(k_upd_copy_chg, dim3(blocks), dim3(256), 0, s, d.chg, A.chg, A.chg_cap, d.upd_res);
+  } else if (L0 < d.levels) {
     if (persistent) {
       if (profile) {
         ProfScope ps(d, s, kKUpdate);
@@ -446,15 +695,27 @@ void launch_update(DevTopo& d, int32_t* w, int32_t* ud, const i64* e_dev, const 
       } else {
         hipLaunchKernelGGL(k_upd_persistent, dim3(1), dim3(1024), 0, s, A, L0, d.levels);
       }
+    } else if (graphs && !(profile && d.prof_on)) {
+      std::lock_guard<std::mutex> lock(d.graph_mu);
+      const int slot = upd_slot(d, s);
+      UpdCall* rec = static_cast<UpdCall*>(d.upd_args) + slot;
+      const UpdCall u{A.w, A.ud, A.chg, A.chg_cap, A.epoch};
+      hipLaunchKernelGGL(k_upd_set_rec, dim3(1), dim3(1), 0, s, rec, u);
+      std::vector<hipGraphExec_t>& ex = d.upd_exec[slot];
+      for (int32_t j = L0 / kUpdSegment; j * kUpdSegment < d.levels; j++) {
+        if (!ex[j])
+          ex[j] = capture_heights(d, A, rec, j * kUpdSegment, std::min(d.levels, (j + 1) * kUpdSegment));
+        HIP_OK(hipGraphLaunch(ex[j], s));
+      }
     } else {
       for (int32_t L = L0; L < d.levels; L++) {
-        const i64 cap = d.lv[L].end - d.lv[L].begin;
-        const int blocks = (int)std::max<i64>(1, std::min<i64>((cap + 3) / 4, d.upd_max_blocks));
+        const int blocks = upd_blocks(d, L);
+        const int32_t base = d.lvl_cap_host[L], cap = d.lvl_cap_host[L + 1] - base;
         if (profile) {
           ProfScope ps(d, s, kKUpdate);
-          hipLaunchKernelGGL(k_upd_level, dim3(blocks), dim3(256), 0, s, A, L);
+          hipLaunchKernelGGL(k_upd_level, dim3(blocks), dim3(256), 0, s, A, L, base, cap);
         } else {
-          hipLaunchKernelGGL(k_upd_level, dim3(blocks), dim3(256), 0, s, A, L);
+          hipLaunchKernelGGL(k_upd_level, dim3(blocks), dim3(256), 0, s, A, L, base, cap);
         }
       }
     }
@@ -469,6 +730,10 @@ extern "C" int nwcch_update(nwcch_dev* h, int32_t* w, int32_t* ud, const int64_t
                             int64_t changed_cap, int64_t* result, void* stream) {
   if (!h || !w || !ud || (k > 0 && (!edge_idx || !new_w)) || k < 0 || !result || changed_cap < 0) {
     set_error("null argument");
+    return NWCCH_ERR_ARG;
+  }
+  if (mode < 0 || mode > 3) {
+    set_error("mode is 0 (auto), 1 (persistent), 2 (per height) or 3 (full recustomization)");
     return NWCCH_ERR_ARG;
   }
   DevTopo& d = h->d;
@@ -513,7 +778,8 @@ extern "C" int nwcch_update(nwcch_dev* h, int32_t* w, int32_t* ud, const int64_t
       HIP_OK(hipMemcpyAsync(sg.w, sg.hw, k * sizeof(int32_t), hipMemcpyHostToDevice, s));
       HIP_OK(hipEventRecord(sg.ev, s));
     }
-    launch_update(d, w, ud, sg.e, sg.w, k, L0, mode, changed, changed_cap, result, s, true);
+    launch_update(d, w, ud, sg.e, sg.w, k, L0, mode, changed, changed_cap, result, s, true,
+                  d.update_graph && !capturing(s), true);
     HIP_OK(hipMemcpyAsync(result, d.upd_res, 4 * sizeof(long long), hipMemcpyDeviceToDevice, s));
     HIP_OK(hipGetLastError());
   } catch (const std::exception& e) {
@@ -570,6 +836,10 @@ extern "C" int nwcch_update_noalloc(nwcch_dev* h, int32_t* w, int32_t* ud, const
     set_error("null argument");
     return NWCCH_ERR_ARG;
   }
+  if (mode < 0 || mode > 3) {
+    set_error("mode is 0 (auto), 1 (persistent), 2 (per height) or 3 (full recustomization)");
+    return NWCCH_ERR_ARG;
+  }
   if (k > kNoAllocMaxChanges) {
     set_error("nwcch_update_noalloc: at most 65536 changes per call");
     return NWCCH_ERR_CAPACITY;
@@ -592,7 +862,8 @@ extern "C" int nwcch_update_noalloc(nwcch_dev* h, int32_t* w, int32_t* ud, const
       return NWCCH_ERR_RUNTIME;
     }
   }
-  launch_update(d, w, ud, e_dev, w_dev, k, L0, mode, changed, changed_cap, result, s, false);
+  launch_update(d, w, ud, e_dev, w_dev, k, L0, mode == 3 ? 2 : mode, changed, changed_cap, result, s, false, false,
+                false);
   if (hipMemcpyAsync(result, d.upd_res, 4 * sizeof(long long), hipMemcpyDeviceToDevice, s) != hipSuccess) {
     set_error("nwcch_update_noalloc: copying the result failed");
     return NWCCH_ERR_RUNTIME;

@@ -74,3 +74,42 @@ def test_batch_layouts_equal_default(order, xcd):
         dt.set_tuning("batch_xcd", 0)
     assert dt.status() == 0
 
+
+
+@pytest.mark.parametrize("K,slab", [(8, 0), (16, 8), (5, 8)])
+def test_batch_row_merge_levels(c0, K, slab):
+    """Batched levels in row-merge form (forced on every level; with 8 metrics
+    per slab and K a multiple of 8, else the triangle-list kernel keeps them)
+    equal the triangle-list kernel bit for bit and every metric equals its
+    oracle customization, on config 0 with a metric holding a negative cycle
+    (saturated values and the overflow status) and on config 1."""
+    import torch
+    g0, rank0 = c0
+    g1 = graphs.make_config(1)
+    rank1 = _lib.order(g1.n, g1.off, g1.tgt, g1.x, g1.y, 64)
+    for g, rank in ((g0, rank0), (g1, rank1)):
+        print(f"case: batched row-merge levels K={K} slab={slab} on {g.name}", flush=True)
+        W, _ = graphs.metric_family(g, K, seed=K + 1, hmax=600)
+        if g is g0:
+            tri = graphs.find_directed_triangle(g)
+            if tri is not None:
+                W[K - 1] = graphs.inject_negative_cycle(W[K - 1], tri)
+        topo = _lib.Topology.from_graph(g.n, g.off, g.tgt, rank)
+        dt = DeviceTopology(topo)
+        dt.set_tuning("rows_batch", 0)
+        ref = dt.customize_batch(W, slab=slab)
+        st_ref = dt.status(clear=True)
+        dt.set_tuning("rows_batch", 1)
+        dt.set_tuning("rows_batch_min_rows", 0)
+        dt.set_tuning("rows_batch_max_rows", 1 << 30)
+        dt.set_tuning("rows_batch_min_level_tri", 0)
+        dt.set_tuning("rows_batch_min_len", 0)
+        for _ in range(2):
+            got = dt.customize_batch(W, slab=slab)
+            assert torch.equal(got, ref)
+            assert dt.status(clear=True) == st_ref
+        st = O.Structure(g.n, g.off, g.tgt, rank)
+        for k in (0, K // 2):
+            up, dn, _ = st.customize(W[k])
+            v = dt.metric_view(got, k).cpu().numpy()
+            assert np.array_equal(v[:, 0], up) and np.array_equal(v[:, 1], dn), k

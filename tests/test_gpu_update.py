@@ -22,13 +22,17 @@ def _setup(g, rank):
     return dt, st
 
 
-@pytest.mark.parametrize("mode", [1, 2])
+@pytest.mark.parametrize("mode,graph", [(1, 1), (2, 1), (2, 0), (3, 1), (0, 1)])
 @pytest.mark.parametrize("k", [1, 10, 1000])
-def test_update_matches_oracle(c0, k, mode):
-    """Random changes of +-U(0,500) on k arcs, four rounds in a row."""
+def test_update_matches_oracle(c0, k, mode, graph):
+    """Random changes of +-U(0,500) on k arcs, four rounds in a row; mode 2
+    with the per-height launches replayed from captured graphs and one by
+    one, mode 3 (full recustomization diffed against the old values: the same
+    evaluated count, changed set and overflow status) and auto."""
     g, rank = c0
-    print(f"case: config 0 update k={k} mode={mode}", flush=True)
+    print(f"case: config 0 update k={k} mode={mode} update_graph={graph}", flush=True)
     dt, st = _setup(g, rank)
+    dt.set_tuning("update_graph", graph)
     w = dt.weights(g.w)
     ud = dt.customize(w)
     up, dn, _ = st.customize(g.w)
@@ -175,7 +179,7 @@ def test_dropin_matches_reference_python_vectors(name, gpu_index):
     assert np.array_equal(ud[:, 0], G[f"{name}/neg_stop_up"]) and np.array_equal(ud[:, 1], G[f"{name}/neg_stop_dn"])
 
 
-@pytest.mark.parametrize("mode", [1, 2])
+@pytest.mark.parametrize("mode", [1, 2, 3])
 def test_updates_queued_back_to_back(c0, mode):
     """Twelve updates issued without any host read in between (more than the
     staging slots) end in the oracle's state after the same twelve updates."""
@@ -200,6 +204,48 @@ def test_updates_queued_back_to_back(c0, mode):
         assert int(res[0].item()) == ev0
     got = ud.cpu().numpy()
     assert np.array_equal(got[:, 0], upd.up) and np.array_equal(got[:, 1], upd.dn)
+    assert dt.status() == 0
+
+
+def test_update_graphs_two_streams_two_metrics(c0):
+    """Per-height launches replayed from captured graphs serve any buffers:
+    two metrics updated alternately, each on its own stream (so each with its
+    own argument record and graphs), the second stream's updates starting at
+    other heights; both end in the oracle's state, and every result equals the
+    oracle's evaluated count."""
+    g, rank = c0
+    print("case: config 0, graph-replayed updates of two metrics on two streams", flush=True)
+    dt, st = _setup(g, rank)
+    assert dt.get_tuning("update_graph") == 1
+    W = [g.w, (g.w + np.random.default_rng(5).integers(0, 200, g.m)).astype(np.int32)]
+    ws = [dt.weights(x) for x in W]
+    uds = [dt.customize(x) for x in ws]
+    upds = []
+    for x in W:
+        up, dn, _ = st.customize(x)
+        upds.append(O.Updater(st, x, up, dn))
+    streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+    torch.cuda.synchronize()
+    rng = np.random.default_rng(77)
+    results = []
+    for rnd in range(10):
+        i = rnd % 2
+        k = [20, 300, 5][rnd % 3]
+        idx = rng.choice(g.m, k, replace=False)
+        new = (upds[i].w[idx] + rng.integers(1, 400, k)).astype(np.int32)
+        # the update workspace is shared: one update at a time per handle
+        streams[i].wait_stream(streams[1 - i])
+        res = dt.update(ws[i], uds[i], idx, new, mode=2, stream=streams[i])
+        ev0, _, _ = upds[i].update(idx, new)
+        with torch.cuda.stream(streams[i]):
+            results.append((res.clone(), ev0))
+    torch.cuda.synchronize()
+    for res, ev0 in results:
+        assert int(res[0].item()) == ev0
+    for i in range(2):
+        got = uds[i].cpu().numpy()
+        assert np.array_equal(got[:, 0], upds[i].up) and np.array_equal(got[:, 1], upds[i].dn), i
+        assert np.array_equal(ws[i].cpu().numpy(), upds[i].w), i
     assert dt.status() == 0
 
 
