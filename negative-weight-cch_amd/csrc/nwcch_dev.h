@@ -218,7 +218,7 @@ struct ProfScope {
   int kernel;
   hipEvent_t b = nullptr;
   ProfScope(DevTopo& dd, hipStream_t ss, int k) : d(dd), s(ss), kernel(k) {
-    if (!d.prof_on || d.ev_used + 2 > d.ev_pool.size()) return;
+    if (k < 0 || !d.prof_on || d.ev_used + 2 > d.ev_pool.size()) return;  // k < 0: not recorded
     hipEvent_t a = d.ev_pool[d.ev_used++];
     b = d.ev_pool[d.ev_used++];
     (void)hipEventRecord(a, s);

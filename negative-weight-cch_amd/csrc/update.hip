@@ -99,7 +99,28 @@ __device__ __forceinline__ void enqueue(const UpdArgs& A, int32_t c) {
 // issued at once, so an evaluation costs three dependent memory round trips
 // (ranges; arc list and triangle pairs; weights and operand pairs) and a change
 // two more (queue stamp, bucket slot).
-__device__ void wave_update_arc(const UpdArgs& A, int32_t c, int lane, uint32_t& st) {
+// The frontier's queue in device memory: stamps, bucket fills and entries.
+struct GlobalQueue {
+  __device__ void operator()(const UpdArgs& A, int32_t c) const { enqueue(A, c); }
+};
+
+// The same with the bucket fills and offsets in the LDS of the one persistent
+// workgroup (every enqueue of that driver happens inside it): the slot is
+// claimed without a device-memory round trip.
+struct LdsQueue {
+  int32_t* cnt;        // [levels]
+  const int32_t* off;  // [levels + 1]
+  __device__ void operator()(const UpdArgs& A, int32_t c) const {
+    const int32_t L = A.arc_lvl[c];
+    if (atomicExch(&A.qstamp[c], A.epoch) != A.epoch) {
+      const int32_t slot = atomicAdd(&cnt[L], 1);
+      A.qarcs[off[L] + slot] = c;
+    }
+  }
+};
+
+template <class Queue>
+__device__ void wave_update_arc(const UpdArgs& A, int32_t c, int lane, uint32_t& st, const Queue& queue) {
   const int32_t e0 = A.ae_off[c], e1 = A.ae_off[c + 1];
   const PosRec r = A.arc_rec[c];
   const int2 old = A.ud[c];
@@ -176,8 +197,8 @@ __device__ void wave_update_arc(const UpdArgs& A, int32_t c, int lane, uint32_t&
       atomicMin(&A.res[3], (long long)c);
     }
   }
-  if (top0 >= 0) enqueue(A, top0);
-  for (i64 j = s0 + 64 + lane; j < s1; j += 64) enqueue(A, A.side_top[j]);
+  if (top0 >= 0) queue(A, top0);
+  for (i64 j = s0 + 64 + lane; j < s1; j += 64) queue(A, A.side_top[j]);
 }
 
 __global__ void k_upd_reset(UpdArgs A, int32_t levels) {
@@ -219,7 +240,7 @@ __device__ __forceinline__ void upd_level(const UpdArgs& A, int32_t L, int32_t b
   uint32_t st = 0;
   for (int32_t j = wave; j < n; j += nwaves) {
     if (j != wave) c = A.qarcs[base + j];
-    wave_update_arc(A, c, lane, st);
+    wave_update_arc(A, c, lane, st, GlobalQueue{});
   }
   if (st) atomicOr(A.status, st);
 }
@@ -358,12 +379,24 @@ __global__ void __launch_bounds__(256) k_upd_copy_chg(const int32_t* __restrict_
 
 // Single workgroup: all non-empty buckets from `L0` up, barrier between heights.
 // The scan for the next non-empty bucket keeps the counts it reads, so the
-// bucket's size needs no second round trip.
+// bucket's size needs no second round trip.  With LQ the bucket fills and
+// offsets live in LDS (copied in at the start, the fills written back at the
+// end): the scan reads LDS and an enqueue claims its slot there.
+template <bool LQ>
 __global__ void __launch_bounds__(1024) k_upd_persistent(UpdArgs A, int32_t L0, int32_t levels) {
+  extern __shared__ int32_t qlds[];  // LQ: fills [levels], offsets [levels + 1]
   __shared__ int32_t s_next, s_n;
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int nwaves = blockDim.x >> 6;
+  int32_t* cnt = qlds;
+  int32_t* off = qlds + levels;
+  if (LQ) {
+    for (int32_t l = threadIdx.x; l <= levels; l += blockDim.x) {
+      if (l < levels) cnt[l] = A.qcnt[l];
+      off[l] = A.lvl_cap_off[l];
+    }
+  }
   uint32_t st = 0;
   int32_t L = L0;
   while (true) {
@@ -372,10 +405,11 @@ __global__ void __launch_bounds__(1024) k_upd_persistent(UpdArgs A, int32_t L0, 
     __syncthreads();
     for (int32_t base = L; base < levels; base += blockDim.x) {
       const int32_t l = base + threadIdx.x;
-      const int32_t cnt = l < levels ? __hip_atomic_load(&A.qcnt[l], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
-      if (cnt > 0) atomicMin(&s_next, l);
+      int32_t c = 0;
+      if (l < levels) c = LQ ? cnt[l] : __hip_atomic_load(&A.qcnt[l], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (c > 0) atomicMin(&s_next, l);
       __syncthreads();
-      if (l == s_next) s_n = cnt;
+      if (l == s_next) s_n = c;
       __syncthreads();
       if (s_next < levels) break;
     }
@@ -384,11 +418,18 @@ __global__ void __launch_bounds__(1024) k_upd_persistent(UpdArgs A, int32_t L0, 
     __syncthreads();
     if (nl >= levels) break;
     L = nl;
-    const int32_t base = A.lvl_cap_off[L];
-    for (int32_t j = wave; j < n; j += nwaves) wave_update_arc(A, A.qarcs[base + j], lane, st);
+    const int32_t base = LQ ? off[L] : A.lvl_cap_off[L];
+    for (int32_t j = wave; j < n; j += nwaves) {
+      if (LQ)
+        wave_update_arc(A, A.qarcs[base + j], lane, st, LdsQueue{cnt, off});
+      else
+        wave_update_arc(A, A.qarcs[base + j], lane, st, GlobalQueue{});
+    }
     __syncthreads();
     L++;
   }
+  if (LQ)
+    for (int32_t l = threadIdx.x; l < levels; l += blockDim.x) A.qcnt[l] = cnt[l];
   if (st) atomicOr(A.status, st);
 }
 
@@ -689,12 +730,12 @@ void launch_update(DevTopo& d, int32_t* w, int32_t* ud, const i64* e_dev, const 
       hipLaunchKernelGGL(k_upd_copy_chg, dim3(blocks), dim3(256), 0, s, d.chg, A.chg, A.chg_cap, d.upd_res);
   } else if (L0 < d.levels) {
     if (persistent) {
-      if (profile) {
-        ProfScope ps(d, s, kKUpdate);
-        hipLaunchKernelGGL(k_upd_persistent, dim3(1), dim3(1024), 0, s, A, L0, d.levels);
-      } else {
-        hipLaunchKernelGGL(k_upd_persistent, dim3(1), dim3(1024), 0, s, A, L0, d.levels);
-      }
+      ProfScope ps(d, s, profile ? kKUpdate : -1);
+      const size_t lq = (size_t)4 * (2 * (size_t)d.levels + 1);
+      if (lq <= 64 * 1024)
+        hipLaunchKernelGGL(k_upd_persistent<true>, dim3(1), dim3(1024), lq, s, A, L0, d.levels);
+      else
+        hipLaunchKernelGGL(k_upd_persistent<false>, dim3(1), dim3(1024), 0, s, A, L0, d.levels);
     } else if (graphs && !(profile && d.prof_on)) {
       std::lock_guard<std::mutex> lock(d.graph_mu);
       const int slot = upd_slot(d, s);

@@ -120,6 +120,7 @@ class Server:
         self.free_table = list(range(max_metrics))
         self._rr = 0
         self.last_query_events: list = []
+        self._updates_warm = False
 
     # ----------------------------------------------------------- metrics
     def metric_bytes(self) -> int:
@@ -146,7 +147,25 @@ class Server:
             m.w_spare, m.ud_spare = torch.empty_like(wd), torch.empty_like(ud)
         self.metrics[mid] = m
         self.used += need
+        if not self._updates_warm:
+            self._warm_up_updates(m)
         return True
+
+    def _warm_up_updates(self, m: _Metric):
+        """One no-op update of three arcs on scratch copies, on the update
+        stream: the library captures that stream's launch sequences on first
+        use, which belongs to setup, not to the first served update."""
+        k = min(3, self.dt.m)
+        if k == 0:
+            return
+        idx = np.arange(k, dtype=np.int64)
+        self.upd_stream.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(self.upd_stream):
+            w2, ud2 = m.w.clone(), m.ud.clone()
+            self.dt.update(w2, ud2, idx, m.w_host[idx], stream=self.upd_stream)
+        self.upd_stream.synchronize()
+        self.dt.status(clear=True)
+        self._updates_warm = True
 
     def drop_metric(self, mid: int):
         m = self.metrics.pop(mid, None)
