@@ -1006,15 +1006,27 @@ def headline(args, out):
         forms["split_2"] = {"batch_ms": ms, "queries_per_s": nq / (ms / 1e3), "split_ms": split_ms}
         log(f"config {args.graph} queries, split_2: {nq} in {ms:.1f} ms (metric split {split_ms:.3f} ms)")
         # one query per launch: the blocked form with 8 wavefronts per side (its
-        # default for launches of up to 256 queries) and the older chunked form
-        singles = {}
-        for name, blk, chunk in (("blocked_8", 1, 0), ("chunked_latency", 0, 1)):
+        # default for launches of up to 256 queries) with 8 or 16 64-arc chunks
+        # per wavefront in flight, the same on the split metric, and the older
+        # chunked form: (query_blk, query_chunk, query_blk_unroll, split)
+        single_forms = {"blocked_8": (1, 0, 8, False), "blocked_8_u16": (1, 0, 16, False),
+                        "split_8_u16": (1, 0, 16, True), "chunked_latency": (0, 1, 8, False)}
+
+        def use_single(name):
+            blk, chunk, unroll, split = single_forms[name]
             dt.set_tuning("query_blk", blk)
             dt.set_tuning("query_blk_waves", 0)
             dt.set_tuning("query_chunk", chunk)
             dt.set_tuning("query_deep_max", 1 << 30)
-            one = [event_ms(lambda i=i: dt.query(ud, qs[i:i + 1], qt[i:i + 1], want_scanned=False), 1)[0]
-                   for i in range(30)]
+            dt.set_tuning("query_blk_unroll", unroll)
+            if split:
+                return lambda i: dt.query_split(up_s, dn_s, qs[i:i + 1], qt[i:i + 1], want_scanned=False)
+            return lambda i: dt.query(ud, qs[i:i + 1], qt[i:i + 1], want_scanned=False)
+        singles = {}
+        for name in single_forms:
+            f1 = use_single(name)
+            f1(0)
+            one = [event_ms(lambda i=i: f1(i), 1)[0] for i in range(30)]
             singles[name] = float(np.median(one))
             log(f"config {args.graph} single query, {name}: p50 {np.median(one):.2f} ms")
         best_b = min(forms, key=lambda k: forms[k]["batch_ms"])
@@ -1028,11 +1040,9 @@ def headline(args, out):
         vdep = topo.export()["depth"].astype(np.int64)
         chain = vdep[rank[qs_h]] + vdep[rank[qt_h]] + 2
         qbytes = 8 * int(scanned.to(torch.int64).sum().item()) + 12 * int(chain.sum()) + 12 * nq
-        dt.set_tuning("query_blk", 1 if best_s == "blocked_8" else 0)
-        dt.set_tuning("query_blk_waves", 0)
-        dt.set_tuning("query_chunk", 0 if best_s == "blocked_8" else 1)
-        single = [event_ms(lambda i=i: dt.query(ud, qs[i:i + 1], qt[i:i + 1], want_scanned=False), 1)[0]
-                  for i in range(args.single_full)]
+        f1 = use_single(best_s)
+        single = [event_ms(lambda i=i: f1(i), 1)[0] for i in range(args.single_full)]
+        dt.set_tuning("query_blk_unroll", 8)
         dt.check_status()
         queries = {"count": nq, "batch_ms": qms, "queries_per_s": nq / (qms / 1e3), "batch_kernel": best_b,
                    "roofline": {"bound": "hbm", "achieved": qbytes / (qms / 1e3) / 1e9, "peak": HBM_PEAK_GBS,
