@@ -1144,6 +1144,8 @@ def headline(args, out):
                      "traffic_vs_algorithmic": traffic["vs_algorithmic"] if traffic else None,
                      "traffic_source": traffic["source"] if traffic else None,
                      "kernel": "k_customize_batch_level", "level_launches": n_lv,
+                     "level_kernels": "k_customize_batch_level (triangle list) and k_rows_batch (row-merge "
+                                      "form, levels of 8-40 rows); traffic counted on the triangle-list kernel",
                      "algorithmic_bytes_per_launch": B8d_lv / n_lv, "avg_launch_us": lv_avg_us,
                      "what": "algorithmic bytes 4 T + 4 (n+1) + 4 m+ + 16 K m+ over the level launches "
                              "(c-index stream, first/head, every (up, down) pair read and written once); "
