@@ -381,7 +381,8 @@ __global__ void __launch_bounds__(256) k_upd_copy_chg(const int32_t* __restrict_
 // The scan for the next non-empty bucket keeps the counts it reads, so the
 // bucket's size needs no second round trip.  With LQ the bucket fills and
 // offsets live in LDS (copied in at the start, the fills written back at the
-// end): the scan reads LDS and an enqueue claims its slot there.
+// end): each wavefront scans them itself and an enqueue claims its slot there,
+// so a height costs one barrier.
 template <bool LQ>
 __global__ void __launch_bounds__(1024) k_upd_persistent(UpdArgs A, int32_t L0, int32_t levels) {
   extern __shared__ int32_t qlds[];  // LQ: fills [levels], offsets [levels + 1]
@@ -399,14 +400,42 @@ __global__ void __launch_bounds__(1024) k_upd_persistent(UpdArgs A, int32_t L0, 
   }
   uint32_t st = 0;
   int32_t L = L0;
+  if (LQ) {
+    __syncthreads();
+    while (true) {
+      // next non-empty bucket at height >= L and its size: every wavefront
+      // scans the LDS fills itself (all see the same fills after the barrier
+      // that ended the previous height), so the scan needs no barrier
+      int32_t nl = levels, n = 0;
+      for (int32_t base = L; base < levels; base += 64) {
+        const int32_t l = base + lane;
+        const int32_t c = l < levels ? cnt[l] : 0;
+        const unsigned long long m = __ballot(c > 0);
+        if (m) {
+          const int f = __ffsll((long long)m) - 1;
+          nl = base + f;
+          n = __shfl(c, f, 64);
+          break;
+        }
+      }
+      if (nl >= levels) break;  // the same for every wavefront
+      L = nl;
+      const int32_t base = off[L];
+      for (int32_t j = wave; j < n; j += nwaves) wave_update_arc(A, A.qarcs[base + j], lane, st, LdsQueue{cnt, off});
+      __syncthreads();
+      L++;
+    }
+    for (int32_t l = threadIdx.x; l < levels; l += blockDim.x) A.qcnt[l] = cnt[l];
+    if (st) atomicOr(A.status, st);
+    return;
+  }
   while (true) {
     // next non-empty bucket at height >= L, and its size
     if (threadIdx.x == 0) s_next = levels;
     __syncthreads();
     for (int32_t base = L; base < levels; base += blockDim.x) {
       const int32_t l = base + threadIdx.x;
-      int32_t c = 0;
-      if (l < levels) c = LQ ? cnt[l] : __hip_atomic_load(&A.qcnt[l], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const int32_t c = l < levels ? __hip_atomic_load(&A.qcnt[l], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0;
       if (c > 0) atomicMin(&s_next, l);
       __syncthreads();
       if (l == s_next) s_n = c;
@@ -418,18 +447,11 @@ __global__ void __launch_bounds__(1024) k_upd_persistent(UpdArgs A, int32_t L0, 
     __syncthreads();
     if (nl >= levels) break;
     L = nl;
-    const int32_t base = LQ ? off[L] : A.lvl_cap_off[L];
-    for (int32_t j = wave; j < n; j += nwaves) {
-      if (LQ)
-        wave_update_arc(A, A.qarcs[base + j], lane, st, LdsQueue{cnt, off});
-      else
-        wave_update_arc(A, A.qarcs[base + j], lane, st, GlobalQueue{});
-    }
+    const int32_t base = A.lvl_cap_off[L];
+    for (int32_t j = wave; j < n; j += nwaves) wave_update_arc(A, A.qarcs[base + j], lane, st, GlobalQueue{});
     __syncthreads();
     L++;
   }
-  if (LQ)
-    for (int32_t l = threadIdx.x; l < levels; l += blockDim.x) A.qcnt[l] = cnt[l];
   if (st) atomicOr(A.status, st);
 }
 
