@@ -978,7 +978,7 @@ def headline(args, out):
         dist = torch.empty(nq, dtype=torch.int32, device=dt.dev)
         order = dt.chain_order(qs, qt)
         # query kernel forms: (query_blk, wavefronts per side, query_chunk, query_deep_max)
-        kinds = {"blocked_4": (1, 4, 0, 1 << 30), "blocked_2": (1, 2, 0, 1 << 30),
+        kinds = {"blocked_4": (1, 4, 0, 1 << 30), "blocked_2": (1, 2, 0, 1 << 30), "blocked_1": (1, 1, 0, 1 << 30),
                  "chunked_latency": (0, 0, 1, 1 << 30), "chain_walk_throughput": (0, 0, 0, 0)}
 
         def use(name):
@@ -996,15 +996,18 @@ def headline(args, out):
             forms[name] = {"batch_ms": ms, "queries_per_s": nq / (ms / 1e3)}
             log(f"config {args.graph} queries, {name}: {nq} in {ms:.1f} ms")
         # the metric split once into the reference's separate up / down arrays
-        # (timed apart), then the blocked kernel on it
-        use("blocked_2")
+        # (timed apart), then the blocked kernel on it with the fastest
+        # wavefront count of the pair form
+        best_blk = min(("blocked_1", "blocked_2", "blocked_4"), key=lambda k: forms[k]["batch_ms"])
+        use(best_blk)
         up_s, dn_s = dt.split(ud)
         split_ms = statistics.median(event_ms(lambda: dt.split(ud, out=(up_s, dn_s)), 3))
         dt.query_split(up_s, dn_s, qs, qt, want_scanned=False, dist=dist, order=order)
         ms = statistics.median(event_ms(lambda: dt.query_split(up_s, dn_s, qs, qt, want_scanned=False, dist=dist,
                                                                order=order), 2))
-        forms["split_2"] = {"batch_ms": ms, "queries_per_s": nq / (ms / 1e3), "split_ms": split_ms}
-        log(f"config {args.graph} queries, split_2: {nq} in {ms:.1f} ms (metric split {split_ms:.3f} ms)")
+        forms["split"] = {"batch_ms": ms, "queries_per_s": nq / (ms / 1e3), "split_ms": split_ms,
+                          "wavefronts_per_side": kinds[best_blk][1]}
+        log(f"config {args.graph} queries, split ({best_blk}): {nq} in {ms:.1f} ms (metric split {split_ms:.3f} ms)")
         # one query per launch: the blocked form with 8 wavefronts per side (its
         # default for launches of up to 256 queries) with 8 or 16 64-arc chunks
         # per wavefront in flight, the same on the split metric, and the older
@@ -1031,8 +1034,8 @@ def headline(args, out):
             log(f"config {args.graph} single query, {name}: p50 {np.median(one):.2f} ms")
         best_b = min(forms, key=lambda k: forms[k]["batch_ms"])
         best_s = min(singles, key=lambda k: singles[k])
-        use("blocked_2" if best_b == "split_2" else best_b)
-        if best_b == "split_2":
+        use(best_blk if best_b == "split" else best_b)
+        if best_b == "split":
             dist_full, scanned = dt.query_split(up_s, dn_s, qs, qt, order=order)
         else:
             dist_full, scanned = dt.query(ud, qs, qt, order=order)
