@@ -1,9 +1,11 @@
-"""CPU restatements of two device algorithms, checked against brute force and
-the host build: the segmented prefix minimum of the level kernel (DPP row
-shifts and row broadcasts over a 64-lane wavefront) and the pull-form
+"""CPU restatements of device algorithms, checked against brute force, the
+host build or the oracle: the segmented prefix minimum of the level kernel
+(DPP row shifts and row broadcasts over a 64-lane wavefront), the pull-form
 triangle enumeration of the GPU-built index (down lists, binary search in the
-lower row, x ascending).  The GPU tests check the kernels themselves bit for
-bit; these pin the networks they implement."""
+lower row, x ascending), the chordal completion by ancestor bitmaps, the
+blocked ET query, the row-merge enumeration, and the outputs the
+full-recustomization update driver derives.  The GPU tests check the kernels
+themselves bit for bit; these pin the algorithms they implement."""
 import numpy as np
 
 from conftest import order_of, small_graphs
@@ -283,3 +285,56 @@ def test_row_merge_enumeration_is_the_lower_triangle_set():
         c = np.repeat(sched.astype(np.int64), np.diff(toff))
         want = sorted(zip(c.tolist(), tab[:, 0].tolist(), tab[:, 1].tolist()))
         assert _row_merge_triangles(ex["first"], ex["head"]) == want
+
+
+def _full_update_outputs(first, head, e_arc, w_new_edges, up0, dn0, up1, dn1):
+    """The outputs the full-recustomization update driver (update.hip) derives
+    from the values before (up0, dn0) and after (up1, dn1) a full
+    customization of the new weights: the changed arcs, and the number of
+    arcs the heap-ordered update would evaluate -- the touched arcs plus the
+    top arc of every lower triangle one of whose sides changed (the side
+    lists of python/cch.py:309-324)."""
+    first = np.asarray(first, np.int64)
+    head = np.asarray(head, np.int64)
+    n = len(first) - 1
+    tail = np.repeat(np.arange(n), np.diff(first))
+    changed = np.nonzero((up0 != up1) | (dn0 != dn1))[0]
+    queued = {int(e_arc[k]) for k in w_new_edges if e_arc[k] >= 0}
+    for c in changed:
+        x, y = int(tail[c]), int(head[c])
+        for k in range(first[x], first[x + 1]):
+            if k == c:
+                continue
+            lo, hi = min(y, int(head[k])), max(y, int(head[k]))
+            row = head[first[lo]:first[lo + 1]]
+            j = int(np.searchsorted(row, hi))
+            assert j < len(row) and row[j] == hi, "chordal completion lacks a top arc"
+            queued.add(int(first[lo]) + j)
+    return changed, len(queued)
+
+
+def test_full_update_derivation_equals_heap_update():
+    """The full-recustomization update driver: a full customization of the
+    new weights, diffed against the old values, gives the heap-ordered
+    partial update's values (Theorem 4 of docs/technical_note.md), its
+    changed set and its evaluated count -- over rounds of 1 to 40 changes,
+    decreases included, on the reference's test family, against the oracle's
+    restatement of python/cch.py:349-402."""
+    from oracle import oracle as O
+    rng = np.random.default_rng(9)
+    for g in small_graphs()[::2]:
+        rank = order_of(g)
+        st = O.Structure(g.n, g.off, g.tgt, rank)
+        w = g.w.copy()
+        up, dn, _ = st.customize(w)
+        upd = O.Updater(st, w, up, dn)
+        for k in (1, 3, 10, 40):
+            idx = rng.choice(g.m, size=min(k, g.m), replace=False)
+            new = (upd.w[idx] + rng.integers(-300, 301, len(idx))).astype(np.int32)
+            up0, dn0 = upd.up.copy(), upd.dn.copy()
+            ev, ch, _ = upd.update(idx, new)
+            up1, dn1, _ = st.customize(upd.w)
+            assert np.array_equal(up1, upd.up) and np.array_equal(dn1, upd.dn), (g.name, k)
+            changed, evaluated = _full_update_outputs(st.first, st.head, st.e_arc, idx, up0, dn0, up1, dn1)
+            assert sorted(changed.tolist()) == sorted(ch.tolist()), (g.name, k)
+            assert evaluated == ev, (g.name, k, evaluated, ev)
