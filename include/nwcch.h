@@ -375,6 +375,8 @@ int nwcch_status(nwcch_dev* h, uint32_t* out, int clear, void* stream);
  *                   queries (default 4,096), 0 = never; wavefronts per search
  *                   side 1, 2, 4 or 8 (0 = by launch size, default); 64-arc
  *                   chunks in flight per wavefront 4, 8 (default) or 16.
+ *   "query_blk64"   1 = with 8 wavefronts per side the blocked kernel walks
+ *                   the chains in blocks of 64 ancestors (default), 0 = 32.
  *   "query_deep_max"  query launches of at most this many queries use the
  *                   latency-oriented kernels (deeper prefetch), larger launches
  *                   the throughput-oriented one (more wavefronts per CU);

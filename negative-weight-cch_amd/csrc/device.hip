@@ -1923,6 +1923,11 @@ extern "C" int nwcch_set_tuning(nwcch_dev* h, const char* key, int32_t value) {
                                        : h->d.rows_batch_min_len) = value;
     return NWCCH_OK;
   }
+  if (k == "query_blk64") {
+    if (value < 0 || value > 1) { set_error("query_blk64 is 0 or 1"); return NWCCH_ERR_ARG; }
+    h->d.query_blk64 = value;
+    return NWCCH_OK;
+  }
   if (k == "query_blk") {
     if (value < 0 || value > 1) { set_error("query_blk is 0 or 1"); return NWCCH_ERR_ARG; }
     h->d.query_blk = value;
@@ -1986,6 +1991,7 @@ extern "C" int nwcch_get_tuning(const nwcch_dev* h, const char* key, int32_t* va
   else if (k == "query_blk_small") *value = h->d.query_blk_small;
   else if (k == "query_blk_waves") *value = h->d.query_blk_waves;
   else if (k == "query_blk_unroll") *value = h->d.query_blk_unroll;
+  else if (k == "query_blk64") *value = h->d.query_blk64;
   else if (k == "customize_graph") *value = h->d.customize_graph;
   else if (k == "flat_store") *value = h->d.flat_store;
   else if (k == "query_deep_max") *value = h->d.query_deep_max;

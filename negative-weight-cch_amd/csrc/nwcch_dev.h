@@ -105,6 +105,7 @@ struct DevTopo {
   int32_t query_blk_small = 4096;
   int32_t query_blk_waves = 0;               // its wavefronts per search side (0: by launch size)
   int32_t query_blk_unroll = 8;              // its 64-arc chunks in flight per wavefront: 4, 8, 16
+  int32_t query_blk64 = 1;                   // 8 wavefronts per side: blocks of 64 ancestors
   int32_t query_deep_max = 32768;            // query launches up to this size prefetch 4 x 32
                                              // arcs of the next row, larger ones 32
   int32_t flat_min_tri = 0;                  // levels whose heaviest arc has >= this many

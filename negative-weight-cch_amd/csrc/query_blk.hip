@@ -1,4 +1,4 @@
-// Elimination-tree query in blocks of 32 ancestors (python/cch.py:675-732,
+// Elimination-tree query in blocks of 32 or 64 ancestors (python/cch.py:675-732,
 // python/cch_nb.py:577-615, cpp/cch.cpp:549-575).
 //
 // One query per workgroup: the first half of the workgroup runs the upward
@@ -6,14 +6,15 @@
 // walk the vertex's ancestor chain bottom-up; every arc out of a chain vertex
 // ends at a higher chain vertex, so labels live in LDS indexed by ET depth.
 //
-// The chain is cut into blocks of 32 consecutive ancestors.  Within a block
+// The chain is cut into blocks of kBlk consecutive ancestors (32; 64 for
+// launches of a few queries, with 8 wavefronts per side).  Within a block
 // the labels become final in chain order, which is a sequential chain of
 // dependencies; everything else is independent of the labels:
 //   1. the arcs that stay inside the block (a prefix of each row, since rows
 //      are sorted by head rank, i.e. nearest ancestor first) are scattered into
-//      a 32 x 32 (min,+) matrix in LDS;
+//      a kBlk x kBlk (min,+) matrix in LDS;
 //   2. one wavefront settles the block: lane t holds the label of block vertex
-//      t and column t of the matrix in registers, and for j = 0..31 takes the
+//      t and column t of the matrix in registers, and for j = 0..kBlk-1 takes the
 //      settled label of vertex j with a lane read and relaxes its own label --
 //      a few instructions per ancestor instead of a memory round trip;
 //   3. all wavefronts of the side relax the arcs of the settled block that
@@ -99,20 +100,19 @@ struct QMetric {
   }
 };
 
-constexpr int kBlk = 32;          // ancestors per block
-constexpr int kMStride = kBlk + 1;  // padded matrix rows: lane t reads row t without bank conflicts
+constexpr int kBlkMin = 32;  // ancestors per block: 32, or 64 for launches of a few queries
 
 // LDS words of one query: labels of both sides, two block matrices, per side
 // the block rows' first arc, degree, settled label and flattened end, and two
 // words for the lowest common ancestor and the scan counts.
-__host__ __device__ inline size_t blk_lds_words(int32_t maxlen) {
-  return (size_t)2 * maxlen + 2 * kBlk * kMStride + 2 * 4 * kBlk + 4;
+__host__ __device__ inline size_t blk_lds_words(int32_t maxlen, int blk = kBlkMin) {
+  return (size_t)2 * maxlen + 2 * (size_t)blk * (blk + 1) + 2 * 4 * (size_t)blk + 4;
 }
 
 // NWS wavefronts per search side; U 64-arc chunks per wavefront with their
 // loads in flight at once in step 3; SPLIT: the metric as separate up and down
 // arrays (up, dn) with 16-bit head depths (adep16), else interleaved pairs.
-template <int NWS, int U, bool SPLIT>
+template <int NWS, int U, bool SPLIT, int kBlk>
 __global__ void __launch_bounds__(128 * NWS) k_query_blk(const VInfo* __restrict__ vinfo,
                                                        const int2* __restrict__ euler,
                                                        const int32_t* __restrict__ adep,
@@ -129,6 +129,8 @@ __global__ void __launch_bounds__(128 * NWS) k_query_blk(const VInfo* __restrict
                                                        int32_t maxlen, int32_t* __restrict__ dist,
                                                        int32_t* __restrict__ scanned, uint32_t* status) {
   extern __shared__ int32_t lds[];
+  static_assert(kBlk == 32 || kBlk == 64, "blocks of 32 or 64 ancestors");
+  constexpr int kMStride = kBlk + 1;  // padded matrix rows: lane t reads row t without bank conflicts
   constexpr int SL = 64 * NWS;  // lanes per side
   const int tid = threadIdx.x;
   const int side = tid >= SL ? 1 : 0;  // 0: s, upward arcs; 1: t, downward arcs
@@ -163,18 +165,26 @@ __global__ void __launch_bounds__(128 * NWS) k_query_blk(const VInfo* __restrict
   const VInfo v0 = vinfo[x0];
   const int32_t D = v0.depth;
   const int32_t Dmax = max(vinfo[rs].depth, vinfo[rt].depth);
-  const int32_t nb = (D + kBlk) / kBlk;  // blocks of this side: ceil((D + 1) / 32)
+  const int32_t nb = (D + kBlk) / kBlk;  // blocks of this side: ceil((D + 1) / kBlk)
   const int32_t nbmax = (Dmax + kBlk) / kBlk;
   const int32_t tin_s = euler[rs].x;
   for (int32_t i = stid; i <= D; i += SL) lab[i] = kInf;
   uint32_t st = 0;
-  // chain of the current block on lanes 0..31 of the side's first wavefront
+  // chain of the current block on lanes 0..kBlk-1 of the side's first wavefront
   const bool lead = swave == 0 && lane < kBlk;
   int32_t cid = -1;
   VInfo cv{0, 0, -1, 0};
   bool cin = false;
+  if (swave == 0) {
+    // ancestors 1..32 from x0's row of the ancestor table, 33..64 (blocks of
+    // 64) from the row of the 32nd
+    if (lane < 33) cid = lane == 0 ? x0 : anc[(i64)x0 * 32 + lane - 1];
+    if (kBlk == 64) {
+      const int32_t a32 = __shfl(cid, 32, 64);
+      if (lane >= 33) cid = a32 >= 0 ? anc[(i64)a32 * 32 + lane - 33] : -1;
+    }
+  }
   if (lead) {
-    cid = lane == 0 ? x0 : anc[(i64)x0 * 32 + lane - 1];
     if (cid >= 0) {
       cv = lane == 0 ? v0 : vinfo[cid];
       if (side) {
@@ -196,7 +206,11 @@ __global__ void __launch_bounds__(128 * NWS) k_query_blk(const VInfo* __restrict
     int32_t nid = -1;
     if (swave == 0 && act) {
       const int32_t last = __shfl(cid, kBlk - 1, 64);
-      if (lane < kBlk && last >= 0) nid = anc[(i64)last * 32 + lane];
+      if (lane < 32 && last >= 0) nid = anc[(i64)last * 32 + lane];
+      if (kBlk == 64) {  // the next 32 from the row of the 32nd
+        const int32_t n31 = __shfl(nid, 31, 64);
+        if (lane >= 32) nid = n31 >= 0 ? anc[(i64)n31 * 32 + lane - 32] : -1;
+      }
       if (lead) {
         rfirst[lane] = cv.first;
         rdeg[lane] = (lane < bsz && cid >= 0) ? cv.deg : 0;
@@ -211,12 +225,14 @@ __global__ void __launch_bounds__(128 * NWS) k_query_blk(const VInfo* __restrict
     // 1. arcs inside the block: among the first bsz - 1 - j arcs of row j; a
     //    wavefront takes two rows per pass (one per half), all loads first
     if (act) {
-      constexpr int RPW = kBlk / (2 * NWS);  // row pairs per wavefront
-      const int half = lane >> 5, hl = lane & 31;
+      constexpr int HW = kBlk;              // lanes per row: a row has at most kBlk - 1 arcs inside
+      constexpr int RPP = 64 / HW;          // rows per wavefront pass
+      constexpr int RPW = kBlk / (RPP * NWS);  // passes per wavefront
+      const int half = lane / HW, hl = lane % HW;
       int32_t ad1[RPW], w1[RPW];
 #pragma unroll
       for (int r = 0; r < RPW; r++) {  // loads on a valid arc for every lane: no branch between them
-        const int j = (r * NWS + swave) * 2 + half;
+        const int j = (r * NWS + swave) * RPP + half;
         const bool ok = j < bsz && hl < min(rdeg[j], bsz - 1 - j);
         const int32_t a = ok ? rfirst[j] + hl : 0;
         ad1[r] = SPLIT ? (int32_t)adep16[a] : adep[a];
@@ -225,7 +241,7 @@ __global__ void __launch_bounds__(128 * NWS) k_query_blk(const VInfo* __restrict
       }
 #pragma unroll
       for (int r = 0; r < RPW; r++) {  // arcs leaving the block go to the unread pad word M[31][32]
-        const int j = (r * NWS + swave) * 2 + half;
+        const int j = (r * NWS + swave) * RPP + half;
         M[ad1[r] >= dtop ? (dlo - ad1[r]) * kMStride + j : kBlk * kMStride - 1] = w1[r];
       }
     }
@@ -266,7 +282,7 @@ __global__ void __launch_bounds__(128 * NWS) k_query_blk(const VInfo* __restrict
     // 3. arcs leaving the block upward, flattened over the finite rows
     if (act) {
       const int32_t tot = rend[kBlk - 1];
-      const int32_t e_l = rend[lane & (kBlk - 1)];  // lane r < 32 holds the end of row r
+      const int32_t e_l = rend[lane & (kBlk - 1)];  // lane r < kBlk holds the end of row r
       for (int32_t c0 = swave * 64; c0 < tot; c0 += SL * U) {
         // the row of each lane's arc in every chunk: rows ending at or before
         // the chunk start precede it, rows ending inside it split it
@@ -355,14 +371,25 @@ void launch_query_blk(const DevTopo& d, int nws, int unroll, const int2* ud, i64
                       const int32_t* dn, const int32_t* s_rank, const int32_t* t_rank, i64 nq,
                       const int32_t* order, int32_t* dist, int32_t* scanned, hipStream_t s) {
   const int32_t maxlen = d.depth + 1;
-  const size_t lds = query_blk_lds_bytes(maxlen);
+  // launches of a few queries (8 wavefronts per side) walk the chains in
+  // blocks of 64 ancestors when the LDS allows: half the blocks, each a fixed
+  // chain of barriers and round trips
+  const bool blk64 = nws >= 8 && d.query_blk64 && blk_lds_words(maxlen, 64) * sizeof(int32_t) <= 64 * 1024;
+  const size_t lds = blk_lds_words(maxlen, blk64 ? 64 : kBlkMin) * sizeof(int32_t);
   const dim3 grid((unsigned)nq);
   const bool split = up != nullptr;
   if (split && !d.adep16) throw std::runtime_error("split-metric query: hierarchy too deep for 16-bit depths");
-#define NWCCH_BLK_LAUNCH(NW, U, SP)                                                                              \
-  hipLaunchKernelGGL((k_query_blk<NW, U, SP>), grid, dim3(128 * NW), lds, s, d.vinfo, d.euler, d.adep, d.adep16, \
-                     d.anc, up, dn, ud, ud_stride, ud_table, qmetric, n_metrics, s_rank, t_rank, nq, order,         \
+#define NWCCH_BLK_LAUNCH2(NW, U, SP, B)                                                                               \
+  hipLaunchKernelGGL((k_query_blk<NW, U, SP, B>), grid, dim3(128 * NW), lds, s, d.vinfo, d.euler, d.adep, d.adep16, \
+                     d.anc, up, dn, ud, ud_stride, ud_table, qmetric, n_metrics, s_rank, t_rank, nq, order,            \
                      (int32_t)d.n, maxlen, dist, scanned, d.status)
+#define NWCCH_BLK_LAUNCH(NW, U, SP)                 \
+  do {                                              \
+    if (NW == 8 && blk64)                           \
+      NWCCH_BLK_LAUNCH2(NW, U, SP, (NW == 8 ? 64 : 32)); \
+    else                                            \
+      NWCCH_BLK_LAUNCH2(NW, U, SP, 32);             \
+  } while (0)
 #define NWCCH_BLK_UNROLL(NW)                                                   \
   if (split) {                                                                 \
     if (unroll == 16) NWCCH_BLK_LAUNCH(NW, 16, true);                          \
@@ -382,6 +409,7 @@ void launch_query_blk(const DevTopo& d, int nws, int unroll, const int2* ud, i64
   }
 #undef NWCCH_BLK_UNROLL
 #undef NWCCH_BLK_LAUNCH
+#undef NWCCH_BLK_LAUNCH2
 }
 
 }  // namespace nwcch

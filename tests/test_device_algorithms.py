@@ -246,7 +246,7 @@ def test_blocked_query_equals_chain_sweep():
         qs = rng.integers(0, g.n, 60)
         qt = rng.integers(0, g.n, 60)
         d0, s0, _ = st.query_ranks(up, dn, rank[qs], rank[qt])
-        for B in (32, 4):
+        for B in (64, 32, 4):
             for i in range(len(qs)):
                 d, s = _blocked_query(st.first, st.head, st.parent, st.vdepth, up, dn,
                                       int(rank[qs[i]]), int(rank[qt[i]]), B)
