@@ -369,8 +369,8 @@ int nwcch_status(nwcch_dev* h, uint32_t* out, int clear, void* stream);
  *                   default; 8..40 rows, 262,144 triangles, pieces of 16 arcs
  *                   by default).  Every form gives identical values.
  *   "query_blk", "query_blk_small", "query_blk_waves", "query_blk_unroll"
- *                   the blocked query kernel (blocks of 32 ancestors, a 32 x 32
- *                   (min, +) matrix in LDS): 1 = used for hierarchies deeper
+ *                   the blocked query kernel (blocks of 32 or 64 ancestors, a
+ *                   block-sized (min, +) matrix in LDS): 1 = used for hierarchies deeper
  *                   than 1,024 and for launches of at most query_blk_small
  *                   queries (default 4,096), 0 = never; wavefronts per search
  *                   side 1, 2, 4 or 8 (0 = by launch size, default); 64-arc
