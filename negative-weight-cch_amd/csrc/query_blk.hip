@@ -223,7 +223,8 @@ __global__ void __launch_bounds__(128 * NWS) k_query_blk(const VInfo* __restrict
     for (int i = stid; i < kBlk * kMStride; i += SL) M[i] = kInf;
     lds_barrier();
     // 1. arcs inside the block: among the first bsz - 1 - j arcs of row j; a
-    //    wavefront takes two rows per pass (one per half), all loads first
+    //    wavefront takes 64 / kBlk rows per pass (a row per kBlk lanes), all
+    //    loads first
     if (act) {
       constexpr int HW = kBlk;              // lanes per row: a row has at most kBlk - 1 arcs inside
       constexpr int RPP = 64 / HW;          // rows per wavefront pass
@@ -240,7 +241,7 @@ __global__ void __launch_bounds__(128 * NWS) k_query_blk(const VInfo* __restrict
         if (!ok) ad1[r] = -1;
       }
 #pragma unroll
-      for (int r = 0; r < RPW; r++) {  // arcs leaving the block go to the unread pad word M[31][32]
+      for (int r = 0; r < RPW; r++) {  // arcs leaving the block go to the unread pad word M[kBlk-1][kBlk]
         const int j = (r * NWS + swave) * RPP + half;
         M[ad1[r] >= dtop ? (dlo - ad1[r]) * kMStride + j : kBlk * kMStride - 1] = w1[r];
       }
